@@ -1,0 +1,252 @@
+/*
+ * prestige_hip.h — C ABI of libprestige_hip.so, the MI355X (gfx950) particle
+ * hot path: cell-linked-list neighbour search, the per-particle WCSPH / DEM
+ * gather loops, the FP64 symplectic-Euler integrator and their adjoint.
+ *
+ * Reference interfaces this boundary replaces (paths relative to the
+ * reference tree):
+ *   - particle data: one slice per named field, indexed by particle, `i` the
+ *     destination and `j` the source of every pairwise body
+ *     (prestige/src/lib.rs:8);
+ *   - the equation record {name, reads, writes, body}
+ *     (prestige/src/equations/ir.rs:3-14) produced by #[equation]
+ *     (prestige_macros/src/lib.rs:8-58);
+ *   - fuse(Vec<EquationIR>) -> FusedEquations (prestige/src/equations/fuse.rs:14-40);
+ *   - generate_simple_cpu(&FusedEquations) -> String, the i-outer / j-inner
+ *     all-pairs loop (prestige/src/codegen/simple_cpu.rs:3-22), which this
+ *     library executes instead of printing, with `for j in 0..n` replaced by
+ *     the 27-cell neighbour iterator.
+ * The reference has no C ABI and no runtime; every entry point below names the
+ * reference item whose behaviour it carries.
+ *
+ * Conventions: every function returns prestige_status (0 = OK) unless stated;
+ * no C++ exception crosses the ABI; host pointers are borrowed for the call
+ * only; one context is externally synchronised (the `&mut` of the reference),
+ * separate contexts may be used from separate threads.  All per-particle host
+ * arrays are in ORIGINAL particle-id order (the order of the first upload),
+ * whatever order the device currently keeps them in.
+ */
+#ifndef PRESTIGE_HIP_H
+#define PRESTIGE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PRESTIGE_ABI_VERSION 1u
+
+typedef enum prestige_status {
+  PRESTIGE_OK = 0,
+  PRESTIGE_E_INVALID = 1,     /* bad argument, unknown field name, wrong length */
+  PRESTIGE_E_HIP = 2,         /* HIP runtime error (message in prestige_last_error) */
+  PRESTIGE_E_OOM = 3,         /* device allocation failed */
+  PRESTIGE_E_CAPACITY = 4,    /* more than max_contacts DEM partners on one particle */
+  PRESTIGE_E_NONFINITE = 5,   /* NaN or Inf found in a state field */
+  PRESTIGE_E_DOMAIN = 6,      /* particle outside the cell grid */
+  PRESTIGE_E_UNSUPPORTED = 7, /* valid call that this model/precision does not provide */
+  PRESTIGE_E_STATE = 8        /* call out of order (e.g. adjoint before a taped forward) */
+} prestige_status;
+
+typedef enum prestige_model {
+  PRESTIGE_MODEL_PAIRS = 0,   /* neighbour search + equation probes only */
+  PRESTIGE_MODEL_WCSPH = 1,   /* weakly compressible SPH (density, Tait EOS, momentum) */
+  PRESTIGE_MODEL_DEM = 2      /* spring-dashpot + Coulomb spheres */
+} prestige_model;
+
+typedef enum prestige_precision {
+  PRESTIGE_F64 = 0,           /* everything FP64 */
+  PRESTIGE_F32 = 1,           /* everything FP32, positions absolute */
+  PRESTIGE_MIXED = 2          /* FP32 gathers on FP32 offsets from FP64 cell origins,
+                                 Kahan-compensated sums, FP64 integrator */
+} prestige_precision;
+
+typedef enum prestige_loop {
+  PRESTIGE_LOOP_ALLPAIRS = 0, /* for j in 0..n (prestige/src/codegen/simple_cpu.rs:8) */
+  PRESTIGE_LOOP_CELLS = 1     /* j over the 27 cells around i's cell, |x_i-x_j| < cell_size */
+} prestige_loop;
+
+/* Built-in equations (prestige/src/lib.rs:7-12 holds the reference's one, eq1). */
+typedef enum prestige_equation_id {
+  PRESTIGE_EQ1 = 0,           /* force[i] += mass[j]  (prestige/src/lib.rs:10) */
+  PRESTIGE_EQ_SPH_DENSITY = 1,
+  PRESTIGE_EQ_SPH_EOS = 2,
+  PRESTIGE_EQ_SPH_MOMENTUM = 3,
+  PRESTIGE_EQ_INTEGRATE = 4,
+  PRESTIGE_EQ_DEM_CONTACT = 5,
+  PRESTIGE_EQ_COUNT = 6
+} prestige_equation_id;
+
+/* Role bits of a field inside one equation body. */
+#define PRESTIGE_ROLE_I 1u    /* indexed by the destination particle i */
+#define PRESTIGE_ROLE_J 2u    /* indexed by the source particle j */
+
+typedef struct prestige_desc {
+  uint32_t abi_version;       /* must be PRESTIGE_ABI_VERSION */
+  int32_t model;              /* prestige_model */
+  int32_t precision;          /* prestige_precision */
+  uint32_t max_contacts;      /* DEM history slots per particle (<= 32) */
+  uint64_t n_particles;       /* total particles */
+  uint64_t n_moving;          /* ids [0, n_moving) are integrated; [n_moving, n) stay fixed */
+  double grid_lo[3];          /* cell-grid origin; particles must stay in [lo, hi) */
+  double grid_hi[3];
+  double cell_size;           /* >= interaction cutoff (2h for WCSPH, 2 r_max for DEM) */
+  double gravity[3];
+  double dt;
+  /* WCSPH */
+  double h;                   /* smoothing length; support 2h, cubic spline */
+  double rho0;
+  double c0;
+  double alpha;               /* artificial viscosity */
+  double mass;                /* particle mass (uniform) */
+  /* DEM */
+  double kn;                  /* normal stiffness, N/m */
+  double kt;                  /* tangential stiffness, N/m */
+  double restitution;         /* normal restitution e in (0, 1] */
+  double mu;                  /* Coulomb friction */
+  double wall_lo[3];          /* half-space walls x>=lo, x<=hi per axis */
+  double wall_hi[3];
+  uint32_t wall_mask;         /* bit 2a: wall at lo of axis a; bit 2a+1: wall at hi */
+  uint32_t flags;             /* reserved, 0 */
+} prestige_desc;
+
+/* Opaque context: owns every device buffer and one HIP stream. */
+typedef struct prestige_ctx prestige_ctx;
+
+/* Fill *d with defaults for `model` (abi_version, gravity -9.81 z, gamma 7, ...). */
+void prestige_desc_init(prestige_desc* d, int32_t model);
+
+/* Create / destroy a context on the current HIP device. */
+prestige_status prestige_create(const prestige_desc* desc, prestige_ctx** out);
+void prestige_destroy(prestige_ctx* ctx);
+
+/* Copy one SoA field in id order.  Names follow the reference's slice naming
+ * (prestige/src/lib.rs:8): "x","y","z" positions; "vx","vy","vz" velocities;
+ * "wx","wy","wz" angular velocities (DEM); "radius", "mass" (DEM, per
+ * particle); "rho","p" (WCSPH, after a step); "ax","ay","az" accelerations of
+ * the last step; "force" (eq1 output, PAIRS/any model); "key" cell key of the
+ * current sort (download only, as double).  n must equal n_particles.
+ * Download synchronises the context's stream. */
+prestige_status prestige_field_upload(prestige_ctx* ctx, const char* name,
+                                      const double* data, uint64_t n);
+prestige_status prestige_field_download(prestige_ctx* ctx, const char* name,
+                                        double* out, uint64_t n);
+
+/* Cell keys of the current positions, stable LSD radix sort, reorder of every
+ * field into key order, cell lower-bound table.  The neighbour relation the
+ * gathers use afterwards is |x_i - x_j| < cutoff over the 27 surrounding
+ * cells. */
+prestige_status prestige_neighbors_build(prestige_ctx* ctx);
+
+/* Parity view of the last build: sorted keys (n), perm (n; sorted slot k holds
+ * the particle that was in slot perm[k] of the previous order), ids (n; the
+ * original id in slot k), cell_lb (n_cells + 1; first sorted slot with key >=
+ * c).  Any pointer may be NULL. */
+prestige_status prestige_neighbors_download(prestige_ctx* ctx, uint32_t* keys,
+                                            uint32_t* perm, uint32_t* ids,
+                                            uint32_t* cell_lb);
+/* n_cells = nx*ny*nz of the grid. */
+prestige_status prestige_grid_dims(prestige_ctx* ctx, uint32_t dims[3]);
+
+/* Run one built-in equation over the current build (prestige/src/codegen/simple_cpu.rs:7-19
+ * semantics: i outer, j inner, j == i included, gather into [i]).  Only
+ * PRESTIGE_EQ1 is evaluated standalone: force[i] = force[i] + sum_j mass[j],
+ * with j over all n (ALLPAIRS) or over neighbours within `cutoff` (CELLS;
+ * cutoff <= cell_size).  "force" and "mass" must have been uploaded. */
+prestige_status prestige_eval_equation(prestige_ctx* ctx, int32_t eq_id,
+                                       int32_t loop, double cutoff);
+
+/* Advance nsteps full steps (neighbour rebuild, gathers, integrator) on the
+ * device; one host synchronisation at the end to collect error flags. */
+prestige_status prestige_step(prestige_ctx* ctx, uint32_t nsteps);
+
+/* Reverse mode.  prestige_forward_tape runs nsteps like prestige_step from
+ * the current state and keeps FP64 checkpoints of the state every
+ * `checkpoint_every` steps.  prestige_adjoint then takes dL/dx and dL/dv of
+ * the final state (3*n each, id order, x-block then y then z; NULL = 0) and
+ * returns dL/dparams and dL/dv0 (3*n, id order, NULL to skip), recomputing
+ * each checkpoint segment forward with its neighbour lists frozen.
+ * params: WCSPH {alpha, c0}; DEM {kn, restitution, mu}. */
+prestige_status prestige_forward_tape(prestige_ctx* ctx, uint32_t nsteps,
+                                      uint32_t checkpoint_every);
+prestige_status prestige_adjoint(prestige_ctx* ctx, const double* seed_x,
+                                 const double* seed_v, double* grad_params,
+                                 uint32_t n_params, double* grad_v0);
+
+/* Per-kernel HIP-event timing on the context's stream.  enable=1 starts
+ * recording (and clears the totals). */
+prestige_status prestige_timing_enable(prestige_ctx* ctx, int32_t enable);
+/* Number of distinct kernels timed so far. */
+uint32_t prestige_timing_count(prestige_ctx* ctx);
+/* Kernel idx: name (static storage), summed milliseconds, launches, and the
+ * algorithmic bytes of one launch (every array element the kernel must read
+ * or write counted once). */
+prestige_status prestige_timing_get(prestige_ctx* ctx, uint32_t idx,
+                                    const char** name, double* total_ms,
+                                    uint64_t* launches, double* bytes_per_launch);
+
+/* Block until all queued work on the context's stream is done. */
+prestige_status prestige_sync(prestige_ctx* ctx);
+
+/* Last error message of this context ("" if none; static storage valid until
+ * the next call on ctx).  ctx may be NULL for creation errors. */
+const char* prestige_last_error(const prestige_ctx* ctx);
+
+/* ---- equation interface (prestige/src/equations/ir.rs:3-14, fuse.rs:14-40,
+ *      codegen/simple_cpu.rs:3-22) -------------------------------------- */
+
+/* Describe built-in equation eq_id: name, read and write field names
+ * (comma-separated, sorted; every indexed name is a read, LHS included, as in
+ * prestige_macros/src/lib.rs:71-133), the i/j role of every read (one
+ * PRESTIGE_ROLE_* mask per read, same order) and the Rust body text.  Buffers
+ * are NUL-terminated and truncated to their capacity. */
+prestige_status prestige_equation_describe(int32_t eq_id, char* name, size_t name_cap,
+                                           char* reads, size_t reads_cap,
+                                           char* writes, size_t writes_cap,
+                                           uint8_t* read_roles, size_t roles_cap,
+                                           char* body, size_t body_cap);
+
+/* Fuse equations eq_ids[0..n) in order (fuse.rs:20-31) and split them into
+ * gather passes wherever a body reads at [j] a field an earlier body of the
+ * same pass writes.  pass_of[k] receives the pass index of eq_ids[k];
+ * *n_passes the count. */
+prestige_status prestige_fuse_schedule(const int32_t* eq_ids, uint32_t n,
+                                       uint32_t* pass_of, uint32_t* n_passes);
+
+/* Text of the reference's CPU loop for the fused bodies
+ * (prestige/src/codegen/simple_cpu.rs:5-21).  *len receives the full length
+ * (without NUL); the text is truncated to cap-1 bytes. */
+prestige_status prestige_generate_simple_cpu(const int32_t* eq_ids, uint32_t n,
+                                             char* out, size_t cap, size_t* len);
+
+/* ---- scenes (host only; no device needed) -------------------------------
+ * Deterministic particle layouts of the benchmark configurations, drawn with
+ * SplitMix64.  Arrays are caller-owned, sized from prestige_scene_count. */
+typedef enum prestige_scene_id {
+  PRESTIGE_SCENE_DEM_LATTICE = 0,   /* 10,000 spheres r=0.01 jittered lattice in a 0.5 m box */
+  PRESTIGE_SCENE_DAM_BREAK = 1,     /* fluid cube + 3-layer tank */
+  PRESTIGE_SCENE_DEM_POUR = 2       /* polydisperse spheres above a 0.3 m bin */
+} prestige_scene_id;
+
+typedef struct prestige_scene_params {
+  double dx;                  /* DAM_BREAK lattice spacing */
+  uint64_t n_target;          /* particles requested (fluid for DAM_BREAK) */
+  uint64_t seed;
+} prestige_scene_params;
+
+/* n_total and n_moving for a scene. */
+prestige_status prestige_scene_count(int32_t scene, const prestige_scene_params* sp,
+                                     uint64_t* n_total, uint64_t* n_moving);
+/* Fill a descriptor (grid, physics constants) and the particle arrays
+ * (x,y,z,vx,vy,vz each n_total; radius and mass may be NULL for WCSPH). */
+prestige_status prestige_scene_build(int32_t scene, const prestige_scene_params* sp,
+                                     prestige_desc* desc, double* x, double* y,
+                                     double* z, double* vx, double* vy, double* vz,
+                                     double* radius, double* mass);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PRESTIGE_HIP_H */

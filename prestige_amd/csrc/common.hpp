@@ -1,0 +1,213 @@
+// common.hpp — internal types of libprestige_hip.so (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/prestige_hip.h"
+
+namespace prestige {
+
+// Error carrying a prestige_status across internal C++ code; converted to a
+// status code at the C ABI.
+struct Error : std::runtime_error {
+  prestige_status code;
+  Error(prestige_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define PR_HIP(call)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      throw ::prestige::Error(e_ == hipErrorOutOfMemory ? PRESTIGE_E_OOM : PRESTIGE_E_HIP, \
+                              std::string(#call) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+#define PR_CHECK(cond, code, msg)                                   \
+  do {                                                              \
+    if (!(cond)) throw ::prestige::Error((code), (msg));            \
+  } while (0)
+
+// Device flags written by kernels on error paths only.
+enum FlagSlot : int { FLAG_DOMAIN = 0, FLAG_CAPACITY = 1, FLAG_NONFINITE = 2, FLAG_COUNT = 4 };
+
+// Cell grid: key = (cz*ny + cy)*nx + cx, c = floor((x - lo) * inv_cs).
+struct Grid {
+  double lo[3];
+  double cs;
+  double inv_cs;
+  double off_scale;   // MIXED offsets are (x - origin) * off_scale (1/h for WCSPH, 1 for DEM)
+  int32_t dims[3];
+  uint32_t n_cells;
+};
+
+// Constants of the WCSPH bodies, derived once on the host from the
+// descriptor.  Every device form reads the same FP64 values (FP32 forms
+// round them once).
+struct SphConst {
+  double h, inv_h, H2, eps2, inv_rho0, B, alpha_c0, msig, msh, dt;
+  double g[3];
+};
+
+struct DemConst {
+  double kn, kt, mu, two_zeta, neg_inv_kt, dt;
+  double g[3];
+  double wall_lo[3], wall_hi[3];
+  uint32_t wall_mask;
+  uint32_t max_contacts;
+};
+
+// Owned device allocation.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t b) {
+    if (b == bytes && p) return;
+    release();
+    if (b == 0) return;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Error(PRESTIGE_E_OOM, "hipMalloc(" + std::to_string(b) + " bytes) failed: " + hipGetErrorString(e));
+    }
+    bytes = b;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Per-kernel HIP-event timing on the context stream.
+struct KernelStat {
+  const char* name;
+  double total_ms = 0.0;
+  uint64_t launches = 0;
+  double bytes_per_launch = 0.0;
+};
+
+struct Timing {
+  bool enabled = false;
+  std::vector<KernelStat> stats;
+  struct Pending { hipEvent_t a, b; size_t stat; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t get_event();
+  size_t stat_index(const char* name, double bytes);
+  void flush();   // collect elapsed times of completed pairs (synchronises on each)
+  void clear();
+  ~Timing();
+};
+
+struct Ctx;
+
+// Records `name` around one launch when timing is on.
+struct LaunchScope {
+  Ctx* ctx;
+  size_t stat;
+  hipEvent_t a = nullptr, b = nullptr;
+  LaunchScope(Ctx* c, const char* name, double bytes);
+  ~LaunchScope() noexcept(false);
+};
+
+// Adjoint tape: FP64 checkpoints of the state every K steps.
+struct Tape {
+  bool valid = false;
+  uint32_t nsteps = 0, every = 0;
+  std::vector<DevBuf*> ckpt;   // one buffer per checkpoint (state snapshot, id order)
+  ~Tape() { for (auto* b : ckpt) delete b; }
+  void clear() { for (auto* b : ckpt) delete b; ckpt.clear(); valid = false; }
+};
+
+struct Ctx {
+  prestige_desc desc{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t n = 0, n_moving = 0;
+  Grid grid{};
+  SphConst sph{};
+  DemConst dem{};
+  std::string err;
+
+  // slot bookkeeping: cur = buffer (0/1) holding the newest positions and
+  // velocities; `sorted` = cur is in key order of the last build;
+  // `keys_valid` = keys[cur_key] hold the cell keys of buffer cur.
+  int cur = 0;
+  bool sorted = false;
+  bool keys_valid = false;
+  bool built = false;
+  int cur_key = 0;
+
+  // state, ping-pong pairs (element type depends on precision)
+  DevBuf pos[2], vel[2];         // double4 (F64, MIXED) or float4 (F32); DEM: pos.w = radius, vel.w = mass
+  DevBuf omega[2];               // DEM: double4 / float4
+  DevBuf ids[2];                 // uint32 original id per slot
+  int cur_ids = 0;
+  DevBuf keys[2], vals[2];       // radix-sort ping-pong
+  int perm_buf = 0;              // which vals[] holds the last permutation
+  DevBuf sort_hist, sort_tot;    // per-tile digit counts, per-digit totals
+  DevBuf lb;                     // n_cells + 1
+  DevBuf flags;                  // FLAG_COUNT uint32
+  // gather outputs
+  DevBuf rp;                     // (rho, p/rho^2) per slot: double2 or float2
+  DevBuf pres;                   // p per slot (double or float)
+  DevBuf acc;                    // a per slot (double4 / float4)
+  // MIXED form gather copies
+  DevBuf off32, vel32, om32;     // float4: offset from cell origin (cell x in .w); velocity
+                                 // (mass in .w); DEM angular velocity (radius in .w)
+  // eq1 probe
+  DevBuf force, mass, scratch;
+  bool have_force = false, have_mass = false;
+  // DEM history (indexed by original id, double-buffered)
+  DevBuf hcnt[2], hid[2], hxi[2], wxi[2];
+  int cur_hist = 0;
+  Timing timing;
+  Tape tape;
+  bool keep_acc = true;
+
+  void check_flags();            // syncs; throws on a raised flag
+};
+
+// Launch helpers -------------------------------------------------------------
+constexpr int kBlock = 256;
+
+inline uint32_t blocks_for(uint64_t n, int block = kBlock) {
+  return (uint32_t)((n + (uint64_t)block - 1) / (uint64_t)block);
+}
+
+// Gather kernels run many blocks over spatially sorted slots; the hardware
+// deals workgroup b to XCD b % 8, so consecutive tiles would land on eight
+// different L2s.  Launch a multiple of 8 blocks and map physical block b to
+// logical tile (b % 8) * per + b / 8: each XCD then walks one contiguous
+// range of tiles and its L2 keeps the neighbour cells those tiles share.
+inline uint32_t xcd_grid(uint32_t nb) { return ((nb + 7u) / 8u) * 8u; }
+
+__device__ inline uint32_t xcd_tile(uint32_t b, uint32_t grid) {
+  uint32_t per = grid / 8u;
+  return (b % 8u) * per + b / 8u;
+}
+
+// Per-form kernel entry points (defined in the .hip files).
+void neighbors_build(Ctx& c);
+void keys_from_positions(Ctx& c);
+void eq1_eval(Ctx& c, int loop, double cutoff);
+void sph_step_f64(Ctx& c);
+void sph_step_f32(Ctx& c);
+void sph_step_mixed(Ctx& c);
+void dem_step_f64(Ctx& c);
+void dem_step_mixed(Ctx& c);
+void scatter_to_ids(Ctx& c, const void* src, int src_kind, int comp, double* d_out);
+void gather_from_ids(Ctx& c, const double* d_in, void* dst, int dst_kind, int comp);
+
+}  // namespace prestige

@@ -1,0 +1,531 @@
+// neighbors.hip — cell keys, stable LSD radix sort, reorder into key order,
+// cell lower-bound table, id-order scatter/gather, and the eq1 probe.
+//
+// The sorted layout is what every gather kernel iterates: for a particle in
+// cell (cx, cy, cz) the three cells cx-1..cx+1 of each of the nine (dy, dz)
+// rows are one contiguous run [lb[row + cx-1], lb[row + cx+2]) of slots.
+#include "common.hpp"
+
+namespace prestige {
+
+// ---------------------------------------------------------------------------
+// cell keys
+// ---------------------------------------------------------------------------
+
+// floor((x - lo) * inv_cs) per axis in FP64, exactly as the host oracle does;
+// out-of-grid particles raise FLAG_DOMAIN and get key 0 so the sort stays in
+// bounds.
+__device__ inline uint32_t cell_key_of(double x, double y, double z, const Grid& g,
+                                       uint32_t* flags) {
+  double fx = floor((x - g.lo[0]) * g.inv_cs);
+  double fy = floor((y - g.lo[1]) * g.inv_cs);
+  double fz = floor((z - g.lo[2]) * g.inv_cs);
+  bool ok = fx >= 0.0 && fy >= 0.0 && fz >= 0.0 && fx < (double)g.dims[0] &&
+            fy < (double)g.dims[1] && fz < (double)g.dims[2];
+  if (!ok) {
+    atomicOr(&flags[FLAG_DOMAIN], 1u);
+    return 0u;
+  }
+  uint32_t cx = (uint32_t)fx, cy = (uint32_t)fy, cz = (uint32_t)fz;
+  return (cz * (uint32_t)g.dims[1] + cy) * (uint32_t)g.dims[0] + cx;
+}
+
+template <class T4>
+__global__ __launch_bounds__(kBlock) void k_cell_keys(uint32_t n, const T4* __restrict__ pos,
+                                                      Grid g, uint32_t* __restrict__ keys,
+                                                      uint32_t* flags) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T4 p = pos[i];
+  keys[i] = cell_key_of((double)p.x, (double)p.y, (double)p.z, g, flags);
+}
+
+void keys_from_positions(Ctx& c) {
+  if (c.n == 0) { c.keys_valid = true; return; }
+  uint32_t n = (uint32_t)c.n;
+  LaunchScope ls(&c, "cell_keys", 32.0 * n);
+  if (c.desc.precision == PRESTIGE_F32)
+    k_cell_keys<float4><<<blocks_for(n), kBlock, 0, c.stream>>>(
+        n, c.pos[c.cur].as<float4>(), c.grid, c.keys[c.cur_key].as<uint32_t>(), c.flags.as<uint32_t>());
+  else
+    k_cell_keys<double4><<<blocks_for(n), kBlock, 0, c.stream>>>(
+        n, c.pos[c.cur].as<double4>(), c.grid, c.keys[c.cur_key].as<uint32_t>(), c.flags.as<uint32_t>());
+  PR_HIP(hipGetLastError());
+  c.keys_valid = true;
+}
+
+// ---------------------------------------------------------------------------
+// stable LSD radix sort of (key, slot) pairs, 8-bit digits
+// ---------------------------------------------------------------------------
+constexpr int kRadixBits = 8;
+constexpr int kBuckets = 1 << kRadixBits;
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kTile = kSortThreads * kSortItems;
+static_assert(kSortThreads == kBuckets, "one thread per bucket in the scan phases");
+
+// Lanes of this wavefront whose digit equals mine (and whose item is valid):
+// one ballot per digit bit, no shared memory.
+__device__ inline uint64_t match_digit(uint32_t d, bool valid) {
+  uint64_t m = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < kRadixBits; ++b) {
+    uint64_t bal = __ballot((d >> b) & 1u);
+    m &= ((d >> b) & 1u) ? bal : ~bal;
+  }
+  return m;
+}
+
+__device__ inline uint64_t lanes_below() {
+  return (1ull << __lane_id()) - 1ull;
+}
+
+// Per-tile digit histogram: hist[d * n_tiles + tile].
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t* __restrict__ keys,
+                                                            uint32_t n, int shift,
+                                                            uint32_t n_tiles,
+                                                            uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[kBuckets];
+  const uint32_t t = threadIdx.x;
+  h[t] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * (uint32_t)kTile;
+  for (int it = 0; it < kSortItems; ++it) {
+    uint32_t idx = base + (uint32_t)it * kSortThreads + t;
+    bool valid = idx < n;
+    uint32_t d = valid ? (keys[idx] >> shift) & (kBuckets - 1) : 0u;
+    uint64_t peers = match_digit(d, valid);
+    if (valid && (peers & lanes_below()) == 0ull) atomicAdd(&h[d], (uint32_t)__popcll(peers));
+  }
+  __syncthreads();
+  hist[t * n_tiles + blockIdx.x] = h[t];
+}
+
+// Inclusive scan within a wavefront of 64 lanes.
+__device__ inline uint32_t wave_incl_scan(uint32_t x) {
+  const int lane = (int)__lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, (unsigned)o, 64);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+// Exclusive scan over a 256-thread block; *total receives the sum.
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* s4, uint32_t* total) {
+  const uint32_t w = threadIdx.x >> 6;
+  uint32_t x = wave_incl_scan(v);
+  if (__lane_id() == 63) s4[w] = x;
+  __syncthreads();
+  uint32_t base = 0;
+  for (uint32_t k = 0; k < w; ++k) base += s4[k];
+  *total = s4[0] + s4[1] + s4[2] + s4[3];
+  __syncthreads();
+  return base + x - v;
+}
+
+// One block per digit: exclusive scan of hist[d][*] across tiles, tot[d] = count.
+__global__ __launch_bounds__(kSortThreads) void k_sort_scan(uint32_t* __restrict__ hist,
+                                                            uint32_t n_tiles,
+                                                            uint32_t* __restrict__ tot) {
+  __shared__ uint32_t s4[4];
+  uint32_t* row = hist + (size_t)blockIdx.x * n_tiles;
+  uint32_t carry = 0;
+  for (uint32_t b = 0; b < n_tiles; b += kSortThreads) {
+    uint32_t idx = b + threadIdx.x;
+    uint32_t v = idx < n_tiles ? row[idx] : 0u;
+    uint32_t total;
+    uint32_t ex = block_excl_scan(v, s4, &total);
+    if (idx < n_tiles) row[idx] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) tot[blockIdx.x] = carry;
+}
+
+// Stable scatter of one tile.  Items are taken in rounds of 256 consecutive
+// slots; inside a round the rank of an item among equal digits is its
+// wavefront rank (ballot match) plus the counts of earlier wavefronts, and
+// rounds advance a per-digit running offset, so equal digits keep their
+// input order.
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
+    uint32_t* __restrict__ vout, uint32_t n, int shift, uint32_t n_tiles,
+    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ tot) {
+  __shared__ uint32_t s_run[kBuckets];
+  __shared__ uint32_t s_wave[4][kBuckets];
+  __shared__ uint32_t s4[4];
+  const uint32_t t = threadIdx.x;
+  const uint32_t w = t >> 6;
+  {
+    uint32_t total;
+    uint32_t digit_base = block_excl_scan(tot[t], s4, &total);
+    s_run[t] = digit_base + hist[t * n_tiles + blockIdx.x];
+  }
+  const uint32_t base = blockIdx.x * (uint32_t)kTile;
+  for (int it = 0; it < kSortItems; ++it) {
+    uint32_t idx = base + (uint32_t)it * kSortThreads + t;
+    bool valid = idx < n;
+    uint32_t k = valid ? kin[idx] : 0u;
+    uint32_t v = valid ? (vin ? vin[idx] : idx) : 0u;
+    uint32_t d = (k >> shift) & (kBuckets - 1);
+    s_wave[0][t] = 0; s_wave[1][t] = 0; s_wave[2][t] = 0; s_wave[3][t] = 0;
+    __syncthreads();
+    uint64_t peers = match_digit(d, valid);
+    uint32_t rank = (uint32_t)__popcll(peers & lanes_below());
+    if (valid && rank == 0) s_wave[w][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    {
+      uint32_t c0 = s_wave[0][t], c1 = s_wave[1][t], c2 = s_wave[2][t], c3 = s_wave[3][t];
+      uint32_t r = s_run[t];
+      s_wave[0][t] = r;
+      s_wave[1][t] = r + c0;
+      s_wave[2][t] = r + c0 + c1;
+      s_wave[3][t] = r + c0 + c1 + c2;
+      s_run[t] = r + c0 + c1 + c2 + c3;
+    }
+    __syncthreads();
+    if (valid) {
+      uint32_t dst = s_wave[w][d] + rank;
+      kout[dst] = k;
+      vout[dst] = v;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_iota(uint32_t n, uint32_t* v) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = i;
+}
+
+static int key_bits(uint32_t n_cells) {
+  int b = 0;
+  while (b < 32 && ((uint64_t)1 << b) < (uint64_t)n_cells) ++b;
+  return b;
+}
+
+// Sorts keys[cur_key]; afterwards keys[cur_key] holds sorted keys and
+// vals[perm_buf] the permutation (sorted slot k <- old slot perm[k]).
+static void radix_sort(Ctx& c) {
+  const uint32_t n = (uint32_t)c.n;
+  const int bits = key_bits(c.grid.n_cells);
+  const int passes = (bits + kRadixBits - 1) / kRadixBits;
+  const uint32_t n_tiles = (n + kTile - 1) / kTile;
+  if (passes == 0) {
+    k_iota<<<blocks_for(n), kBlock, 0, c.stream>>>(n, c.vals[0].as<uint32_t>());
+    PR_HIP(hipGetLastError());
+    c.perm_buf = 0;
+    return;
+  }
+  int src = c.cur_key;
+  int vsrc = -1;   // first pass generates slot indices itself
+  for (int p = 0; p < passes; ++p) {
+    const int shift = p * kRadixBits;
+    const int dst = src ^ 1;
+    const int vdst = (vsrc < 0) ? 0 : (vsrc ^ 1);
+    {
+      LaunchScope ls(&c, "sort_hist", 4.0 * n);
+      k_sort_hist<<<n_tiles, kSortThreads, 0, c.stream>>>(c.keys[src].as<uint32_t>(), n, shift,
+                                                          n_tiles, c.sort_hist.as<uint32_t>());
+      PR_HIP(hipGetLastError());
+    }
+    {
+      LaunchScope ls(&c, "sort_scan", 8.0 * kBuckets * n_tiles);
+      k_sort_scan<<<kBuckets, kSortThreads, 0, c.stream>>>(c.sort_hist.as<uint32_t>(), n_tiles,
+                                                           c.sort_tot.as<uint32_t>());
+      PR_HIP(hipGetLastError());
+    }
+    {
+      LaunchScope ls(&c, "sort_scatter", (vsrc < 0 ? 12.0 : 16.0) * n);
+      k_sort_scatter<<<n_tiles, kSortThreads, 0, c.stream>>>(
+          c.keys[src].as<uint32_t>(), vsrc < 0 ? nullptr : c.vals[vsrc].as<uint32_t>(),
+          c.keys[dst].as<uint32_t>(), c.vals[vdst].as<uint32_t>(), n, shift, n_tiles,
+          c.sort_hist.as<uint32_t>(), c.sort_tot.as<uint32_t>());
+      PR_HIP(hipGetLastError());
+    }
+    src = dst;
+    vsrc = vdst;
+  }
+  c.cur_key = src;
+  c.perm_buf = vsrc;
+}
+
+// ---------------------------------------------------------------------------
+// reorder into key order (+ FP32 offset copies for the MIXED form)
+// ---------------------------------------------------------------------------
+
+template <class T4, bool OMEGA, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_reorder(
+    uint32_t n, const uint32_t* __restrict__ perm, const T4* __restrict__ pos_in,
+    const T4* __restrict__ vel_in, const T4* __restrict__ om_in,
+    const uint32_t* __restrict__ id_in, T4* __restrict__ pos_out, T4* __restrict__ vel_out,
+    T4* __restrict__ om_out, uint32_t* __restrict__ id_out, const uint32_t* __restrict__ skeys,
+    Grid g, float4* __restrict__ off32, float4* __restrict__ vel32, float4* __restrict__ om32) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  uint32_t s = perm[k];
+  T4 p = pos_in[s];
+  T4 v = vel_in[s];
+  pos_out[k] = p;
+  vel_out[k] = v;
+  id_out[k] = id_in[s];
+  if constexpr (OMEGA) {
+    T4 w = om_in[s];
+    om_out[k] = w;
+    if constexpr (MIXED) om32[k] = make_float4((float)w.x, (float)w.y, (float)w.z, (float)p.w);
+  }
+  if constexpr (MIXED) {
+    uint32_t key = skeys[k];
+    uint32_t cx = key % (uint32_t)g.dims[0];
+    uint32_t rest = key / (uint32_t)g.dims[0];
+    uint32_t cy = rest % (uint32_t)g.dims[1];
+    uint32_t cz = rest / (uint32_t)g.dims[1];
+    float ox = (float)(((double)p.x - (g.lo[0] + (double)cx * g.cs)) * g.off_scale);
+    float oy = (float)(((double)p.y - (g.lo[1] + (double)cy * g.cs)) * g.off_scale);
+    float oz = (float)(((double)p.z - (g.lo[2] + (double)cz * g.cs)) * g.off_scale);
+    off32[k] = make_float4(ox, oy, oz, __uint_as_float(cx));
+    vel32[k] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
+  }
+}
+
+// lb[c] = first slot with key >= c.  Slot k (k >= 1) fills the cells
+// (key[k-1], key[k]]; cells up to key[0] and after key[n-1] are filled by
+// k_cell_lb_ends.  Every cell is written exactly once; no clearing pass.
+__global__ __launch_bounds__(kBlock) void k_cell_lb(uint32_t n, const uint32_t* __restrict__ skeys,
+                                                    uint32_t* __restrict__ lb) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k == 0 || k >= n) return;
+  uint32_t a = skeys[k - 1], b = skeys[k];
+  for (uint32_t c = a + 1; c <= b; ++c) lb[c] = k;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cell_lb_ends(uint32_t n, uint32_t n_cells,
+                                                         const uint32_t* __restrict__ skeys,
+                                                         uint32_t* __restrict__ lb) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > n_cells) return;
+  if (n == 0) { lb[c] = 0; return; }
+  uint32_t first = skeys[0], last = skeys[n - 1];
+  if (c <= first) lb[c] = 0;
+  else if (c > last) lb[c] = n;
+}
+
+void neighbors_build(Ctx& c) {
+  const uint32_t n = (uint32_t)c.n;
+  if (!c.keys_valid) keys_from_positions(c);
+  if (n == 0) {
+    k_cell_lb_ends<<<blocks_for((uint64_t)c.grid.n_cells + 1), kBlock, 0, c.stream>>>(
+        0, c.grid.n_cells, nullptr, c.lb.as<uint32_t>());
+    PR_HIP(hipGetLastError());
+    c.sorted = true; c.built = true;
+    return;
+  }
+  radix_sort(c);
+  const uint32_t* perm = c.vals[c.perm_buf].as<uint32_t>();
+  const uint32_t* skeys = c.keys[c.cur_key].as<uint32_t>();
+  const int src = c.cur, dst = c.cur ^ 1;
+  const bool dem = c.desc.model == PRESTIGE_MODEL_DEM;
+  const bool mixed = c.desc.precision == PRESTIGE_MIXED;
+  {
+    double elem = c.desc.precision == PRESTIGE_F32 ? 16.0 : 32.0;
+    double bytes = 4.0 + 2.0 * (2.0 * elem + 4.0) + (dem ? 2.0 * elem : 0.0) + (mixed ? 4.0 + 32.0 : 0.0);
+    LaunchScope ls(&c, "reorder", bytes * n);
+    if (c.desc.precision == PRESTIGE_F32) {
+      if (dem)
+        k_reorder<float4, true, false><<<blocks_for(n), kBlock, 0, c.stream>>>(
+            n, perm, c.pos[src].as<float4>(), c.vel[src].as<float4>(), c.omega[src].as<float4>(),
+            c.ids[c.cur_ids].as<uint32_t>(), c.pos[dst].as<float4>(), c.vel[dst].as<float4>(),
+            c.omega[dst].as<float4>(), c.ids[c.cur_ids ^ 1].as<uint32_t>(), skeys, c.grid, nullptr, nullptr, nullptr);
+      else
+        k_reorder<float4, false, false><<<blocks_for(n), kBlock, 0, c.stream>>>(
+            n, perm, c.pos[src].as<float4>(), c.vel[src].as<float4>(), nullptr,
+            c.ids[c.cur_ids].as<uint32_t>(), c.pos[dst].as<float4>(), c.vel[dst].as<float4>(),
+            nullptr, c.ids[c.cur_ids ^ 1].as<uint32_t>(), skeys, c.grid, nullptr, nullptr, nullptr);
+    } else if (mixed) {
+      if (dem)
+        k_reorder<double4, true, true><<<blocks_for(n), kBlock, 0, c.stream>>>(
+            n, perm, c.pos[src].as<double4>(), c.vel[src].as<double4>(), c.omega[src].as<double4>(),
+            c.ids[c.cur_ids].as<uint32_t>(), c.pos[dst].as<double4>(), c.vel[dst].as<double4>(),
+            c.omega[dst].as<double4>(), c.ids[c.cur_ids ^ 1].as<uint32_t>(), skeys, c.grid,
+            c.off32.as<float4>(), c.vel32.as<float4>(), c.om32.as<float4>());
+      else
+        k_reorder<double4, false, true><<<blocks_for(n), kBlock, 0, c.stream>>>(
+            n, perm, c.pos[src].as<double4>(), c.vel[src].as<double4>(), nullptr,
+            c.ids[c.cur_ids].as<uint32_t>(), c.pos[dst].as<double4>(), c.vel[dst].as<double4>(),
+            nullptr, c.ids[c.cur_ids ^ 1].as<uint32_t>(), skeys, c.grid, c.off32.as<float4>(),
+            c.vel32.as<float4>(), nullptr);
+    } else {
+      if (dem)
+        k_reorder<double4, true, false><<<blocks_for(n), kBlock, 0, c.stream>>>(
+            n, perm, c.pos[src].as<double4>(), c.vel[src].as<double4>(), c.omega[src].as<double4>(),
+            c.ids[c.cur_ids].as<uint32_t>(), c.pos[dst].as<double4>(), c.vel[dst].as<double4>(),
+            c.omega[dst].as<double4>(), c.ids[c.cur_ids ^ 1].as<uint32_t>(), skeys, c.grid, nullptr, nullptr, nullptr);
+      else
+        k_reorder<double4, false, false><<<blocks_for(n), kBlock, 0, c.stream>>>(
+            n, perm, c.pos[src].as<double4>(), c.vel[src].as<double4>(), nullptr,
+            c.ids[c.cur_ids].as<uint32_t>(), c.pos[dst].as<double4>(), c.vel[dst].as<double4>(),
+            nullptr, c.ids[c.cur_ids ^ 1].as<uint32_t>(), skeys, c.grid, nullptr, nullptr, nullptr);
+    }
+    PR_HIP(hipGetLastError());
+  }
+  c.cur = dst;
+  c.cur_ids ^= 1;
+  {
+    LaunchScope ls(&c, "cell_lb", 8.0 * n + 4.0 * c.grid.n_cells);
+    k_cell_lb<<<blocks_for(n), kBlock, 0, c.stream>>>(n, skeys, c.lb.as<uint32_t>());
+    PR_HIP(hipGetLastError());
+    k_cell_lb_ends<<<blocks_for((uint64_t)c.grid.n_cells + 1), kBlock, 0, c.stream>>>(
+        n, c.grid.n_cells, skeys, c.lb.as<uint32_t>());
+    PR_HIP(hipGetLastError());
+  }
+  c.sorted = true;
+  c.built = true;
+}
+
+// ---------------------------------------------------------------------------
+// id-order transfer helpers
+// ---------------------------------------------------------------------------
+// src_kind / dst_kind: 0 = double, 1 = float; comp and stride select one
+// component of a packed element (stride 4 for double4/float4).
+
+template <class T>
+__global__ void k_scatter_ids(uint32_t n, const T* __restrict__ src, int stride, int comp,
+                              const uint32_t* __restrict__ ids, double* __restrict__ out) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[ids[k]] = (double)src[(size_t)k * stride + comp];
+}
+
+template <class T>
+__global__ void k_gather_ids(uint32_t n, const double* __restrict__ in,
+                             const uint32_t* __restrict__ ids, T* __restrict__ dst, int stride,
+                             int comp) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[(size_t)k * stride + comp] = (T)in[ids[k]];
+}
+
+__global__ void k_scatter_keys(uint32_t n, const uint32_t* __restrict__ keys,
+                               const uint32_t* __restrict__ ids, double* __restrict__ out) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[ids[k]] = (double)keys[k];
+}
+
+void scatter_to_ids(Ctx& c, const void* src, int src_kind, int comp, double* d_out) {
+  const uint32_t n = (uint32_t)c.n;
+  if (!n) return;
+  const uint32_t* ids = c.ids[c.cur_ids].as<uint32_t>();
+  int stride = comp >= 0 ? (comp >= 100 ? 2 : 4) : 1;
+  int cc = comp >= 100 ? comp - 100 : (comp >= 0 ? comp : 0);
+  if (src_kind == 2) {
+    k_scatter_keys<<<blocks_for(n), kBlock, 0, c.stream>>>(n, (const uint32_t*)src, ids, d_out);
+  } else if (src_kind == 0) {
+    k_scatter_ids<double><<<blocks_for(n), kBlock, 0, c.stream>>>(n, (const double*)src, stride, cc, ids, d_out);
+  } else {
+    k_scatter_ids<float><<<blocks_for(n), kBlock, 0, c.stream>>>(n, (const float*)src, stride, cc, ids, d_out);
+  }
+  PR_HIP(hipGetLastError());
+}
+
+void gather_from_ids(Ctx& c, const double* d_in, void* dst, int dst_kind, int comp) {
+  const uint32_t n = (uint32_t)c.n;
+  if (!n) return;
+  const uint32_t* ids = c.ids[c.cur_ids].as<uint32_t>();
+  int stride = comp >= 0 ? (comp >= 100 ? 2 : 4) : 1;
+  int cc = comp >= 100 ? comp - 100 : (comp >= 0 ? comp : 0);
+  if (dst_kind == 0)
+    k_gather_ids<double><<<blocks_for(n), kBlock, 0, c.stream>>>(n, d_in, ids, (double*)dst, stride, cc);
+  else
+    k_gather_ids<float><<<blocks_for(n), kBlock, 0, c.stream>>>(n, d_in, ids, (float*)dst, stride, cc);
+  PR_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// eq1: force[i] += mass[j] (prestige/src/lib.rs:10) inside the loop of
+// prestige/src/codegen/simple_cpu.rs:7-19.
+// ---------------------------------------------------------------------------
+
+// All pairs: thread t is particle id t, j runs over ids 0..n-1 in id order,
+// exactly the generated loop.  force and mass are held in id order.
+__global__ __launch_bounds__(kBlock) void k_eq1_allpairs(uint32_t n,
+                                                         const double* __restrict__ mass,
+                                                         double* __restrict__ force) {
+  __shared__ double tile[kBlock];
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = i < n ? force[i] : 0.0;
+  for (uint32_t j0 = 0; j0 < n; j0 += kBlock) {
+    uint32_t jj = j0 + threadIdx.x;
+    tile[threadIdx.x] = jj < n ? mass[jj] : 0.0;
+    __syncthreads();
+    uint32_t m = n - j0 < (uint32_t)kBlock ? n - j0 : (uint32_t)kBlock;
+    for (uint32_t t = 0; t < m; ++t) acc += tile[t];
+    __syncthreads();
+  }
+  if (i < n) force[i] = acc;
+}
+
+// Cells: j over the nine row runs of slot i, kept when |x_i - x_j|^2 < cutoff^2.
+// force / mass are in id order; slot i updates force[ids[i]] only.
+template <class T4>
+__global__ __launch_bounds__(kBlock) void k_eq1_cells(uint32_t n, const T4* __restrict__ pos,
+                                                      const uint32_t* __restrict__ ids,
+                                                      const uint32_t* __restrict__ skeys,
+                                                      const uint32_t* __restrict__ lb, Grid g,
+                                                      double c2, const double* __restrict__ mass,
+                                                      double* __restrict__ force) {
+#pragma clang fp contract(off)
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t nx = (uint32_t)g.dims[0], ny = (uint32_t)g.dims[1], nz = (uint32_t)g.dims[2];
+  uint32_t key = skeys[i];
+  int cx = (int)(key % nx), cy = (int)((key / nx) % ny), cz = (int)(key / (nx * ny));
+  int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < (int)nx - 1 ? cx + 1 : (int)nx - 1;
+  T4 pi = pos[i];
+  const uint32_t me = ids[i];
+  double acc = force[me];
+  for (int dz = -1; dz <= 1; ++dz) {
+    int zz = cz + dz;
+    if (zz < 0 || zz >= (int)nz) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      int yy = cy + dy;
+      if (yy < 0 || yy >= (int)ny) continue;
+      uint32_t row = ((uint32_t)zz * ny + (uint32_t)yy) * nx;
+      uint32_t j0 = lb[row + (uint32_t)x0], j1 = lb[row + (uint32_t)x1 + 1];
+      for (uint32_t j = j0; j < j1; ++j) {
+        T4 pj = pos[j];
+        double dx = (double)pi.x - (double)pj.x, dy_ = (double)pi.y - (double)pj.y,
+               dz_ = (double)pi.z - (double)pj.z;
+        double r2 = dx * dx + dy_ * dy_ + dz_ * dz_;
+        if (r2 < c2) acc += mass[ids[j]];
+      }
+    }
+  }
+  force[me] = acc;
+}
+
+void eq1_eval(Ctx& c, int loop, double cutoff) {
+  const uint32_t n = (uint32_t)c.n;
+  PR_CHECK(c.have_force && c.have_mass, PRESTIGE_E_STATE, "eq1 needs fields 'force' and 'mass' uploaded");
+  if (!n) return;
+  if (loop == PRESTIGE_LOOP_ALLPAIRS) {
+    LaunchScope ls(&c, "eq1_allpairs", 24.0 * n);
+    k_eq1_allpairs<<<blocks_for(n), kBlock, 0, c.stream>>>(n, c.mass.as<double>(), c.force.as<double>());
+    PR_HIP(hipGetLastError());
+    return;
+  }
+  PR_CHECK(loop == PRESTIGE_LOOP_CELLS, PRESTIGE_E_INVALID, "unknown loop mode");
+  PR_CHECK(cutoff > 0.0 && cutoff <= c.grid.cs, PRESTIGE_E_INVALID, "cutoff must be in (0, cell_size]");
+  if (!c.sorted) neighbors_build(c);
+  LaunchScope ls(&c, "eq1_cells", 56.0 * n);
+  const uint32_t* ids = c.ids[c.cur_ids].as<uint32_t>();
+  if (c.desc.precision == PRESTIGE_F32)
+    k_eq1_cells<float4><<<blocks_for(n), kBlock, 0, c.stream>>>(
+        n, c.pos[c.cur].as<float4>(), ids, c.keys[c.cur_key].as<uint32_t>(), c.lb.as<uint32_t>(),
+        c.grid, cutoff * cutoff, c.mass.as<double>(), c.force.as<double>());
+  else
+    k_eq1_cells<double4><<<blocks_for(n), kBlock, 0, c.stream>>>(
+        n, c.pos[c.cur].as<double4>(), ids, c.keys[c.cur_key].as<uint32_t>(), c.lb.as<uint32_t>(),
+        c.grid, cutoff * cutoff, c.mass.as<double>(), c.force.as<double>());
+  PR_HIP(hipGetLastError());
+}
+
+}  // namespace prestige

@@ -1,0 +1,78 @@
+"""WCSPH on the device vs the FP64 oracle.
+
+FP64 form: bit-identical fields (same operation order, no FMA contraction,
+same neighbour visiting order).  FP32 and mixed forms: norm-wise relative
+error per field, ||f - f_ref||_2 / ||f_ref||_2 over fluid particles, with
+the tolerances written in each test."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import prestige_amd as pa
+from helpers import make_scene, oracle_state, rel_l2, upload_all
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "y", "z", "vx", "vy", "vz", "rho", "p", "ax", "ay", "az")
+
+
+def _run_pair(d, arrs, nsteps):
+    st = oracle_state(d, arrs)
+    assert st.step(nsteps) == 0
+    with pa.Context(d) as ctx:
+        upload_all(ctx, arrs)
+        ctx.step(nsteps)
+        got = {k: ctx.download(k) for k in FIELDS}
+    ref = {k: st.get(k) for k in FIELDS}
+    return got, ref
+
+
+@pytest.mark.parametrize("nsteps", [1, 30])
+def test_f64_bitexact(nsteps):
+    d, arrs = make_scene(pa.SCENE_DAM_BREAK, n_target=8000, dx=0.005)
+    got, ref = _run_pair(d, arrs, nsteps)
+    for k in FIELDS:
+        bad = np.flatnonzero(got[k] != ref[k])
+        assert bad.size == 0, f"{k}: {bad.size} values differ, e.g. {got[k][bad[:3]]} vs {ref[k][bad[:3]]}"
+
+
+@pytest.mark.parametrize("precision,tol", [(pa.MIXED, {"x": 1e-9, "v": 1e-4, "rho": 1e-6, "p": 1e-3}),
+                                           (pa.F32, {"x": 1e-6, "v": 0.5, "rho": 1e-5, "p": 0.1})])
+def test_reduced_precision_error(precision, tol):
+    d, arrs = make_scene(pa.SCENE_DAM_BREAK, n_target=8000, dx=0.005)
+    st = oracle_state(d, arrs)
+    assert st.step(50) == 0
+    dm = d.copy()
+    dm.precision = precision
+    with pa.Context(dm) as ctx:
+        upload_all(ctx, arrs)
+        ctx.step(50)
+        got = {k: ctx.download(k) for k in FIELDS}
+    nf = d.n_moving
+    ex = rel_l2(np.stack([got[k][:nf] for k in "xyz"]), np.stack([st.get(k)[:nf] for k in "xyz"]))
+    ev = rel_l2(np.stack([got[k][:nf] for k in ("vx", "vy", "vz")]),
+                np.stack([st.get(k)[:nf] for k in ("vx", "vy", "vz")]))
+    er = rel_l2(got["rho"][:nf], st.get("rho")[:nf])
+    ep = rel_l2(got["p"][:nf], st.get("p")[:nf])
+    print(f"precision {precision}: x {ex:.3e} v {ev:.3e} rho {er:.3e} p {ep:.3e}")
+    assert ex < tol["x"] and ev < tol["v"] and er < tol["rho"] and ep < tol["p"]
+
+
+def test_domain_error_reported():
+    d, arrs = make_scene(pa.SCENE_DAM_BREAK, n_target=8000, dx=0.005)
+    x = arrs["x"].copy()
+    x[0] = d.grid_hi[0] + 1.0
+    with pa.Context(d) as ctx:
+        upload_all(ctx, dict(arrs, x=x))
+        with pytest.raises(pa.PrestigeError) as ei:
+            ctx.step(1)
+        assert ei.value.name == "E_DOMAIN"
+
+
+@pytest.mark.slow
+def test_full_size_f64_bitexact_two_steps():
+    d, arrs = make_scene(pa.SCENE_DAM_BREAK, n_target=1_000_000, dx=0.005)
+    got, ref = _run_pair(d, arrs, 2)
+    for k in ("x", "vz", "rho", "p", "az"):
+        assert np.array_equal(got[k], ref[k]), k

@@ -1,0 +1,377 @@
+"""Pins of the CPU FP64 oracle (no device).
+
+Each check fixes a property a plausible mistake would break: closed forms of
+the spline and the lattice density, brute-force neighbour sets and counts,
+the stable sort and lower-bound table against numpy, momentum conservation,
+the discrete free-fall trajectory, DEM restitution, contact duration,
+static overlap and the 5/7 rolling law, and complex-step derivatives against
+central differences.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import prestige_amd as pa
+from helpers import make_scene, oracle_state
+
+# ---------------------------------------------------------------------------
+# building blocks
+# ---------------------------------------------------------------------------
+
+
+def test_spline_normalisation_and_support():
+    L = oracle.lib()
+    # radial integral of W = sigma f(r/h) over the sphere of radius 2h equals 1 (sigma = 1/(pi h^3))
+    q = np.linspace(0.0, 2.0, 200001)
+    f = np.array([L.orc_spline_f(v) for v in q[::10]])
+    qq = q[::10]
+    integral = np.trapezoid(4 * math.pi * qq**2 * f, qq) / math.pi
+    assert abs(integral - 1.0) < 1e-8
+    assert L.orc_spline_f(2.0) == 0.0 and L.orc_spline_df(2.0) == 0.0
+    assert L.orc_spline_f(0.0) == 1.0
+    # C1 continuity at q = 1
+    e = 1e-12
+    assert abs(L.orc_spline_f(1 - e) - L.orc_spline_f(1 + e)) < 1e-11
+    assert abs(L.orc_spline_df(1 - e) - L.orc_spline_df(1 + e)) < 1e-11
+    # df is the derivative of f
+    for v in (0.3, 0.9, 1.2, 1.7):
+        fd = (L.orc_spline_f(v + 1e-6) - L.orc_spline_f(v - 1e-6)) / 2e-6
+        assert abs(fd - L.orc_spline_df(v)) < 1e-8
+
+
+def _lattice_offsets(radius_dx):
+    r = int(math.ceil(radius_dx))
+    g = np.arange(-r, r + 1)
+    o = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).astype(np.float64)
+    return o
+
+
+def test_lattice_density_closed_form():
+    """Interior of a cubic lattice at h = 1.2 dx: rho/rho0 = sum f(|k| dx / h) / (pi 1.2^3) = 1.000809548."""
+    L = oracle.lib()
+    o = _lattice_offsets(2.4)
+    qs = np.linalg.norm(o, axis=1) / 1.2
+    s = sum(L.orc_spline_f(v) for v in qs if v < 2.0)
+    ratio = s / (math.pi * 1.2**3)
+    assert abs(ratio - 1.000809548) < 5e-10
+    n_inside = int(np.sum(qs < 2.0))
+    assert n_inside == 57
+    # the oracle's summation density on a real tank reproduces it
+    d, arrs = make_scene(pa.SCENE_DAM_BREAK, n_target=8000, dx=0.005)
+    st = oracle_state(d, arrs)
+    assert st.density() == 0
+    rho = st.get("rho")
+    inner = np.arange(8000).reshape(20, 20, 20)[5:15, 5:15, 5:15].ravel()
+    assert np.allclose(rho[inner] / d.rho0, ratio, rtol=0, atol=1e-12)
+
+
+def test_cell_key_sort_lb_against_numpy():
+    rng = np.random.default_rng(7)
+    d, arrs = make_scene(pa.SCENE_DEM_POUR, n_target=5000, seed=9)
+    p = oracle.params_from_desc(d)
+    L = oracle.lib()
+    x, y, z = arrs["x"], arrs["y"], arrs["z"]
+    keys = np.array([L.orc_cell_key(p, a, b, c) for a, b, c in zip(x, y, z)], np.int64)
+    inv = 1.0 / d.cell_size
+    c = [np.floor((v - lo) * inv).astype(np.int64) for v, lo in zip((x, y, z), d.grid_lo)]
+    ref = (c[2] * p.dims[1] + c[1]) * p.dims[0] + c[0]
+    assert np.array_equal(keys, ref)
+    assert L.orc_cell_key(p, -1.0, 0.1, 0.1) == -1
+    assert L.orc_cell_key(p, 0.1, 0.1, 1e9) == -1
+    # stable sort == numpy stable argsort, including many duplicates
+    k32 = rng.integers(0, 50, 20000).astype(np.uint32)
+    perm = np.empty(k32.size, np.uint32)
+    u = lambda a: a.ctypes.data_as(oracle._U32)  # noqa: E731
+    L.orc_stable_sort(k32.size, u(k32), 50, u(perm))
+    assert np.array_equal(perm, np.argsort(k32, kind="stable"))
+    sk = np.ascontiguousarray(k32[perm])
+    lb = np.empty(51, np.uint32)
+    L.orc_cell_lb(sk.size, u(sk), 50, u(lb))
+    assert np.array_equal(lb, np.searchsorted(sk, np.arange(51), side="left"))
+
+
+def _brute_counts(pos, cutoff):
+    n = len(pos)
+    out = np.zeros(n)
+    for i in range(0, n, 512):
+        d2 = ((pos[i:i + 512, None, :] - pos[None, :, :]) ** 2).sum(-1)
+        out[i:i + 512] = (d2 < cutoff * cutoff).sum(1)
+    return out
+
+
+@pytest.mark.parametrize("scene,kw", [
+    (pa.SCENE_DEM_POUR, dict(n_target=3000, seed=1)),
+    (pa.SCENE_DAM_BREAK, dict(n_target=1000, dx=0.01)),
+])
+def test_cell_neighbours_equal_brute_force(scene, kw):
+    """eq1 with mass = 1 through the cell iterator counts exactly the all-pairs neighbours."""
+    d, arrs = make_scene(scene, **kw)
+    st = oracle_state(d, arrs)
+    assert st.build() == 0
+    cutoff = d.cell_size
+    got = st.eq1_cells(cutoff, np.zeros(d.n_particles), np.ones(d.n_particles))
+    pos = np.stack([arrs["x"], arrs["y"], arrs["z"]], 1)
+    assert np.array_equal(got, _brute_counts(pos, cutoff))
+
+
+def test_cell_neighbours_clustered_and_duplicate_points():
+    """Coincident points, particles on cell faces and empty cells."""
+    rng = np.random.default_rng(3)
+    d = pa.desc_init(pa.MODEL_PAIRS)
+    n = 2000
+    cs = 0.1
+    pts = np.concatenate([
+        rng.random((n // 2, 3)) * 0.2,                        # dense cluster
+        np.repeat(rng.random((10, 3)), 20, axis=0),           # 20-fold duplicates
+        np.round(rng.random((n // 2 - 200, 3)) * 10) * cs,    # exactly on cell faces
+    ])
+    n = len(pts)
+    d.n_particles = n
+    d.n_moving = n
+    d.cell_size = cs
+    for a in range(3):
+        d.grid_lo[a] = 0.0
+        d.grid_hi[a] = 1.0 + cs
+    st = oracle.State(oracle.params_from_desc(d), {"x": pts[:, 0], "y": pts[:, 1], "z": pts[:, 2],
+                                                   "vx": np.zeros(n), "vy": np.zeros(n), "vz": np.zeros(n)})
+    assert st.build() == 0
+    got = st.eq1_cells(cs, np.zeros(n), np.ones(n))
+    assert np.array_equal(got, _brute_counts(pts, cs))
+
+
+def test_eq1_allpairs_closed_form():
+    rng = np.random.default_rng(2)
+    m = np.round(rng.random(700) * 256) / 256          # dyadic: every partial sum exact
+    f0 = np.round(rng.standard_normal(700) * 64) / 64
+    out = oracle.eq1_allpairs(f0, m)
+    assert np.array_equal(out, f0 + m.sum())
+
+
+# ---------------------------------------------------------------------------
+# WCSPH invariants
+# ---------------------------------------------------------------------------
+
+
+def _fluid_block(side, dx, jitter, seed=0, gravity=0.0):
+    rng = np.random.default_rng(seed)
+    g = (np.arange(side) + 0.5) * dx
+    p = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    p = p + rng.uniform(-jitter, jitter, p.shape)
+    n = len(p)
+    d = pa.desc_init(pa.MODEL_WCSPH)
+    d.n_particles = n
+    d.n_moving = n
+    d.h = 1.2 * dx
+    d.cell_size = 2 * d.h
+    d.mass = d.rho0 * dx**3
+    d.c0 = 20.0
+    d.dt = 0.25 * d.h / d.c0
+    d.gravity[2] = -gravity
+    for a in range(3):
+        d.grid_lo[a] = -5 * dx
+        d.grid_hi[a] = (side + 5) * dx
+    arrs = {"x": p[:, 0], "y": p[:, 1], "z": p[:, 2]}
+    v = rng.uniform(-0.05, 0.05, p.shape)
+    arrs.update({"vx": v[:, 0], "vy": v[:, 1], "vz": v[:, 2]})
+    return d, arrs
+
+
+def test_momentum_conservation():
+    d, arrs = _fluid_block(12, 0.01, 0.002)
+    st = oracle_state(d, arrs)
+    assert st.step(1) == 0
+    a = np.stack([st.get("ax"), st.get("ay"), st.get("az")])
+    total = a.sum(1)
+    scale = np.abs(a).sum()
+    assert np.all(np.abs(total) < 1e-12 * scale)
+
+
+def test_uniform_interior_lattice_has_zero_pressure_force():
+    d, arrs = _fluid_block(16, 0.01, 0.0)
+    for k in ("vx", "vy", "vz"):
+        arrs[k] = np.zeros_like(arrs[k])
+    st = oracle_state(d, arrs)
+    assert st.step(1) == 0
+    ax = np.stack([st.get("ax"), st.get("ay"), st.get("az")])
+    inner = np.arange(16**3).reshape(16, 16, 16)[4:12, 4:12, 4:12].ravel()
+    edge = np.abs(ax).max()
+    assert edge > 1.0                       # the block's surface is pushed outwards
+    assert np.abs(ax[:, inner]).max() < 1e-9 * edge
+
+
+def test_free_fall_discrete_closed_form():
+    """A lone particle: v_n = v0 - n g dt, z_n = z0 + n v0 dt - g dt^2 n (n+1) / 2."""
+    d, arrs = _fluid_block(1, 0.01, 0.0, gravity=9.81)
+    for a in range(3):
+        d.grid_lo[a] = -50.0
+        d.grid_hi[a] = 50.0
+    d.cell_size = 1.0
+    arrs["vx"][:] = 0.0
+    arrs["vy"][:] = 0.0
+    arrs["vz"][:] = 0.7
+    z0 = arrs["z"][0]
+    st = oracle_state(d, arrs)
+    nsteps = 2000
+    assert st.step(nsteps) == 0
+    n, dt, g = nsteps, d.dt, 9.81
+    assert abs(st.get("vz")[0] - (0.7 - n * g * dt)) < 1e-12
+    assert abs(st.get("z")[0] - (z0 + n * 0.7 * dt - g * dt * dt * n * (n + 1) / 2)) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# DEM closed forms
+# ---------------------------------------------------------------------------
+
+
+def _dem_desc(n, dt=1e-5, box=1.0, walls=0, mu=0.5, e=0.3, gravity=0.0):
+    d = pa.desc_init(pa.MODEL_DEM)
+    d.n_particles = n
+    d.n_moving = n
+    d.cell_size = 0.05
+    d.dt = dt
+    d.mu = mu
+    d.restitution = e
+    d.gravity[2] = -gravity
+    for a in range(3):
+        d.grid_lo[a] = 0.0
+        d.grid_hi[a] = box
+        d.wall_lo[a] = 0.0
+        d.wall_hi[a] = box
+    d.wall_mask = walls
+    return d
+
+
+def _sphere_mass(r, rho=2500.0):
+    return rho * 4.0 / 3.0 * math.pi * r**3
+
+
+def _head_on(dt, e=0.3):
+    r, v = 0.01, 0.2
+    d = _dem_desc(2, dt=dt, e=e)
+    m = _sphere_mass(r)
+    arrs = {"x": np.array([0.5 - r - 1e-4, 0.5 + r + 1e-4]), "y": np.array([0.5, 0.5]),
+            "z": np.array([0.5, 0.5]), "vx": np.array([v, -v]), "vy": np.zeros(2), "vz": np.zeros(2),
+            "radius": np.full(2, r), "mass": np.full(2, m)}
+    st = oracle_state(d, arrs)
+    steps_contact = 0
+    for _ in range(int(0.004 / dt)):
+        assert st.step(1) == 0
+        if st.contact_count():
+            steps_contact += 1
+    vx = st.get("vx")
+    return (vx[1] - vx[0]) / (2 * v), steps_contact * dt, m, r
+
+
+def test_head_on_restitution_and_duration():
+    """Linear spring-dashpot: e = exp(-pi zeta / sqrt(1 - zeta^2)), t_c = pi / omega_d."""
+    kn = 1e5
+    res = {}
+    for dt in (1e-5, 2.5e-6):
+        e_num, t_c, m, r = _head_on(dt)
+        res[dt] = e_num
+    meff = m / 2
+    zeta = -math.log(0.3) / math.sqrt(math.pi**2 + math.log(0.3) ** 2)
+    wn = math.sqrt(kn / meff)
+    td = math.pi / (wn * math.sqrt(1 - zeta**2))
+    # first-order convergence of symplectic Euler towards the closed form
+    err_coarse, err_fine = abs(res[1e-5] - 0.3), abs(res[2.5e-6] - 0.3)
+    assert err_fine < 0.01
+    assert err_fine < err_coarse or err_coarse < 2e-3
+    assert abs(t_c - td) < 5 * 2.5e-6 + 0.02 * td
+
+
+def test_resting_overlap_and_rolling_transition():
+    r = 0.01
+    m = _sphere_mass(r)
+    kn = 1e5
+    g = 9.81
+    # resting sphere: overlap m g / kn once the dashpot has damped the bounce
+    d = _dem_desc(1, walls=0x10, gravity=g)
+    arrs = {"x": np.array([0.5]), "y": np.array([0.5]), "z": np.array([r]), "vx": np.zeros(1),
+            "vy": np.zeros(1), "vz": np.zeros(1), "radius": np.full(1, r), "mass": np.full(1, m)}
+    st = oracle_state(d, arrs)
+    assert st.step(20000) == 0
+    assert abs((r - st.get("z")[0]) - m * g / kn) < 1e-3 * m * g / kn
+    # sliding sphere reaches rolling at 5/7 of its initial speed (uniform sphere, I = 2/5 m r^2)
+    v0 = 0.5
+    arrs = {"x": np.array([0.2]), "y": np.array([0.5]), "z": np.array([r - m * g / kn]),
+            "vx": np.array([v0]), "vy": np.zeros(1), "vz": np.zeros(1), "radius": np.full(1, r),
+            "mass": np.full(1, m)}
+    st = oracle_state(d, arrs)
+    assert st.step(29600) == 0
+    vs, ws = [], []
+    for _ in range(400):            # average over ~2 periods of the undamped tangential spring
+        assert st.step(1) == 0
+        vs.append(st.get("vx")[0])
+        ws.append(st.get("wy")[0])
+    vx, wy = float(np.mean(vs)), float(np.mean(ws))
+    assert abs(vx / v0 - 5.0 / 7.0) < 0.01
+    assert abs(wy * r - vx) < 0.01 * v0          # rolling without slip: v = omega r
+
+
+def test_dem_pair_forces_cancel():
+    """Gather-form contact forces are exactly antisymmetric, so sum F = 0 without walls or gravity."""
+    d, arrs = make_scene(pa.SCENE_DEM_LATTICE, n_target=2000, seed=4)
+    d.wall_mask = 0
+    d.gravity[2] = 0.0
+    arrs = dict(arrs)
+    arrs["vx"] = arrs["vx"] * 20
+    st = oracle_state(d, arrs)
+    assert st.step(200) == 0
+    assert st.contact_count() > 0
+    m = arrs["mass"]
+    f = np.stack([st.get(k) * m for k in ("ax", "ay", "az")])
+    assert np.all(np.abs(f.sum(1)) < 1e-12 * np.abs(f).sum())
+
+
+# ---------------------------------------------------------------------------
+# complex-step forward mode vs central differences
+# ---------------------------------------------------------------------------
+
+
+def _fd_tangent(params, arrs, dparam, dv0, nsteps, which, eps):
+    """Central difference of the final state along (dparam, dv0)."""
+    outs = []
+    for s in (+1, -1):
+        p = oracle.Params()
+        oracle.C.memmove(oracle.C.byref(p), oracle.C.byref(params), oracle.C.sizeof(p))
+        a = {k: np.array(v, copy=True) for k, v in arrs.items()}
+        for k, name in enumerate(which):
+            setattr(p, name, getattr(p, name) + s * eps * dparam[k])
+        if "kn" in which:
+            p.kt = params.kt * (p.kn / params.kn)   # kt = 2/7 kn moves with kn
+        if dv0 is not None:
+            for c, key in enumerate(("vx", "vy", "vz")):
+                a[key] = a[key] + s * eps * dv0[c]
+        x, v, _, _ = oracle.cs_tangent(p, a, None, None, nsteps)
+        outs.append(np.concatenate([x.ravel(), v.ravel()]))
+    return (outs[0] - outs[1]) / (2 * eps)
+
+
+def test_complex_step_matches_fd_wcsph():
+    d, arrs = _fluid_block(8, 0.01, 0.002, gravity=9.81)
+    p = oracle.params_from_desc(d)
+    rng = np.random.default_rng(5)
+    dparam = np.array([0.3, 2.0])
+    dv0 = rng.standard_normal((3, d.n_particles)) * 0.01
+    x, v, tx, tv = oracle.cs_tangent(p, arrs, dparam, dv0, 20)
+    cs = np.concatenate([tx.ravel(), tv.ravel()])
+    fd = _fd_tangent(p, arrs, dparam, dv0, 20, ("alpha", "c0"), 1e-6)
+    assert np.linalg.norm(cs - fd) < 1e-5 * np.linalg.norm(cs)
+
+
+def test_complex_step_matches_fd_dem():
+    d, arrs = make_scene(pa.SCENE_DEM_POUR, n_target=600, seed=8)
+    p = oracle.params_from_desc(d)
+    rng = np.random.default_rng(6)
+    dparam = np.array([1e4, 0.05, 0.1])
+    dv0 = rng.standard_normal((3, d.n_particles)) * 0.01
+    x, v, tx, tv = oracle.cs_tangent(p, arrs, dparam, dv0, 200)
+    cs = np.concatenate([tx.ravel(), tv.ravel()])
+    fd = _fd_tangent(p, arrs, dparam, dv0, 200, ("kn", "e", "mu"), 1e-7)
+    assert np.linalg.norm(cs) > 0
+    assert np.linalg.norm(cs - fd) < 1e-4 * np.linalg.norm(cs)
