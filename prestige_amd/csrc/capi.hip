@@ -98,7 +98,9 @@ void Ctx::check_flags() {
   if (any) PR_HIP(hipMemsetAsync(flags.p, 0, sizeof(h), stream));
   if (h[FLAG_DOMAIN]) throw Error(PRESTIGE_E_DOMAIN, "a particle left the cell grid");
   if (h[FLAG_CAPACITY])
-    throw Error(PRESTIGE_E_CAPACITY, "a particle has more DEM contacts than max_contacts");
+    throw Error(PRESTIGE_E_CAPACITY,
+                "a particle has more neighbours than the neighbour list holds, or more DEM contacts "
+                "than max_contacts");
   if (h[FLAG_NONFINITE]) throw Error(PRESTIGE_E_NONFINITE, "non-finite acceleration");
 }
 
@@ -203,10 +205,15 @@ static void allocate(Ctx& c) {
   PR_HIP(hipMemsetAsync(c.rp.p, 0, nn * 2 * es, c.stream));
   PR_HIP(hipMemsetAsync(c.pres.p, 0, nn * es, c.stream));
   PR_HIP(hipMemsetAsync(c.acc.p, 0, nn * e4, c.stream));
+  if (mixed || c.desc.model == PRESTIGE_MODEL_WCSPH) c.off32.alloc(nn * sizeof(float4));
   if (mixed) {
-    c.off32.alloc(nn * sizeof(float4));
     c.vel32.alloc(nn * sizeof(float4));
     if (dem) c.om32.alloc(nn * sizeof(float4));
+  }
+  if (c.desc.model == PRESTIGE_MODEL_WCSPH) {
+    c.nlist.alloc(nn * c.list_cap * 4);
+    c.ncount.alloc(nn * 4);
+    PR_HIP(hipMemsetAsync(c.ncount.p, 0, nn * 4, c.stream));
   }
   c.scratch.alloc(nn * 8);
   if (dem) {

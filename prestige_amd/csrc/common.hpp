@@ -166,6 +166,11 @@ struct Ctx {
   // MIXED form gather copies
   DevBuf off32, vel32, om32;     // float4: offset from cell origin (cell x in .w); velocity
                                  // (mass in .w); DEM angular velocity (radius in .w)
+  // neighbour lists of the last density pass: entry k of slot i at
+  // nlist[k * n + i] (scan order), ncount[i] entries
+  DevBuf nlist, ncount;
+  uint32_t list_cap = 128;
+  double avg_neighbours = 57.0;  // for the algorithmic-byte count of the list
   // eq1 probe
   DevBuf force, mass, scratch;
   bool have_force = false, have_mass = false;
@@ -196,6 +201,18 @@ inline uint32_t xcd_grid(uint32_t nb) { return ((nb + 7u) / 8u) * 8u; }
 __device__ inline uint32_t xcd_tile(uint32_t b, uint32_t grid) {
   uint32_t per = grid / 8u;
   return (b % 8u) * per + b / 8u;
+}
+
+// The FP32 probe's .w carries the particle's cell coordinates modulo 4 (2
+// bits per axis).  Neighbours sit at a cell difference in {-1, 0, 1}, which
+// the residues recover exactly: ((c_j - c_i + 1) mod 4) - 1.
+__device__ inline uint32_t cell_code(uint32_t cx, uint32_t cy, uint32_t cz) {
+  return (cx & 3u) | ((cy & 3u) << 2) | ((cz & 3u) << 4);
+}
+
+__device__ inline int cell_shift(uint32_t wi, uint32_t wj, int axis) {
+  uint32_t t = ((wj >> (2 * axis)) - (wi >> (2 * axis)) + 1u) & 3u;
+  return (int)t - 1;
 }
 
 // Per-form kernel entry points (defined in the .hip files).

@@ -93,8 +93,7 @@ __global__ __launch_bounds__(kBlock) void k_dem_force(
       const G4 pj = gpos[j];
       real dx, dy, dz;
       if constexpr (F == kMixed) {
-        int cxj = (int)__float_as_uint(pj.w);
-        dx = (pi.x - pj.x) - (float)(cxj - cr.cx) * cs_u;
+        dx = (pi.x - pj.x) - (float)cell_shift(__float_as_uint(pi.w), __float_as_uint(pj.w), 0) * cs_u;
         dy = (pi.y - pj.y) - (float)ddy * cs_u;
         dz = (pi.z - pj.z) - (float)ddz * cs_u;
       } else {

@@ -252,10 +252,13 @@ static void radix_sort(Ctx& c) {
 }
 
 // ---------------------------------------------------------------------------
-// reorder into key order (+ FP32 offset copies for the MIXED form)
+// reorder into key order.  PROBE also writes the FP32 probe (offset from the
+// FP64 cell origin times off_scale, cell-coordinate residues mod 4 in .w);
+// MIXED also writes FP32 copies of the velocity (mass in .w) and, with OMEGA,
+// of the angular velocity (radius in .w).
 // ---------------------------------------------------------------------------
 
-template <class T4, bool OMEGA, bool MIXED>
+template <class T4, bool OMEGA, bool PROBE, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_reorder(
     uint32_t n, const uint32_t* __restrict__ perm, const T4* __restrict__ pos_in,
     const T4* __restrict__ vel_in, const T4* __restrict__ om_in,
@@ -275,7 +278,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     om_out[k] = w;
     if constexpr (MIXED) om32[k] = make_float4((float)w.x, (float)w.y, (float)w.z, (float)p.w);
   }
-  if constexpr (MIXED) {
+  if constexpr (PROBE) {
     uint32_t key = skeys[k];
     uint32_t cx = key % (uint32_t)g.dims[0];
     uint32_t rest = key / (uint32_t)g.dims[0];
@@ -284,9 +287,21 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     float ox = (float)(((double)p.x - (g.lo[0] + (double)cx * g.cs)) * g.off_scale);
     float oy = (float)(((double)p.y - (g.lo[1] + (double)cy * g.cs)) * g.off_scale);
     float oz = (float)(((double)p.z - (g.lo[2] + (double)cz * g.cs)) * g.off_scale);
-    off32[k] = make_float4(ox, oy, oz, __uint_as_float(cx));
-    vel32[k] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
+    off32[k] = make_float4(ox, oy, oz, __uint_as_float(cell_code(cx, cy, cz)));
   }
+  if constexpr (MIXED) vel32[k] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
+}
+
+template <class T4, bool OMEGA, bool PROBE, bool MIXED>
+static void launch_reorder(Ctx& c, const uint32_t* perm, const uint32_t* skeys, int src, int dst) {
+  const uint32_t n = (uint32_t)c.n;
+  k_reorder<T4, OMEGA, PROBE, MIXED><<<blocks_for(n), kBlock, 0, c.stream>>>(
+      n, perm, c.pos[src].as<T4>(), c.vel[src].as<T4>(), OMEGA ? c.omega[src].as<T4>() : nullptr,
+      c.ids[c.cur_ids].as<uint32_t>(), c.pos[dst].as<T4>(), c.vel[dst].as<T4>(),
+      OMEGA ? c.omega[dst].as<T4>() : nullptr, c.ids[c.cur_ids ^ 1].as<uint32_t>(), skeys, c.grid,
+      PROBE ? c.off32.as<float4>() : nullptr, MIXED ? c.vel32.as<float4>() : nullptr,
+      (MIXED && OMEGA) ? c.om32.as<float4>() : nullptr);
+  PR_HIP(hipGetLastError());
 }
 
 // lb[c] = first slot with key >= c.  Slot k (k >= 1) fills the cells
@@ -326,48 +341,26 @@ void neighbors_build(Ctx& c) {
   const uint32_t* skeys = c.keys[c.cur_key].as<uint32_t>();
   const int src = c.cur, dst = c.cur ^ 1;
   const bool dem = c.desc.model == PRESTIGE_MODEL_DEM;
+  const bool sph = c.desc.model == PRESTIGE_MODEL_WCSPH;
   const bool mixed = c.desc.precision == PRESTIGE_MIXED;
+  const bool f32 = c.desc.precision == PRESTIGE_F32;
   {
-    double elem = c.desc.precision == PRESTIGE_F32 ? 16.0 : 32.0;
-    double bytes = 4.0 + 2.0 * (2.0 * elem + 4.0) + (dem ? 2.0 * elem : 0.0) + (mixed ? 4.0 + 32.0 : 0.0);
+    const double elem = f32 ? 12.0 : 24.0;   // three coordinates per vector
+    const bool probe = sph || (dem && mixed);
+    double bytes = 4.0 + 2.0 * (2.0 * elem + 4.0) + (dem ? 2.0 * elem + 16.0 : 0.0) + 4.0 * (probe ? 1.0 : 0.0) +
+                   (probe ? 16.0 : 0.0) + (mixed ? 16.0 : 0.0) + (mixed && dem ? 16.0 : 0.0);
     LaunchScope ls(&c, "reorder", bytes * n);
-    if (c.desc.precision == PRESTIGE_F32) {
-      if (dem)
-        k_reorder<float4, true, false><<<blocks_for(n), kBlock, 0, c.stream>>>(
-            n, perm, c.pos[src].as<float4>(), c.vel[src].as<float4>(), c.omega[src].as<float4>(),
-            c.ids[c.cur_ids].as<uint32_t>(), c.pos[dst].as<float4>(), c.vel[dst].as<float4>(),
-            c.omega[dst].as<float4>(), c.ids[c.cur_ids ^ 1].as<uint32_t>(), skeys, c.grid, nullptr, nullptr, nullptr);
-      else
-        k_reorder<float4, false, false><<<blocks_for(n), kBlock, 0, c.stream>>>(
-            n, perm, c.pos[src].as<float4>(), c.vel[src].as<float4>(), nullptr,
-            c.ids[c.cur_ids].as<uint32_t>(), c.pos[dst].as<float4>(), c.vel[dst].as<float4>(),
-            nullptr, c.ids[c.cur_ids ^ 1].as<uint32_t>(), skeys, c.grid, nullptr, nullptr, nullptr);
-    } else if (mixed) {
-      if (dem)
-        k_reorder<double4, true, true><<<blocks_for(n), kBlock, 0, c.stream>>>(
-            n, perm, c.pos[src].as<double4>(), c.vel[src].as<double4>(), c.omega[src].as<double4>(),
-            c.ids[c.cur_ids].as<uint32_t>(), c.pos[dst].as<double4>(), c.vel[dst].as<double4>(),
-            c.omega[dst].as<double4>(), c.ids[c.cur_ids ^ 1].as<uint32_t>(), skeys, c.grid,
-            c.off32.as<float4>(), c.vel32.as<float4>(), c.om32.as<float4>());
-      else
-        k_reorder<double4, false, true><<<blocks_for(n), kBlock, 0, c.stream>>>(
-            n, perm, c.pos[src].as<double4>(), c.vel[src].as<double4>(), nullptr,
-            c.ids[c.cur_ids].as<uint32_t>(), c.pos[dst].as<double4>(), c.vel[dst].as<double4>(),
-            nullptr, c.ids[c.cur_ids ^ 1].as<uint32_t>(), skeys, c.grid, c.off32.as<float4>(),
-            c.vel32.as<float4>(), nullptr);
+    if (sph) {
+      if (f32) launch_reorder<float4, false, true, false>(c, perm, skeys, src, dst);
+      else if (mixed) launch_reorder<double4, false, true, true>(c, perm, skeys, src, dst);
+      else launch_reorder<double4, false, true, false>(c, perm, skeys, src, dst);
+    } else if (dem) {
+      if (mixed) launch_reorder<double4, true, true, true>(c, perm, skeys, src, dst);
+      else launch_reorder<double4, true, false, false>(c, perm, skeys, src, dst);
     } else {
-      if (dem)
-        k_reorder<double4, true, false><<<blocks_for(n), kBlock, 0, c.stream>>>(
-            n, perm, c.pos[src].as<double4>(), c.vel[src].as<double4>(), c.omega[src].as<double4>(),
-            c.ids[c.cur_ids].as<uint32_t>(), c.pos[dst].as<double4>(), c.vel[dst].as<double4>(),
-            c.omega[dst].as<double4>(), c.ids[c.cur_ids ^ 1].as<uint32_t>(), skeys, c.grid, nullptr, nullptr, nullptr);
-      else
-        k_reorder<double4, false, false><<<blocks_for(n), kBlock, 0, c.stream>>>(
-            n, perm, c.pos[src].as<double4>(), c.vel[src].as<double4>(), nullptr,
-            c.ids[c.cur_ids].as<uint32_t>(), c.pos[dst].as<double4>(), c.vel[dst].as<double4>(),
-            nullptr, c.ids[c.cur_ids ^ 1].as<uint32_t>(), skeys, c.grid, nullptr, nullptr, nullptr);
+      if (f32) launch_reorder<float4, false, false, false>(c, perm, skeys, src, dst);
+      else launch_reorder<double4, false, false, false>(c, perm, skeys, src, dst);
     }
-    PR_HIP(hipGetLastError());
   }
   c.cur = dst;
   c.cur_ids ^= 1;
