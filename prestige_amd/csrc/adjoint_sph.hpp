@@ -1,0 +1,436 @@
+// adjoint_sph.hpp — reverse mode of one WCSPH step, gather form.
+//
+// Forward step (sph_kernels.hpp), per moving particle k with d = x_k - x_j,
+// e = v_k - v_j, r = |d|, q = r/h over its frozen neighbour masks:
+//   rho_k = m sigma sum_j f(q),  p_k = max(B((rho_k/rho0)^7 - 1), 0),
+//   P_k = p_k / rho_k^2,  B = rho0 c0^2 / 7,
+//   a_k = g - (m sigma/h) sum_j S_kj d,  S = Q g(r),  Q = P_k + P_j + Pi,
+//   g(r) = f'(q)/r,  Pi = -alpha c0 mu / rhobar if w = e.d < 0 (else 0),
+//   mu = h w / D,  D = r^2 + 0.01 h^2,  rhobar = (rho_k + rho_j)/2,
+//   v' = v + dt a,  x' = x + dt v'.
+// With c_k = -(m sigma/h) abar_k, the loss depends on the pair (k, j) through
+// c_k . S d (row k) and c_j . S (-d) (row j).  S, g, Pi, mu and w are
+// symmetric in the pair, so both rows collapse into one body per neighbour
+// with C = c_k - c_j and U = C . d:
+//   xbar_k += S C + U Q g'(r) d/r + mubar h (e/D - 2 w d/D^2),
+//   vbar_k += mubar h d/D,   mubar = U g (-alpha c0 / rhobar)      (w < 0)
+//   Pbar_k += U g,           rhobar_k += U g alpha c0 mu / (2 rhobar^2)
+// and each particle gathers only from its own neighbour set: no atomics.
+// Parameter sums take row k only (each ordered pair once):
+//   dalpha += (c_k.d) g (-c0 mu/rhobar),  dc0 += (c_k.d) g (-alpha mu/rhobar).
+// EOS: rhobar_k += Pbar_k dP/drho,  dc0 += Pbar_k dP/dc0 (zero where p was
+// clamped).  Density: xbar_k += (m sigma/h) sum_j (rhobar_k + rhobar_j) f'(q) d/r.
+// Integrator: vbt = vbar' + dt xbar',  abar = dt vbt,  xbar = xbar' + ...,
+// vbar = vbt + ... .  dt is a fixed input (not differentiated through c0).
+//
+// The reverse kernels use the same pair geometry as the forward form: FP64
+// absolute positions for kF64, FP32 probe offsets in units of h for kMixed
+// (position adjoints are accumulated in those units and scaled by 1/h).
+// Adjoint sums are FP64 in every form.
+#pragma once
+#include "sph_kernels.hpp"
+
+namespace prestige {
+
+// What one recorded forward step left behind (slot order of that step).
+struct SphStepRecord {
+  const void* gpos;       // pair geometry: double4 positions (F64) / float4 probes (mixed)
+  const void* gvel;       // double4 / float4 velocities
+  const void* rp;         // (rho, p/rho^2): double2 / float2
+  const uint32_t* ids;
+  const uint32_t* perm;   // slot k of this step <- slot perm[k] of the previous order
+  const uint32_t* skeys;
+  const uint32_t* lb;
+  const uint64_t* masks;
+};
+
+template <class real>
+__device__ inline real spline_d2f(real q) {
+  if (q < real(1.0)) return real(-3.0) + real(4.5) * q;
+  return real(1.5) * (real(2.0) - q);
+}
+
+// Reverse of the symplectic-Euler update for one slot: vbt = vbar' + dt xbar',
+// c = -(m sigma/h) dt vbt; fixed particles get zero.
+static __global__ __launch_bounds__(kBlock) void k_adj_prep(uint32_t n, uint64_t n_moving,
+                                                     const uint32_t* __restrict__ ids,
+                                                     const double4* __restrict__ xb,
+                                                     double4* __restrict__ vb, double4* __restrict__ cvec,
+                                                     double dt, double msh) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  if ((uint64_t)ids[k] >= n_moving) {
+    vb[k] = make_double4(0, 0, 0, 0);
+    cvec[k] = make_double4(0, 0, 0, 0);
+    return;
+  }
+  double4 x = xb[k], v = vb[k];
+  double4 vt = make_double4(v.x + dt * x.x, v.y + dt * x.y, v.z + dt * x.z, 0.0);
+  vb[k] = vt;
+  double s = -msh * dt;
+  cvec[k] = make_double4(s * vt.x, s * vt.y, s * vt.z, 0.0);
+}
+
+// Block sum of two doubles into partial[blockIdx] (deterministic).
+__device__ inline void block_sum2(double a, double b, double2* __restrict__ partial, uint32_t slot) {
+  __shared__ double sa[kBlock / 64], sb[kBlock / 64];
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_down(a, (unsigned)o, 64);
+    b += __shfl_down(b, (unsigned)o, 64);
+  }
+  const uint32_t w = threadIdx.x >> 6;
+  if (__lane_id() == 0) {
+    sa[w] = a;
+    sb[w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ta = 0.0, tb = 0.0;
+    for (uint32_t k = 0; k < kBlock / 64; ++k) {
+      ta += sa[k];
+      tb += sb[k];
+    }
+    partial[slot] = make_double2(ta, tb);
+  }
+}
+
+// Force + EOS adjoint (all slots; fixed particles collect Pbar and rhobar as
+// neighbours of moving ones).  Writes xbar of the force rows (physical
+// units), adds the viscosity part of vbar, writes rhobar, and block partials
+// of (dalpha, dc0).
+template <int F>
+__global__ __launch_bounds__(kBlock) void k_adj_force(
+    uint32_t n, uint64_t n_moving, SphStepRecord rec, Grid g, SphConst kc, SphUnits u, double alpha, double c0,
+    double rho0, const double4* __restrict__ cvec, double4* __restrict__ xbf, double4* __restrict__ vb,
+    double* __restrict__ rhob, double2* __restrict__ partial) {
+  using real = typename SphForm<F>::real;
+  using G4 = typename SphForm<F>::G4;
+  using R2 = typename SphForm<F>::R2;
+  const G4* gpos = (const G4*)rec.gpos;
+  const G4* gvel = (const G4*)rec.gvel;
+  const R2* rp = (const R2*)rec.rp;
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t i = tile * blockDim.x + threadIdx.x;
+  double pa = 0.0, pc = 0.0;
+  if (i < n) {
+    const G4 pi = gpos[i], vi = gvel[i];
+    const R2 rpi = rp[i];
+    const double4 ci = cvec[i];
+    const real H2 = (real)u.H2, inv_h = (real)u.inv_h, eps2 = (real)u.eps2, hv = (real)u.hv;
+    const double ac0 = kc.alpha_c0;
+    double xx = 0, xy = 0, xz = 0, vx = 0, vy = 0, vz = 0, pb = 0, rb = 0;
+    const CellRows cr = decode_cell(rec.skeys[i], g);
+    uint32_t r = 0;
+    for_each_run(cr, g, rec.lb, [&](uint32_t j0, uint32_t, int, int) {
+      const uint64_t m0 = rec.masks[mask_index(r, 0, n, i)], m1 = rec.masks[mask_index(r, 1, n, i)];
+      ++r;
+      walk_bits(m0, m1, [&](const uint32_t* jj) {
+        G4 pj[kUnr], vj[kUnr];
+        R2 rpj[kUnr];
+        double4 cj[kUnr];
+#pragma unroll
+        for (int v = 0; v < kUnr; ++v) {
+          const uint32_t j = j0 + (jj[v] == kPad ? jj[0] : jj[v]);
+          pj[v] = gpos[j];
+          vj[v] = gvel[j];
+          rpj[v] = rp[j];
+          cj[v] = cvec[j];
+        }
+#pragma unroll
+        for (int v = 0; v < kUnr; ++v) {
+          if (jj[v] == kPad) continue;
+          real dxr, dyr, dzr;
+          form_pair<F>(pi, pj[v], u.cs_u, dxr, dyr, dzr);
+          real r2r = dxr * dxr + dyr * dyr + dzr * dzr;
+          if (!(r2r < H2 && r2r > real(0.0))) continue;
+          // pair scalars in FP64 from the form's pair vector
+          const double dx = dxr, dy = dyr, dz = dzr, r2 = r2r;
+          const double rr = sqrt(r2), q = rr * (double)inv_h;
+          const double f1 = (double)spline_df((real)q), f2 = (double)spline_d2f((real)q);
+          const double gr = f1 / rr;
+          const double gp = f2 * (double)inv_h / rr - f1 / r2;
+          const double ex = (double)vi.x - (double)vj[v].x, ey = (double)vi.y - (double)vj[v].y,
+                       ez = (double)vi.z - (double)vj[v].z;
+          const double w = ex * dx + ey * dy + ez * dz;
+          const double rhoi = (double)rpi.x, rhoj = (double)rpj[v].x;
+          const double rbar = 0.5 * (rhoi + rhoj);
+          double mu = 0.0, Pi = 0.0;
+          const double D = r2 + (double)eps2;
+          if (w < 0.0) {
+            mu = (double)hv * w / D;
+            Pi = -ac0 * mu / rbar;
+          }
+          const double Q = (double)rpi.y + (double)rpj[v].y + Pi;
+          const double S = Q * gr;
+          const double Cx = ci.x - cj[v].x, Cy = ci.y - cj[v].y, Cz = ci.z - cj[v].z;
+          const double U = Cx * dx + Cy * dy + Cz * dz;
+          const double t = U * Q * gp / rr;
+          xx += S * Cx + t * dx;
+          xy += S * Cy + t * dy;
+          xz += S * Cz + t * dz;
+          pb += U * gr;
+          if (w < 0.0) {
+            const double mub = U * gr * (-ac0 / rbar);
+            const double hD = (double)hv / D, k2 = 2.0 * (double)hv * w / (D * D);
+            xx += mub * (hD * ex - k2 * dx);
+            xy += mub * (hD * ey - k2 * dy);
+            xz += mub * (hD * ez - k2 * dz);
+            vx += mub * hD * dx;
+            vy += mub * hD * dy;
+            vz += mub * hD * dz;
+            rb += U * gr * ac0 * mu / (2.0 * rbar * rbar);
+            const double T1 = (ci.x * dx + ci.y * dy + ci.z * dz) * gr;
+            pa += T1 * (-c0 * mu / rbar);
+            pc += T1 * (-alpha * mu / rbar);
+          }
+        }
+      });
+    });
+    // EOS: P = p / rho^2, p = B((rho/rho0)^7 - 1) unless the forward clamped
+    // it to 0 (then P = 0 and the EOS has no derivative)
+    const double rho = (double)rpi.x;
+    const double ratio = rho * kc.inv_rho0;
+    const double r6 = ratio * ratio * ratio * ratio * ratio * ratio;
+    const double p = (double)rpi.y * rho * rho;
+    if ((double)rpi.y > 0.0) {
+      const double dpdrho = 7.0 * kc.B * r6 * kc.inv_rho0;
+      const double dPdrho = dpdrho / (rho * rho) - 2.0 * p / (rho * rho * rho);
+      rb += pb * dPdrho;
+      const double dPdc0 = (r6 * ratio - 1.0) * (2.0 * rho0 * c0 / 7.0) / (rho * rho);
+      pc += pb * dPdc0;
+    }
+    rhob[i] = rb;
+    const double us = u.unit_scale;   // pair-geometry units -> metres for position adjoints
+    xbf[i] = make_double4(us * xx, us * xy, us * xz, 0.0);
+    if ((uint64_t)rec.ids[i] < n_moving) {
+      double4 v0 = vb[i];
+      vb[i] = make_double4(v0.x + vx, v0.y + vy, v0.z + vz, 0.0);
+    }
+  }
+  block_sum2(pa, pc, partial, blockIdx.x);
+}
+
+// Density adjoint + assembly: xbar_k = xbar'_k + force part + density part
+// (moving particles; fixed ones 0), then both adjoints move to the previous
+// step's slot order through the frozen permutation.
+template <int F>
+__global__ __launch_bounds__(kBlock) void k_adj_density(
+    uint32_t n, uint64_t n_moving, SphStepRecord rec, Grid g, SphConst kc, SphUnits u,
+    const double* __restrict__ rhob, const double4* __restrict__ xb_in, const double4* __restrict__ xbf,
+    const double4* __restrict__ vb_in, double4* __restrict__ xb_out, double4* __restrict__ vb_out) {
+  using real = typename SphForm<F>::real;
+  using G4 = typename SphForm<F>::G4;
+  const G4* gpos = (const G4*)rec.gpos;
+  const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t dst = rec.perm[i];
+  if ((uint64_t)rec.ids[i] >= n_moving) {
+    xb_out[dst] = make_double4(0, 0, 0, 0);
+    vb_out[dst] = make_double4(0, 0, 0, 0);
+    return;
+  }
+  const G4 pi = gpos[i];
+  const double rbi = rhob[i];
+  const real H2 = (real)u.H2, inv_h = (real)u.inv_h;
+  double sx = 0, sy = 0, sz = 0;
+  const CellRows cr = decode_cell(rec.skeys[i], g);
+  uint32_t r = 0;
+  for_each_run(cr, g, rec.lb, [&](uint32_t j0, uint32_t, int, int) {
+    const uint64_t m0 = rec.masks[mask_index(r, 0, n, i)], m1 = rec.masks[mask_index(r, 1, n, i)];
+    ++r;
+    walk_bits(m0, m1, [&](const uint32_t* jj) {
+      G4 pj[kUnr];
+      double rbj[kUnr];
+#pragma unroll
+      for (int v = 0; v < kUnr; ++v) {
+        const uint32_t j = j0 + (jj[v] == kPad ? jj[0] : jj[v]);
+        pj[v] = gpos[j];
+        rbj[v] = rhob[j];
+      }
+#pragma unroll
+      for (int v = 0; v < kUnr; ++v) {
+        if (jj[v] == kPad) continue;
+        real dxr, dyr, dzr;
+        form_pair<F>(pi, pj[v], u.cs_u, dxr, dyr, dzr);
+        real r2r = dxr * dxr + dyr * dyr + dzr * dzr;
+        if (!(r2r < H2 && r2r > real(0.0))) continue;
+        const double rr = sqrt((double)r2r);
+        const double f1 = (double)spline_df((real)(rr * (double)inv_h));
+        const double s = (rbi + rbj[v]) * f1 / rr;
+        sx += s * (double)dxr;
+        sy += s * (double)dyr;
+        sz += s * (double)dzr;
+      }
+    });
+  });
+  const double k = kc.msig * (double)inv_h * u.unit_scale;
+  const double4 a = xb_in[i], b = xbf[i];
+  xb_out[dst] = make_double4(a.x + b.x + k * sx, a.y + b.y + k * sy, a.z + b.z + k * sz, 0.0);
+  vb_out[dst] = vb_in[i];
+}
+
+// Sum the block partials of one reverse step into acc[0..1] (one block).
+static __global__ void k_adj_reduce(uint32_t nparts, const double2* __restrict__ partial, double* __restrict__ acc) {
+  __shared__ double sa[256], sb[256];
+  double a = 0.0, b = 0.0;
+  for (uint32_t k = threadIdx.x; k < nparts; k += blockDim.x) {
+    a += partial[k].x;
+    b += partial[k].y;
+  }
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sa[threadIdx.x] += sa[threadIdx.x + s];
+      sb[threadIdx.x] += sb[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    acc[0] += sa[0];
+    acc[1] += sb[0];
+  }
+}
+
+// Device buffers of the reverse sweep.
+struct AdjBufs {
+  double4 *xb, *vb, *xb2, *vb2, *cvec, *xbf;
+  double* rhob;
+  double2* partial;
+  double* acc;
+};
+
+template <int F>
+void sph_reverse_step(Ctx& c, const SphStepRecord& rec, AdjBufs& A) {
+  const uint32_t n = (uint32_t)c.n;
+  const SphUnits u = sph_units<F>(c);
+  const uint32_t grid = xcd_grid(blocks_for(n));
+  {
+    LaunchScope ls(&c, "adj_prep", 2.0 * 32.0 * n + 32.0 * n);
+    k_adj_prep<<<blocks_for(n), kBlock, 0, c.stream>>>(n, c.n_moving, rec.ids, A.xb, A.vb, A.cvec, c.sph.dt,
+                                                       c.sph.msh);
+    PR_HIP(hipGetLastError());
+  }
+  {
+    const double ge = (double)sizeof(typename SphForm<F>::real);
+    LaunchScope ls(&c, F == kF64 ? "adj_force_f64" : "adj_force_mixed", (8.0 * ge + 24.0 + 24.0 + 8.0 + 48.0) * n);
+    k_adj_force<F><<<grid, kBlock, 0, c.stream>>>(n, c.n_moving, rec, c.grid, c.sph, u, c.desc.alpha, c.desc.c0,
+                                                 c.desc.rho0, A.cvec, A.xbf, A.vb, A.rhob, A.partial);
+    PR_HIP(hipGetLastError());
+    k_adj_reduce<<<1, 256, 0, c.stream>>>(grid, A.partial, A.acc);
+    PR_HIP(hipGetLastError());
+  }
+  {
+    const double ge = (double)sizeof(typename SphForm<F>::real);
+    LaunchScope ls(&c, F == kF64 ? "adj_density_f64" : "adj_density_mixed",
+                   (3.0 * ge + 8.0 + 24.0 * 3 + 24.0 * 2 + 8.0) * n);
+    k_adj_density<F><<<grid, kBlock, 0, c.stream>>>(n, c.n_moving, rec, c.grid, c.sph, u, A.rhob, A.xb, A.xbf,
+                                                   A.vb, A.xb2, A.vb2);
+    PR_HIP(hipGetLastError());
+  }
+  std::swap(A.xb, A.xb2);
+  std::swap(A.vb, A.vb2);
+}
+
+}  // namespace prestige
+
+namespace prestige {
+
+// Host helpers shared by the forms (adjoint.hip).
+void snapshot_state(Ctx& c, Snapshot& s);
+void restore_state(Ctx& c, const Snapshot& s);
+void adj_seed_gather(Ctx& c, const uint32_t* ids, const double* seed_x, const double* seed_v);
+void adj_grad_scatter(Ctx& c, const uint32_t* ids, const double4* vb, double* grad_v0_host);
+void ensure_adj_buffers(Ctx& c, uint32_t grid_blocks);
+
+// Copy what the reverse sweep needs from the step that just ran.
+template <int F>
+void record_sph_step(Ctx& c, StepRecord& r) {
+  const size_t n = c.n ? c.n : 1;
+  const size_t g4 = (F == kF64) ? sizeof(double4) : sizeof(float4);
+  const size_t r2 = (F == kF64) ? sizeof(double2) : sizeof(float2);
+  r.gpos.alloc(n * g4);
+  r.gvel.alloc(n * g4);
+  r.rp.alloc(n * r2);
+  r.ids.alloc(n * 4);
+  r.perm.alloc(n * 4);
+  r.skeys.alloc(n * 4);
+  r.lb.alloc(((size_t)c.grid.n_cells + 1) * 4);
+  r.masks.alloc(n * 18 * 8);
+  const int prev = c.cur ^ 1;   // the step left its sorted input in the other buffer
+  const void* gp = (F == kMixed) ? c.off32.p : c.pos[prev].p;
+  const void* gv = (F == kMixed) ? c.vel32.p : c.vel[prev].p;
+  auto cp = [&](DevBuf& d, const void* s, size_t b) {
+    PR_HIP(hipMemcpyAsync(d.p, s, b, hipMemcpyDeviceToDevice, c.stream));
+  };
+  cp(r.gpos, gp, c.n * g4);
+  cp(r.gvel, gv, c.n * g4);
+  cp(r.rp, c.rp.p, c.n * r2);
+  cp(r.ids, c.ids[c.cur_ids].p, c.n * 4);
+  cp(r.perm, c.vals[c.perm_buf].p, c.n * 4);
+  cp(r.skeys, c.keys[c.cur_key ^ 1].p, c.n * 4);
+  cp(r.lb, c.lb.p, ((size_t)c.grid.n_cells + 1) * 4);
+  cp(r.masks, c.nmask.p, c.n * 18 * 8);
+}
+
+template <int F>
+void sph_tape_forward(Ctx& c, uint32_t nsteps, uint32_t every) {
+  c.tape.clear();
+  c.tape.nsteps = nsteps;
+  c.tape.every = every;
+  for (uint32_t s = 0; s < nsteps; ++s) {
+    if (s % every == 0) {
+      c.tape.ckpt.emplace_back(new Snapshot());
+      snapshot_state(c, *c.tape.ckpt.back());
+    }
+    sph_step_impl<F>(c);
+  }
+  c.tape.final_state.reset(new Snapshot());
+  snapshot_state(c, *c.tape.final_state);
+  c.check_flags();
+  c.tape.valid = true;
+}
+
+template <int F>
+void sph_adjoint(Ctx& c, const double* seed_x, const double* seed_v, double* grad_params, double* grad_v0) {
+  Tape& T = c.tape;
+  const uint32_t n = (uint32_t)c.n;
+  const uint32_t grid = xcd_grid(blocks_for(n));
+  ensure_adj_buffers(c, grid);
+  AdjBufs A{T.xb.as<double4>(), T.vb.as<double4>(), T.xb2.as<double4>(), T.vb2.as<double4>(),
+            T.cvec.as<double4>(), T.xbf.as<double4>(), T.rhob.as<double>(), T.partial.as<double2>(),
+            T.acc.as<double>()};
+  PR_HIP(hipMemsetAsync(A.acc, 0, 2 * sizeof(double), c.stream));
+  adj_seed_gather(c, T.final_state->ids.as<uint32_t>(), seed_x, seed_v);
+  const uint32_t nseg = (T.nsteps + T.every - 1) / T.every;
+  while (T.rec.size() < T.every) T.rec.emplace_back(new StepRecord());
+  for (uint32_t seg = nseg; seg-- > 0;) {
+    restore_state(c, *T.ckpt[seg]);
+    const uint32_t s0 = seg * T.every;
+    const uint32_t len = (T.nsteps - s0) < T.every ? (T.nsteps - s0) : T.every;
+    for (uint32_t t = 0; t < len; ++t) {
+      sph_step_impl<F>(c);
+      record_sph_step<F>(c, *T.rec[t]);
+    }
+    for (uint32_t t = len; t-- > 0;) {
+      const StepRecord& r = *T.rec[t];
+      SphStepRecord sr{r.gpos.p, r.gvel.p, r.rp.p, r.ids.as<uint32_t>(), r.perm.as<uint32_t>(),
+                       r.skeys.as<uint32_t>(), r.lb.as<uint32_t>(), r.masks.as<uint64_t>()};
+      sph_reverse_step<F>(c, sr, A);
+    }
+  }
+  // A.xb / A.vb now hold the adjoints of the state the tape started from
+  if (grad_params) {
+    double acc[2];
+    PR_HIP(hipMemcpyAsync(acc, A.acc, sizeof(acc), hipMemcpyDeviceToHost, c.stream));
+    PR_HIP(hipStreamSynchronize(c.stream));
+    grad_params[0] = acc[0];
+    grad_params[1] = acc[1];
+  }
+  if (grad_v0) adj_grad_scatter(c, T.ckpt[0]->ids.as<uint32_t>(), A.vb, grad_v0);
+  restore_state(c, *T.final_state);
+  c.check_flags();
+}
+
+}  // namespace prestige

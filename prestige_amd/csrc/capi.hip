@@ -99,8 +99,8 @@ void Ctx::check_flags() {
   if (h[FLAG_DOMAIN]) throw Error(PRESTIGE_E_DOMAIN, "a particle left the cell grid");
   if (h[FLAG_CAPACITY])
     throw Error(PRESTIGE_E_CAPACITY,
-                "a particle has more neighbours than the neighbour list holds, or more DEM contacts "
-                "than max_contacts");
+                "a row of three cells holds more than 128 candidates, or a particle has more DEM "
+                "contacts than max_contacts");
   if (h[FLAG_NONFINITE]) throw Error(PRESTIGE_E_NONFINITE, "non-finite acceleration");
 }
 
@@ -211,9 +211,7 @@ static void allocate(Ctx& c) {
     if (dem) c.om32.alloc(nn * sizeof(float4));
   }
   if (c.desc.model == PRESTIGE_MODEL_WCSPH) {
-    c.nlist.alloc(nn * c.list_cap * 4);
-    c.ncount.alloc(nn * 4);
-    PR_HIP(hipMemsetAsync(c.ncount.p, 0, nn * 4, c.stream));
+    c.nmask.alloc(nn * 18 * 8);
   }
   c.scratch.alloc(nn * 8);
   if (dem) {

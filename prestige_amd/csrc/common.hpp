@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -121,13 +122,35 @@ struct LaunchScope {
   ~LaunchScope() noexcept(false);
 };
 
-// Adjoint tape: FP64 checkpoints of the state every K steps.
+// Device copy of the full integration state (used as an adjoint checkpoint).
+struct Snapshot {
+  DevBuf pos, vel, omega, ids, keys;
+  DevBuf hcnt, hid, hxi, wxi;   // DEM contact history
+  int cur = 0, cur_ids = 0, cur_key = 0, cur_hist = 0;
+  bool sorted = false, keys_valid = false, built = false;
+};
+
+// What the reverse sweep needs from one forward step: the sorted pre-step
+// pair geometry, the gather outputs, and the frozen neighbour structure.
+struct StepRecord {
+  DevBuf gpos, gvel, gom, rp, ids, perm, skeys, lb, masks;
+  DevBuf hcnt, hid, hxi, wxi;   // DEM history read by the step
+  DevBuf state_pos, state_vel, state_om;   // DEM FP64 pre-step state (sorted)
+};
+
+// Adjoint tape: checkpoints every `every` steps of an nsteps forward run.
 struct Tape {
   bool valid = false;
   uint32_t nsteps = 0, every = 0;
-  std::vector<DevBuf*> ckpt;   // one buffer per checkpoint (state snapshot, id order)
-  ~Tape() { for (auto* b : ckpt) delete b; }
-  void clear() { for (auto* b : ckpt) delete b; ckpt.clear(); valid = false; }
+  std::vector<std::unique_ptr<Snapshot>> ckpt;   // state before step s * every
+  std::unique_ptr<Snapshot> final_state;
+  std::vector<std::unique_ptr<StepRecord>> rec;  // reused across segments
+  DevBuf xb, vb, xb2, vb2, cvec, xbf, rhob, partial, acc, seeds, wb, wb2;
+  void clear() {
+    ckpt.clear();
+    final_state.reset();
+    valid = false;
+  }
 };
 
 struct Ctx {
@@ -166,11 +189,10 @@ struct Ctx {
   // MIXED form gather copies
   DevBuf off32, vel32, om32;     // float4: offset from cell origin (cell x in .w); velocity
                                  // (mass in .w); DEM angular velocity (radius in .w)
-  // neighbour lists of the last density pass: entry k of slot i at
-  // nlist[k * n + i] (scan order), ncount[i] entries
-  DevBuf nlist, ncount;
-  uint32_t list_cap = 128;
-  double avg_neighbours = 57.0;  // for the algorithmic-byte count of the list
+  // neighbour masks of the last density pass: for each of the (up to) nine
+  // row runs of a slot, 128 bits marking the run's candidates inside the
+  // cutoff; word w of run r of slot i at nmask[(2 r + w) n + i]
+  DevBuf nmask;
   // eq1 probe
   DevBuf force, mass, scratch;
   bool have_force = false, have_mass = false;

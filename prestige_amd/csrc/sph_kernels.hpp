@@ -8,19 +8,21 @@
 //
 // Two passes per step, split where the fused bodies would read rho[j] and
 // p[j] that the density body writes:
-//   density  (a) scans the 27-cell candidates with a compact FP32 probe of
-//            the positions (16 B per particle) and appends every candidate
-//            inside the cutoff plus a relative margin of 2^-10 to the
-//            particle's neighbour list, in scan order;
-//            (b) walks that list in the form's own precision, keeps the
-//            exact r < 2h pairs and sums the density, then applies the EOS.
-//   force    walks the same list: momentum + viscosity, symplectic Euler and
+//   density  for each of the nine row runs of the particle's cell: (a) scan
+//            the run's candidates with a compact FP32 probe of the positions
+//            (16 B per particle) and mark every candidate inside the cutoff
+//            plus a relative margin of 2^-10 in a 128-bit run mask, stored
+//            with one coalesced write per word; (b) walk the mask's set bits
+//            in the form's own precision, keep the exact r < 2h pairs and sum
+//            the density.  Then the EOS.
+//   force    walks the same masks: momentum + viscosity, symplectic Euler and
 //            the next step's cell key.
 // Scanning in FP32 halves the bytes each candidate costs, and the expensive
-// pair bodies only ever run over the compact list, where every lane of a
-// wavefront has work, instead of inside the 15 %-hit candidate loop.  The
-// list keeps scan order, so FP64 sums are formed in exactly the order of the
-// host oracle; it is also the frozen neighbour set the adjoint reuses.
+// pair bodies only ever run over the set bits, where every lane of a
+// wavefront has work, instead of inside the 15 %-hit candidate loop.  Runs
+// in order and bits in ascending order are the scan order, so FP64 sums are
+// formed in exactly the order of the host oracle; the masks are also the
+// frozen neighbour sets the adjoint reuses.
 //
 // Forms:
 //   kF64   state and pair arithmetic in FP64, positions absolute.  The
@@ -62,6 +64,7 @@ struct SphUnits {
   double hv;       // viscosity mu = hv * (v.r) / (r^2 + eps2): h, or 1 in units of h
   float cs_u;      // cell size in probe units (cs / h)
   float H2_probe;  // probe cutoff^2 in units of h, with margin for forms that recheck
+  double unit_scale;  // d(pair units)/d(metres): 1/h for the mixed form, 1 otherwise
 };
 
 // Plain or compensated running sum.
@@ -156,69 +159,128 @@ __device__ inline real spline_df(real q) {
   return real(-0.75) * t * t;
 }
 
-// Scan of the 27-cell candidates of slot i with the FP32 probe; appends hits
-// to the k-major list (entry k of slot i at list[k * n + i]).  Returns the
-// number of hits (may exceed cap: the caller flags the overflow).
-__device__ inline uint32_t probe_scan(uint32_t i, uint32_t n, const float4* __restrict__ probe,
-                                      const CellRows& cr, const Grid& g, const uint32_t* __restrict__ lb,
-                                      float cs_u, float H2p, uint32_t cap, uint32_t* __restrict__ list) {
-  const float4 qi = probe[i];
-  const uint32_t wi = __float_as_uint(qi.w);
-  uint32_t cnt = 0;
-  for_each_run(cr, g, lb, [&](uint32_t j0, uint32_t j1, int ddy, int ddz) {
-    const float yi = qi.y - (float)ddy * cs_u;   // row shifts are known per run
-    const float zi = qi.z - (float)ddz * cs_u;
-    for (uint32_t j = j0; j < j1; ++j) {
-      const float4 qj = probe[j];
-      float dx = (qi.x - qj.x) - (float)cell_shift(wi, __float_as_uint(qj.w), 0) * cs_u;
-      float dy = yi - qj.y;
-      float dz = zi - qj.z;
+// Candidate loads kept in flight per lane.  The gathers are latency-bound
+// (dependent loads, little arithmetic per byte), so each lane issues a batch
+// of kUnr independent loads before it uses any of them.
+#ifndef PRESTIGE_UNROLL
+#define PRESTIGE_UNROLL 2
+#endif
+constexpr int kUnr = PRESTIGE_UNROLL;
+constexpr uint32_t kPad = 0xFFFFFFFFu;
+
+// Scan one row run [j0, j0 + len) with the FP32 probe: bit b of the 128-bit
+// mask (m0, m1) is set when candidate j0 + b lies inside the probe cutoff.
+// Loads of a batch use clamped indices, so no load is ever out of the run.
+__device__ inline void scan_run(const float4* __restrict__ probe, const float4& qi, uint32_t wi,
+                                float yi, float zi, float cs_u, float H2p, uint32_t j0, uint32_t len,
+                                uint64_t& m0, uint64_t& m1) {
+  m0 = 0;
+  m1 = 0;
+  for (uint32_t b = 0; b < len; b += kUnr) {
+    float4 q[kUnr];
+#pragma unroll
+    for (int u = 0; u < kUnr; ++u) {
+      uint32_t o = b + (uint32_t)u < len ? b + (uint32_t)u : len - 1;
+      q[u] = probe[j0 + o];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnr; ++u) {
+      float dx = (qi.x - q[u].x) - (float)cell_shift(wi, __float_as_uint(q[u].w), 0) * cs_u;
+      float dy = yi - q[u].y;
+      float dz = zi - q[u].z;
       float r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 < H2p) {
-        if (cnt < cap) list[(size_t)cnt * n + i] = j;
-        ++cnt;
+      uint32_t bit = b + (uint32_t)u;
+      uint64_t hit = (r2 < H2p && bit < len) ? 1ull : 0ull;
+      if (bit < 64) m0 |= hit << bit;
+      else m1 |= hit << (bit & 63u);
+    }
+  }
+}
+
+// Visit the set bits of a 128-bit mask in ascending order, kUnr at a time:
+// fn(jj) receives run-relative indices, kPad for unused batch entries.
+template <class Fn>
+__device__ inline void walk_bits(uint64_t m0, uint64_t m1, Fn&& fn) {
+  while (m0 | m1) {
+    uint32_t jj[kUnr];
+#pragma unroll
+    for (int u = 0; u < kUnr; ++u) {
+      if (m0) {
+        jj[u] = (uint32_t)__builtin_ctzll(m0);
+        m0 &= m0 - 1;
+      } else if (m1) {
+        jj[u] = 64u + (uint32_t)__builtin_ctzll(m1);
+        m1 &= m1 - 1;
+      } else {
+        jj[u] = kPad;
       }
     }
-  });
-  return cnt;
+    fn(jj);
+  }
+}
+
+// Neighbour masks: word w of run r of slot i at masks[(2 r + w) n + i].  Runs
+// are numbered in the order for_each_run visits the existing rows of the
+// cell, so every consumer that enumerates the same cell recovers the same
+// runs.
+__device__ inline size_t mask_index(uint32_t r, uint32_t w, uint32_t n, uint32_t i) {
+  return (size_t)(2u * r + w) * n + i;
 }
 
 // ---------------------------------------------------------------------------
 // density + EOS: rho_i = m sigma sum_j f(|x_ij|/h) over j with |x_ij| < 2h
 // (j == i included), p_i = max(B((rho_i/rho0)^7 - 1), 0), pr_i = p_i / rho_i^2.
+// Per row run: probe scan -> mask (stored for the force pass) -> walk of the
+// mask's set bits in the form's precision.
 // ---------------------------------------------------------------------------
 template <int F>
 __global__ __launch_bounds__(kBlock) void k_sph_density(
     uint32_t n, const float4* __restrict__ probe, const typename SphForm<F>::G4* __restrict__ pos,
     const uint32_t* __restrict__ skeys, const uint32_t* __restrict__ lb, Grid g, SphConst kc, SphUnits u,
-    uint32_t cap, uint32_t* __restrict__ list, uint32_t* __restrict__ count,
-    typename SphForm<F>::R2* __restrict__ rp, typename SphForm<F>::real* __restrict__ pres,
-    uint32_t* flags) {
+    uint64_t* __restrict__ masks, typename SphForm<F>::R2* __restrict__ rp,
+    typename SphForm<F>::real* __restrict__ pres, uint32_t* flags) {
   using real = typename SphForm<F>::real;
   using G4 = typename SphForm<F>::G4;
   const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const CellRows cr = decode_cell(skeys[i], g);
-  uint32_t cnt = probe_scan(i, n, probe, cr, g, lb, u.cs_u, u.H2_probe, cap, list);
-  if (cnt > cap) {
-    atomicOr(&flags[FLAG_CAPACITY], 1u);
-    cnt = cap;
-  }
-  count[i] = cnt;
+  const float4 qi = probe[i];
+  const uint32_t wi = __float_as_uint(qi.w);
   const G4 pi = pos[i];
   const real H2 = (real)u.H2, inv_h = (real)u.inv_h;
   Sum<real, F == kMixed> acc;
-  for (uint32_t k = 0; k < cnt; ++k) {
-    const uint32_t j = list[(size_t)k * n + i];
-    const G4 pj = pos[j];
-    real dx, dy, dz;
-    form_pair<F>(pi, pj, u.cs_u, dx, dy, dz);
-    real r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 < H2) {
-      real q = sqrt(r2) * inv_h;
-      acc.add(spline_f(q));
+  uint32_t r = 0;
+  bool overflow = false;
+  for_each_run(cr, g, lb, [&](uint32_t j0, uint32_t j1, int ddy, int ddz) {
+    uint32_t len = j1 - j0;
+    if (len > 128u) {
+      overflow = true;
+      len = 128u;
     }
-  }
+    uint64_t m0, m1;
+    scan_run(probe, qi, wi, qi.y - (float)ddy * u.cs_u, qi.z - (float)ddz * u.cs_u, u.cs_u, u.H2_probe, j0,
+             len, m0, m1);
+    masks[mask_index(r, 0, n, i)] = m0;
+    masks[mask_index(r, 1, n, i)] = m1;
+    ++r;
+    walk_bits(m0, m1, [&](const uint32_t* jj) {
+      G4 pj[kUnr];
+#pragma unroll
+      for (int v = 0; v < kUnr; ++v) pj[v] = pos[j0 + (jj[v] == kPad ? jj[0] : jj[v])];
+#pragma unroll
+      for (int v = 0; v < kUnr; ++v) {
+        if (jj[v] == kPad) continue;
+        real dx, dy, dz;
+        form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
+        real r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < H2) {
+          real q = sqrt(r2) * inv_h;
+          acc.add(spline_f(q));
+        }
+      }
+    });
+  });
+  if (overflow) atomicOr(&flags[FLAG_CAPACITY], 1u);
   if constexpr (F == kF64) {
     double rho = kc.msig * acc.s;
     double ratio = rho * kc.inv_rho0;
@@ -269,7 +331,7 @@ __global__ __launch_bounds__(kBlock) void k_sph_force(
     uint32_t n, uint64_t n_moving, const typename SphForm<F>::G4* __restrict__ gpos,
     const typename SphForm<F>::G4* __restrict__ gvel, const typename SphForm<F>::R2* __restrict__ rp,
     const uint32_t* __restrict__ ids, const uint32_t* __restrict__ skeys,
-    const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g, SphConst kc,
+    const uint32_t* __restrict__ lb, const uint64_t* __restrict__ masks, Grid g, SphConst kc,
     SphUnits u, const typename SphForm<F>::S4* __restrict__ spos,
     const typename SphForm<F>::S4* __restrict__ svel, typename SphForm<F>::S4* __restrict__ opos,
     typename SphForm<F>::S4* __restrict__ ovel, uint32_t* __restrict__ okeys,
@@ -295,32 +357,47 @@ __global__ __launch_bounds__(kBlock) void k_sph_force(
   const real H2 = (real)u.H2, inv_h = (real)u.inv_h, eps2 = (real)u.eps2, hv = (real)u.hv;
   const real alpha_c0 = (real)kc.alpha_c0;
   Sum<real, F == kMixed> sx, sy, sz;
-  const uint32_t cnt = count[i];
-  for (uint32_t k = 0; k < cnt; ++k) {
-    const uint32_t j = list[(size_t)k * n + i];
-    const G4 pj = gpos[j];
-    real dx, dy, dz;
-    form_pair<F>(pi, pj, u.cs_u, dx, dy, dz);
-    real r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 < H2 && r2 > real(0.0)) {
-      real rr = sqrt(r2);
-      real q = rr * inv_h;
-      real dw = spline_df(q);
-      const G4 vj = gvel[j];
-      const R2 rpj = rp[j];
-      real dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
-      real vdotr = dvx * dx + dvy * dy + dvz * dz;
-      real pij = real(0.0);
-      if (vdotr < real(0.0)) {
-        real muij = hv * vdotr / (r2 + eps2);
-        pij = -alpha_c0 * muij / (real(0.5) * (rhoi + rpj.x));
+  const CellRows cr = decode_cell(skeys[i], g);
+  uint32_t r = 0;
+  for_each_run(cr, g, lb, [&](uint32_t j0, uint32_t j1, int, int) {
+    const uint64_t m0 = masks[mask_index(r, 0, n, i)], m1 = masks[mask_index(r, 1, n, i)];
+    ++r;
+    (void)j1;
+    walk_bits(m0, m1, [&](const uint32_t* jj) {
+      G4 pj[kUnr], vj[kUnr];
+      R2 rpj[kUnr];
+#pragma unroll
+      for (int v = 0; v < kUnr; ++v) {
+        const uint32_t j = j0 + (jj[v] == kPad ? jj[0] : jj[v]);
+        pj[v] = gpos[j];
+        vj[v] = gvel[j];
+        rpj[v] = rp[j];
       }
-      real sc = (pri + rpj.y + pij) * (dw / rr);
-      sx.add(sc * dx);
-      sy.add(sc * dy);
-      sz.add(sc * dz);
-    }
-  }
+#pragma unroll
+      for (int v = 0; v < kUnr; ++v) {
+        if (jj[v] == kPad) continue;
+        real dx, dy, dz;
+        form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
+        real r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < H2 && r2 > real(0.0)) {
+          real rr = sqrt(r2);
+          real q = rr * inv_h;
+          real dw = spline_df(q);
+          real dvx = vi.x - vj[v].x, dvy = vi.y - vj[v].y, dvz = vi.z - vj[v].z;
+          real vdotr = dvx * dx + dvy * dy + dvz * dz;
+          real pij = real(0.0);
+          if (vdotr < real(0.0)) {
+            real muij = hv * vdotr / (r2 + eps2);
+            pij = -alpha_c0 * muij / (real(0.5) * (rhoi + rpj[v].x));
+          }
+          real sc = (pri + rpj[v].y + pij) * (dw / rr);
+          sx.add(sc * dx);
+          sy.add(sc * dy);
+          sz.add(sc * dz);
+        }
+      }
+    });
+  });
   const S4 xs = spos[i];
   const S4 vs = svel[i];
   sreal ax, ay, az;
@@ -366,7 +443,7 @@ SphUnits sph_units(const Ctx& c) {
   // The scan's r^2 rounds differently from the pair arithmetic that defines
   // the neighbour set (another order for the mixed form, FP32 vs FP64 or
   // absolute FP32 coordinates for the others; all within ~1e-6 relative), so
-  // it keeps a 2^-10 relative margin and the list walk applies the exact test.
+  // it keeps a 2^-10 relative margin and the mask walk applies the exact test.
   const float margin = 1.0f + 1.0f / 1024.0f;
   const double cs_h = c.grid.cs * c.sph.inv_h;
   u.cs_u = (float)cs_h;
@@ -376,11 +453,13 @@ SphUnits sph_units(const Ctx& c) {
     u.inv_h = 1.0;
     u.eps2 = 0.01;
     u.hv = 1.0;
+    u.unit_scale = c.sph.inv_h;
   } else {
     u.H2 = c.sph.H2;
     u.inv_h = c.sph.inv_h;
     u.eps2 = c.sph.eps2;
     u.hv = c.sph.h;
+    u.unit_scale = 1.0;
   }
   return u;
 }
@@ -401,34 +480,32 @@ void sph_step_impl(Ctx& c) {
   const G4* gvel = (F == kMixed) ? c.vel32.as<G4>() : c.vel[c.cur].as<G4>();
   const uint32_t* skeys = c.keys[c.cur_key].as<uint32_t>();
   const uint32_t grid = xcd_grid(blocks_for(n));
-  const double ge = (double)sizeof(real);   // bytes per scalar of the pair arithmetic
-  const double se = (F == kF32) ? 4.0 : 8.0;
-  const double lb_bytes = 4.0 * (double)c.grid.n_cells / (double)n;
-  const double list_bytes = 4.0 * c.avg_neighbours;   // one u32 per listed neighbour
+  const double se = (F == kF32) ? 4.0 : 8.0;   // bytes per state scalar
+  const double ge = (double)sizeof(real);     // bytes per scalar of the pair arithmetic
   {
-    // probe 16, key 4, cell table, own position, list write + count, (rho, pr, p) out
-    double bytes = (16.0 + 4.0 + lb_bytes + (F == kMixed ? 0.0 : 3.0 * ge) + list_bytes + 4.0 + 3.0 * ge) * n;
+    // algorithmic bytes per particle: position (3 scalars; the FP32 probe for
+    // the mixed form) read once, (rho, p/rho^2, p) written once
+    double per = (F == kMixed ? 12.0 : 3.0 * ge) + 3.0 * ge;
     LaunchScope ls(&c, F == kF64 ? "sph_density_f64" : (F == kF32 ? "sph_density_f32" : "sph_density_mixed"),
-                   bytes);
+                   per * n);
     k_sph_density<F><<<grid, kBlock, 0, c.stream>>>(n, probe, gpos, skeys, c.lb.as<uint32_t>(), c.grid, c.sph, u,
-                                                    c.list_cap, c.nlist.as<uint32_t>(), c.ncount.as<uint32_t>(),
-                                                    c.rp.as<R2>(), c.pres.as<real>(), c.flags.as<uint32_t>());
+                                                    c.nmask.as<uint64_t>(), c.rp.as<R2>(), c.pres.as<real>(),
+                                                    c.flags.as<uint32_t>());
     PR_HIP(hipGetLastError());
   }
   {
-    // per moving particle: list + count, pos 3, vel 3, (rho, pr) 2 of the pair
-    // arithmetic, id, state in/out 6 + 6 (+ a 3), key out
+    // algorithmic bytes per moving particle: position, velocity (3 + 3
+    // scalars of the pair arithmetic), (rho, p/rho^2), state in (mixed: the
+    // FP64 copy) and out (6 + 6 state scalars), next key
     const double nm = (double)c.n_moving;
-    double bytes = nm * (list_bytes + 4.0 + 8.0 * ge + 4.0 + (F == kMixed ? 6.0 * se : 0.0) + 6.0 * se + 4.0 +
-                         (c.keep_acc ? 3.0 * se : 0.0)) +
-                   ((double)n - nm) * (4.0 + 2.0 * 6.0 * se + 8.0);
-    LaunchScope ls(&c, F == kF64 ? "sph_force_f64" : (F == kF32 ? "sph_force_f32" : "sph_force_mixed"), bytes);
+    double per = 8.0 * ge + (F == kMixed ? 6.0 * se : 0.0) + 6.0 * se + 4.0;
+    LaunchScope ls(&c, F == kF64 ? "sph_force_f64" : (F == kF32 ? "sph_force_f32" : "sph_force_mixed"), per * nm);
     const int dst = c.cur ^ 1;
     const int kdst = c.cur_key ^ 1;
     k_sph_force<F><<<grid, kBlock, 0, c.stream>>>(
-        n, c.n_moving, gpos, gvel, c.rp.as<R2>(), c.ids[c.cur_ids].as<uint32_t>(), skeys,
-        c.nlist.as<uint32_t>(), c.ncount.as<uint32_t>(), c.grid, c.sph, u, c.pos[c.cur].as<S4>(),
-        c.vel[c.cur].as<S4>(), c.pos[dst].as<S4>(), c.vel[dst].as<S4>(), c.keys[kdst].as<uint32_t>(),
+        n, c.n_moving, gpos, gvel, c.rp.as<R2>(), c.ids[c.cur_ids].as<uint32_t>(), skeys, c.lb.as<uint32_t>(),
+        c.nmask.as<uint64_t>(), c.grid, c.sph, u, c.pos[c.cur].as<S4>(), c.vel[c.cur].as<S4>(),
+        c.pos[dst].as<S4>(), c.vel[dst].as<S4>(), c.keys[kdst].as<uint32_t>(),
         c.keep_acc ? c.acc.as<S4>() : nullptr, c.flags.as<uint32_t>());
     PR_HIP(hipGetLastError());
     c.cur = dst;
