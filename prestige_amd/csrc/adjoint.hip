@@ -1,6 +1,7 @@
 // adjoint.hip — reverse mode: checkpoints, seeds, gradient collection, and
 // the FP64 instantiation of the WCSPH tape (mixed precision in
 // adjoint_fast.hip).  Built with -ffp-contract=off like every FP64 unit.
+#include "adjoint_dem.hpp"
 #include "adjoint_sph.hpp"
 
 namespace prestige {
@@ -117,6 +118,8 @@ void adj_grad_scatter(Ctx& c, const uint32_t* ids, const double4* vb, double* gr
 
 void sph_tape_forward_mixed(Ctx& c, uint32_t nsteps, uint32_t every);
 void sph_adjoint_mixed(Ctx& c, const double* sx, const double* sv, double* gp, double* gv);
+void dem_tape_forward_mixed(Ctx& c, uint32_t nsteps, uint32_t every);
+void dem_adjoint_mixed(Ctx& c, const double* sx, const double* sv, double* gp, double* gv);
 
 void tape_forward(Ctx& c, uint32_t nsteps, uint32_t every) {
   PR_CHECK(every >= 1, PRESTIGE_E_INVALID, "checkpoint_every must be >= 1");
@@ -125,6 +128,11 @@ void tape_forward(Ctx& c, uint32_t nsteps, uint32_t every) {
     if (c.desc.precision == PRESTIGE_F64) sph_tape_forward<kF64>(c, nsteps, every);
     else if (c.desc.precision == PRESTIGE_MIXED) sph_tape_forward_mixed(c, nsteps, every);
     else throw Error(PRESTIGE_E_UNSUPPORTED, "the WCSPH adjoint runs in F64 or MIXED precision");
+    return;
+  }
+  if (c.desc.model == PRESTIGE_MODEL_DEM) {
+    if (c.desc.precision == PRESTIGE_F64) dem_tape_forward<kF64>(c, nsteps, every);
+    else dem_tape_forward_mixed(c, nsteps, every);
     return;
   }
   throw Error(PRESTIGE_E_UNSUPPORTED, "no adjoint for this model");
@@ -137,6 +145,12 @@ void adjoint_run(Ctx& c, const double* seed_x, const double* seed_v, double* gra
     PR_CHECK(grad_params == nullptr || n_params >= 2, PRESTIGE_E_INVALID, "WCSPH has 2 parameters (alpha, c0)");
     if (c.desc.precision == PRESTIGE_F64) sph_adjoint<kF64>(c, seed_x, seed_v, grad_params, grad_v0);
     else sph_adjoint_mixed(c, seed_x, seed_v, grad_params, grad_v0);
+    return;
+  }
+  if (c.desc.model == PRESTIGE_MODEL_DEM) {
+    PR_CHECK(grad_params == nullptr || n_params >= 3, PRESTIGE_E_INVALID, "DEM has 3 parameters (kn, e, mu)");
+    if (c.desc.precision == PRESTIGE_F64) dem_adjoint<kF64, double>(c, seed_x, seed_v, grad_params, grad_v0);
+    else dem_adjoint_mixed(c, seed_x, seed_v, grad_params, grad_v0);
     return;
   }
   throw Error(PRESTIGE_E_UNSUPPORTED, "no adjoint for this model");

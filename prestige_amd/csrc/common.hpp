@@ -135,6 +135,7 @@ struct Snapshot {
 struct StepRecord {
   DevBuf gpos, gvel, gom, rp, ids, perm, skeys, lb, masks;
   DevBuf hcnt, hid, hxi, wxi;   // DEM history read by the step
+  DevBuf hcnt_out, hid_out;     // DEM contacts the step found (its frozen contact lists)
   DevBuf state_pos, state_vel, state_om;   // DEM FP64 pre-step state (sorted)
 };
 
@@ -146,6 +147,7 @@ struct Tape {
   std::unique_ptr<Snapshot> final_state;
   std::vector<std::unique_ptr<StepRecord>> rec;  // reused across segments
   DevBuf xb, vb, xb2, vb2, cvec, xbf, rhob, partial, acc, seeds, wb, wb2;
+  DevBuf hb[2], wallb[2], slot_of;   // DEM history adjoints (ping-pong), id -> slot
   void clear() {
     ckpt.clear();
     final_state.reset();
