@@ -3,12 +3,14 @@
 
 A step is one full WCSPH step of the dam-break workload (neighbour rebuild:
 cell keys, radix sort, reorder, cell table; density + Tait EOS gather;
-momentum + viscosity gather fused with the FP64 symplectic-Euler update).
-The workload is 1,000,000 fluid particles (dx = 5 mm, 100^3 block) plus the
+momentum + viscosity gather fused with the symplectic-Euler update).  The
+workload is 1,000,000 fluid particles (dx = 5 mm, 100^3 block) plus the
 799,320 particles of a 3-layer 1.6 x 0.67 x 1.2 m tank.  `value` is
 particle-updates/s = fluid particles x steps / wall time of the timed steps,
-inputs resident in HBM.  The other precision forms and the 4M mixed-precision
-workload are reported under "forms".
+inputs resident in HBM.  Also reported, under "forms": the other precision
+forms, the 4M-particle mixed-precision workload with its error against FP64
+after 1, 10, 100 and 1000 steps, and the forward + adjoint cost of the two
+differentiable workloads.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f64|f32|mixed]
 """
@@ -28,40 +30,43 @@ import numpy as np  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
 PREC = {"f64": 0, "f32": 1, "mixed": 2}
-DTYPE = {"f64": "f64", "f32": "f32", "mixed": "f32 gathers + f64 integrator"}
+DTYPE = {"f64": "f64", "f32": "f32", "mixed": "f32 pair arithmetic + f64 state/integrator"}
+SCENE_1M = dict(n_target=1_000_000, dx=0.005, seed=0)
+SCENE_4M = dict(n_target=4_000_000, dx=0.00315, seed=0)
+STATE = ("x", "y", "z", "vx", "vy", "vz")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_form(pa, scene_args, precision, steps, warmup, timing=True):
-    d, arrs = pa.scene(pa.SCENE_DAM_BREAK, **scene_args)
+def _context(pa, scene_id, scene_args, precision):
+    d, arrs = pa.scene(scene_id, **scene_args)
     d.precision = PREC[precision]
-    with pa.Context(d) as ctx:
-        for k in ("x", "y", "z", "vx", "vy", "vz"):
+    ctx = pa.Context(d)
+    for k in STATE + ("radius", "mass"):
+        if k in arrs:
             ctx.upload(k, arrs[k])
+    return d, arrs, ctx
+
+
+def run_form(pa, scene_args, precision, steps, warmup):
+    d, _, ctx = _context(pa, pa.SCENE_DAM_BREAK, scene_args, precision)
+    with ctx:
         ctx.step(warmup)
         ctx.sync()
-        if timing:
-            ctx.timing_enable(True)
+        ctx.timing_enable(True)   # per-kernel HIP events on the library's stream
         ctx.sync()
         t0 = time.perf_counter()
         ctx.step(steps)
         ctx.sync()
         t1 = time.perf_counter()
-        kt = ctx.timing() if timing else []
+        kt = ctx.timing()
         ctx.timing_enable(False)
     sec = t1 - t0
     nf = int(d.n_moving)
-    res = {
-        "value": nf * steps / sec,
-        "ms_per_step": 1e3 * sec / steps,
-        "n_fluid": nf,
-        "n_total": int(d.n_particles),
-        "kernels": kt,
-    }
-    return res
+    return {"value": nf * steps / sec, "ms_per_step": 1e3 * sec / steps, "n_fluid": nf,
+            "n_total": int(d.n_particles), "kernels": kt}
 
 
 def roofline(kernels):
@@ -70,17 +75,117 @@ def roofline(kernels):
     dom = max(kernels, key=lambda k: k["total_ms"])
     avg_s = dom["total_ms"] / dom["launches"] / 1e3
     achieved = dom["bytes_per_launch"] / avg_s / 1e9
-    return {
-        "kernel": dom["kernel"],
-        "bound": "hbm",
-        "achieved": round(achieved, 2),
-        "peak": PEAK_HBM_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / PEAK_HBM_GBS, 5),
-        "traffic": None,
-        "avg_us": round(avg_s * 1e6, 3),
-        "bytes_per_launch": dom["bytes_per_launch"],
-    }
+    return {"kernel": dom["kernel"], "bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
+            "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": None,
+            "avg_us": round(avg_s * 1e6, 3), "bytes_per_launch": dom["bytes_per_launch"]}
+
+
+def kernel_table(kernels):
+    return [{"kernel": k["kernel"], "avg_us": round(1e3 * k["total_ms"] / k["launches"], 2),
+             "launches": k["launches"], "GB_s": round(k["bytes_per_launch"] / (k["total_ms"] / k["launches"]) / 1e6, 1)}
+            for k in kernels]
+
+
+def accuracy(pa, scene_args, marks=(1, 10, 100, 1000)):
+    """Norm-wise relative error of the mixed and FP32 forms against the FP64 form (bit-identical to the
+    FP64 oracle) over fluid particles, at full size."""
+    ctxs = {}
+    for p in ("f64", "mixed", "f32"):
+        d, _, ctxs[p] = _context(pa, pa.SCENE_DAM_BREAK, scene_args, p)
+    nf = int(d.n_moving)
+    out = {"steps": list(marks), "mixed": {"rho": [], "p": [], "v": [], "x": []},
+           "f32": {"rho": [], "p": [], "v": [], "x": []}}
+    done = 0
+    try:
+        for m in marks:
+            for c in ctxs.values():
+                c.step(m - done)
+            done = m
+            ref = {k: ctxs["f64"].download(k)[:nf] for k in ("rho", "p", "vx", "vy", "vz", "x", "y", "z")}
+            for p in ("mixed", "f32"):
+                got = {k: ctxs[p].download(k)[:nf] for k in ref}
+                rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
+                out[p]["rho"].append(rel(got["rho"], ref["rho"]))
+                out[p]["p"].append(rel(got["p"], ref["p"]))
+                out[p]["v"].append(rel(np.stack([got[k] for k in ("vx", "vy", "vz")]),
+                                       np.stack([ref[k] for k in ("vx", "vy", "vz")])))
+                out[p]["x"].append(rel(np.stack([got[k] for k in "xyz"]), np.stack([ref[k] for k in "xyz"])))
+            log(f"[bench]   errors at step {m}: mixed v {out['mixed']['v'][-1]:.2e} p {out['mixed']['p'][-1]:.2e}")
+    finally:
+        for c in ctxs.values():
+            c.close()
+    return out
+
+
+def adjoint_wcsph(pa, steps=200, every=20):
+    """Differentiable mixed-precision dam break (1M fluid): forward vs forward + adjoint."""
+    from prestige_amd import losses
+
+    d, arrs, ctx = _context(pa, pa.SCENE_DAM_BREAK, SCENE_1M, "mixed")
+    nf = int(d.n_moving)
+    dx = d.h / 1.2
+    extent = (d.grid_hi[0] - 3 * dx, d.grid_hi[1] - 3 * dx)
+    x0 = np.stack([arrs[k] for k in "xyz"])
+    target = losses.dam_break_target(x0[:, :nf], extent, seed=7)
+    with ctx:
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step(steps)
+        ctx.sync()
+        t_fwd = time.perf_counter() - t0
+    d, arrs, ctx = _context(pa, pa.SCENE_DAM_BREAK, SCENE_1M, "mixed")
+    with ctx:
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.forward_tape(steps, every)
+        ctx.sync()
+        t_tape = time.perf_counter() - t0
+        x = np.stack([ctx.download(k) for k in "xyz"])
+        v = np.stack([ctx.download(k) for k in ("vx", "vy", "vz")])
+        L, gx, gv = losses.dam_break_loss(x, v, d.mass, nf, extent, target)
+        t0 = time.perf_counter()
+        gp, gv0 = ctx.adjoint(gx, gv)
+        t_adj = time.perf_counter() - t0
+    return {"workload": f"WCSPH dam break mixed, {nf} fluid, {steps} steps, checkpoint every {every}",
+            "forward_s": round(t_fwd, 4), "tape_s": round(t_tape, 4), "adjoint_s": round(t_adj, 4),
+            "fwd_adj_over_fwd": round((t_tape + t_adj) / t_fwd, 3),
+            "value": nf * steps / (t_tape + t_adj), "unit": "particle-updates/s (forward + adjoint)",
+            "loss": L, "dL_dalpha": float(gp[0]), "dL_dc0": float(gp[1]),
+            "dL_dv0_norm": float(np.linalg.norm(gv0))}
+
+
+def adjoint_dem(pa, steps=500, every=50):
+    """Differentiable DEM pour (100k polydisperse spheres, mixed): forward vs forward + adjoint."""
+    from prestige_amd import losses
+
+    d, arrs, ctx = _context(pa, pa.SCENE_DEM_POUR, dict(n_target=100_000, seed=0), "mixed")
+    target = losses.heap_target(np.stack([arrs[k] for k in "xyz"]), seed=11)
+    with ctx:
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step(steps)
+        ctx.sync()
+        t_fwd = time.perf_counter() - t0
+    d, arrs, ctx = _context(pa, pa.SCENE_DEM_POUR, dict(n_target=100_000, seed=0), "mixed")
+    with ctx:
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.forward_tape(steps, every)
+        ctx.sync()
+        t_tape = time.perf_counter() - t0
+        x = np.stack([ctx.download(k) for k in "xyz"])
+        L, gx, gv = losses.heap_loss(x, target)
+        t0 = time.perf_counter()
+        gp, gv0 = ctx.adjoint(gx, gv)
+        t_adj = time.perf_counter() - t0
+    n = int(d.n_particles)
+    return {"workload": f"DEM pour mixed, {n} spheres, {steps} steps, checkpoint every {every}",
+            "forward_s": round(t_fwd, 4), "tape_s": round(t_tape, 4), "adjoint_s": round(t_adj, 4),
+            "fwd_adj_over_fwd": round((t_tape + t_adj) / t_fwd, 3),
+            "value": n * steps / (t_tape + t_adj), "unit": "particle-updates/s (forward + adjoint)",
+            "forward_value": n * steps / t_fwd,
+            "loss": L, "dL_dkn": float(gp[0]), "dL_de": float(gp[1]), "dL_dmu": float(gp[2]),
+            "dL_dv0_norm": float(np.linalg.norm(gv0))}
 
 
 def cpu_baseline(pa, scene_args, budget_s=12.0):
@@ -98,14 +203,9 @@ def cpu_baseline(pa, scene_args, budget_s=12.0):
     st.step(k)
     sec = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-    return {
-        "value": float(d.n_moving) * k / sec,
-        "unit": "particle-updates/s",
-        "cores": cores,
-        "kind": "port",
-        "sample": f"{k} FP64 steps of the same {d.n_particles}-particle dam break "
-                  f"(after 2 warm-up steps), C oracle, OpenMP {cores} threads",
-    }
+    return {"value": float(d.n_moving) * k / sec, "unit": "particle-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{k} FP64 steps of the same {d.n_particles}-particle dam break (after 2 warm-up steps), "
+                      f"C oracle, OpenMP {cores} threads"}
 
 
 def main():
@@ -114,7 +214,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", choices=list(PREC), default="f64")
-    ap.add_argument("--no-forms", action="store_true", help="skip the other precision forms")
+    ap.add_argument("--no-forms", action="store_true", help="skip the other forms, accuracy and adjoint runs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -125,25 +225,32 @@ def main():
     import prestige_amd as pa
 
     pa.load_library()
-    scene1 = dict(n_target=1_000_000, dx=0.005, seed=0)
     log(f"[bench] main form {args.precision}: {args.steps} steps after {args.warmup} warm-up")
-    main_res = run_form(pa, scene1, args.precision, args.steps, args.warmup)
+    main_res = run_form(pa, SCENE_1M, args.precision, args.steps, args.warmup)
     forms = {}
     if not args.no_forms:
         for p in PREC:
             if p == args.precision:
                 continue
             log(f"[bench] form {p} (1M)")
-            r = run_form(pa, scene1, p, min(args.steps, 500), args.warmup)
-            forms[p] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "roofline": roofline(r["kernels"])}
+            r = run_form(pa, SCENE_1M, p, min(args.steps, 500), args.warmup)
+            forms[p] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "roofline": roofline(r["kernels"]),
+                        "kernels": kernel_table(r["kernels"])}
         log("[bench] form mixed (4M, dx = 3.15 mm)")
-        r = run_form(pa, dict(n_target=4_000_000, dx=0.00315, seed=0), "mixed", min(args.steps, 300), args.warmup)
+        r = run_form(pa, SCENE_4M, "mixed", min(args.steps, 300), args.warmup)
         forms["mixed_4M"] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "n_fluid": r["n_fluid"],
-                             "n_total": r["n_total"], "roofline": roofline(r["kernels"])}
+                             "n_total": r["n_total"], "roofline": roofline(r["kernels"]),
+                             "kernels": kernel_table(r["kernels"])}
+        log("[bench] accuracy of mixed / f32 vs f64 at 4M over 1000 steps")
+        forms["mixed_4M"]["error_vs_f64"] = accuracy(pa, SCENE_4M)
+        log("[bench] adjoint: WCSPH 1M mixed, 200 steps, K = 20")
+        forms["adjoint_wcsph_1M"] = adjoint_wcsph(pa)
+        log("[bench] adjoint: DEM pour 100k mixed, 500 steps, K = 50")
+        forms["adjoint_dem_100k"] = adjoint_dem(pa)
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")
-        cpu = cpu_baseline(pa, scene1)
+        cpu = cpu_baseline(pa, SCENE_1M)
     out = {
         "metric": "particle-updates/s/GPU",
         "value": main_res["value"],
@@ -166,8 +273,7 @@ def main():
         },
         "roofline": roofline(main_res["kernels"]),
         "cpu_baseline": cpu,
-        "kernels": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in kk.items()}
-                    for kk in main_res["kernels"]],
+        "kernels": kernel_table(main_res["kernels"]),
         "forms": forms,
     }
     if cpu:
