@@ -128,7 +128,8 @@ void prestige_destroy(prestige_ctx* ctx);
  * particle); "rho","p" (WCSPH, after a step); "ax","ay","az" accelerations of
  * the last step; "force" (eq1 output, PAIRS/any model); "key" cell key of the
  * current sort (download only, as double).  n must equal n_particles.
- * Download synchronises the context's stream. */
+ * Uploading a position component of a DEM context clears the tangential
+ * contact history.  Download synchronises the context's stream. */
 prestige_status prestige_field_upload(prestige_ctx* ctx, const char* name,
                                       const double* data, uint64_t n);
 prestige_status prestige_field_download(prestige_ctx* ctx, const char* name,

@@ -449,12 +449,11 @@ void dem_tape_forward(Ctx& c, uint32_t nsteps, uint32_t every) {
   c.tape.every = every;
   for (uint32_t s = 0; s < nsteps; ++s) {
     if (s % every == 0) {
-      c.tape.ckpt.emplace_back(new Snapshot());
-      snapshot_state(c, *c.tape.ckpt.back());
+      snapshot_state(c, c.tape.next_ckpt());
     }
     dem_step_impl<F>(c);
   }
-  c.tape.final_state.reset(new Snapshot());
+  if (!c.tape.final_state) c.tape.final_state.reset(new Snapshot());
   snapshot_state(c, *c.tape.final_state);
   c.check_flags();
   c.tape.valid = true;

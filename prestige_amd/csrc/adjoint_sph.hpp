@@ -50,25 +50,34 @@ __device__ inline real spline_d2f(real q) {
   return real(1.5) * (real(2.0) - q);
 }
 
+// Element types of the reverse sweep per form: pair scalars (ar) and the
+// per-particle seeds gathered from neighbours (C4 for c, ar for rhobar).  The
+// mixed form keeps them in FP32 like its forward; every sum is FP64.
+template <int F> struct AdjForm { using ar = double; using C4 = double4; };
+template <> struct AdjForm<kMixed> { using ar = float; using C4 = float4; };
+
 // Reverse of the symplectic-Euler update for one slot: vbt = vbar' + dt xbar',
 // c = -(m sigma/h) dt vbt; fixed particles get zero.
-static __global__ __launch_bounds__(kBlock) void k_adj_prep(uint32_t n, uint64_t n_moving,
+template <int F>
+__global__ __launch_bounds__(kBlock) void k_adj_prep(uint32_t n, uint64_t n_moving,
                                                      const uint32_t* __restrict__ ids,
                                                      const double4* __restrict__ xb,
-                                                     double4* __restrict__ vb, double4* __restrict__ cvec,
+                                                     double4* __restrict__ vb,
+                                                     typename AdjForm<F>::C4* __restrict__ cvec,
                                                      double dt, double msh) {
+  using C4 = typename AdjForm<F>::C4;
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   if ((uint64_t)ids[k] >= n_moving) {
     vb[k] = make_double4(0, 0, 0, 0);
-    cvec[k] = make_double4(0, 0, 0, 0);
+    cvec[k] = C4{0, 0, 0, 0};
     return;
   }
   double4 x = xb[k], v = vb[k];
   double4 vt = make_double4(v.x + dt * x.x, v.y + dt * x.y, v.z + dt * x.z, 0.0);
   vb[k] = vt;
   double s = -msh * dt;
-  cvec[k] = make_double4(s * vt.x, s * vt.y, s * vt.z, 0.0);
+  cvec[k] = C4{(decltype(C4{}.x))(s * vt.x), (decltype(C4{}.x))(s * vt.y), (decltype(C4{}.x))(s * vt.z), 0};
 }
 
 // Block sum of two doubles into partial[blockIdx] (deterministic).
@@ -95,17 +104,19 @@ __device__ inline void block_sum2(double a, double b, double2* __restrict__ part
 }
 
 // Force + EOS adjoint (all slots; fixed particles collect Pbar and rhobar as
-// neighbours of moving ones).  Writes xbar of the force rows (physical
-// units), adds the viscosity part of vbar, writes rhobar, and block partials
-// of (dalpha, dc0).
+// neighbours of moving ones).  Writes xbar of the force rows (metres), adds
+// the viscosity part of vbar, writes rhobar, and block partials of
+// (dalpha, dc0).
 template <int F>
 __global__ __launch_bounds__(kBlock) void k_adj_force(
     uint32_t n, uint64_t n_moving, SphStepRecord rec, Grid g, SphConst kc, SphUnits u, double alpha, double c0,
-    double rho0, const double4* __restrict__ cvec, double4* __restrict__ xbf, double4* __restrict__ vb,
-    double* __restrict__ rhob, double2* __restrict__ partial) {
+    double rho0, const typename AdjForm<F>::C4* __restrict__ cvec, double4* __restrict__ xbf,
+    double4* __restrict__ vb, typename AdjForm<F>::ar* __restrict__ rhob, double2* __restrict__ partial) {
   using real = typename SphForm<F>::real;
   using G4 = typename SphForm<F>::G4;
   using R2 = typename SphForm<F>::R2;
+  using ar = typename AdjForm<F>::ar;
+  using C4 = typename AdjForm<F>::C4;
   const G4* gpos = (const G4*)rec.gpos;
   const G4* gvel = (const G4*)rec.gvel;
   const R2* rp = (const R2*)rec.rp;
@@ -115,10 +126,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
   if (i < n) {
     const G4 pi = gpos[i], vi = gvel[i];
     const R2 rpi = rp[i];
-    const double4 ci = cvec[i];
-    const real H2 = (real)u.H2, inv_h = (real)u.inv_h, eps2 = (real)u.eps2, hv = (real)u.hv;
-    const double ac0 = kc.alpha_c0;
-    double xx = 0, xy = 0, xz = 0, vx = 0, vy = 0, vz = 0, pb = 0, rb = 0;
+    const C4 ci = cvec[i];
+    const real H2 = (real)u.H2;
+    const ar inv_h = (ar)u.inv_h, eps2 = (ar)u.eps2, hv = (ar)u.hv, ac0 = (ar)kc.alpha_c0;
+    const ar rhoi = (ar)rpi.x, pri = (ar)rpi.y;
+    const ar cix = (ar)ci.x, ciy = (ar)ci.y, ciz = (ar)ci.z;
+    double xx = 0, xy = 0, xz = 0, vx = 0, vy = 0, vz = 0, pb = 0, rb = 0, sa = 0, sc0 = 0;
     const CellRows cr = decode_cell(rec.skeys[i], g);
     uint32_t r = 0;
     for_each_run(cr, g, rec.lb, [&](uint32_t j0, uint32_t, int, int) {
@@ -127,7 +140,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
       walk_bits(m0, m1, [&](const uint32_t* jj) {
         G4 pj[kUnr], vj[kUnr];
         R2 rpj[kUnr];
-        double4 cj[kUnr];
+        C4 cj[kUnr];
 #pragma unroll
         for (int v = 0; v < kUnr; ++v) {
           const uint32_t j = j0 + (jj[v] == kPad ? jj[0] : jj[v]);
@@ -143,49 +156,49 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
           form_pair<F>(pi, pj[v], u.cs_u, dxr, dyr, dzr);
           real r2r = dxr * dxr + dyr * dyr + dzr * dzr;
           if (!(r2r < H2 && r2r > real(0.0))) continue;
-          // pair scalars in FP64 from the form's pair vector
-          const double dx = dxr, dy = dyr, dz = dzr, r2 = r2r;
-          const double rr = sqrt(r2), q = rr * (double)inv_h;
-          const double f1 = (double)spline_df((real)q), f2 = (double)spline_d2f((real)q);
-          const double gr = f1 / rr;
-          const double gp = f2 * (double)inv_h / rr - f1 / r2;
-          const double ex = (double)vi.x - (double)vj[v].x, ey = (double)vi.y - (double)vj[v].y,
-                       ez = (double)vi.z - (double)vj[v].z;
-          const double w = ex * dx + ey * dy + ez * dz;
-          const double rhoi = (double)rpi.x, rhoj = (double)rpj[v].x;
-          const double rbar = 0.5 * (rhoi + rhoj);
-          double mu = 0.0, Pi = 0.0;
-          const double D = r2 + (double)eps2;
-          if (w < 0.0) {
-            mu = (double)hv * w / D;
-            Pi = -ac0 * mu / rbar;
+          const ar dx = (ar)dxr, dy = (ar)dyr, dz = (ar)dzr, r2 = (ar)r2r;
+          const ar rr = sqrt(r2), irr = ar(1.0) / rr, q = rr * inv_h;
+          const ar f1 = spline_df(q), f2 = spline_d2f(q);
+          const ar gr = f1 * irr;
+          const ar gp = (f2 * inv_h - gr) * irr;   // d(f'(q)/r)/dr
+          const ar ex = (ar)vi.x - (ar)vj[v].x, ey = (ar)vi.y - (ar)vj[v].y, ez = (ar)vi.z - (ar)vj[v].z;
+          const ar w = ex * dx + ey * dy + ez * dz;
+          const ar D = r2 + eps2, iD = ar(1.0) / D;
+          const ar rbar = ar(0.5) * (rhoi + (ar)rpj[v].x), irb = ar(1.0) / rbar;
+          ar mu = ar(0.0), Pi = ar(0.0);
+          if (w < ar(0.0)) {
+            mu = hv * w * iD;
+            Pi = -ac0 * mu * irb;
           }
-          const double Q = (double)rpi.y + (double)rpj[v].y + Pi;
-          const double S = Q * gr;
-          const double Cx = ci.x - cj[v].x, Cy = ci.y - cj[v].y, Cz = ci.z - cj[v].z;
-          const double U = Cx * dx + Cy * dy + Cz * dz;
-          const double t = U * Q * gp / rr;
-          xx += S * Cx + t * dx;
-          xy += S * Cy + t * dy;
-          xz += S * Cz + t * dz;
-          pb += U * gr;
-          if (w < 0.0) {
-            const double mub = U * gr * (-ac0 / rbar);
-            const double hD = (double)hv / D, k2 = 2.0 * (double)hv * w / (D * D);
-            xx += mub * (hD * ex - k2 * dx);
-            xy += mub * (hD * ey - k2 * dy);
-            xz += mub * (hD * ez - k2 * dz);
-            vx += mub * hD * dx;
-            vy += mub * hD * dy;
-            vz += mub * hD * dz;
-            rb += U * gr * ac0 * mu / (2.0 * rbar * rbar);
-            const double T1 = (ci.x * dx + ci.y * dy + ci.z * dz) * gr;
-            pa += T1 * (-c0 * mu / rbar);
-            pc += T1 * (-alpha * mu / rbar);
+          const ar Q = pri + (ar)rpj[v].y + Pi;
+          const ar S = Q * gr;
+          const ar Cx = cix - (ar)cj[v].x, Cy = ciy - (ar)cj[v].y, Cz = ciz - (ar)cj[v].z;
+          const ar U = Cx * dx + Cy * dy + Cz * dz;
+          const ar Ug = U * gr;
+          const ar t = U * Q * gp * irr;
+          xx += (double)(S * Cx + t * dx);
+          xy += (double)(S * Cy + t * dy);
+          xz += (double)(S * Cz + t * dz);
+          pb += (double)Ug;
+          if (w < ar(0.0)) {
+            const ar mub = -Ug * ac0 * irb;
+            const ar hD = hv * iD, k2 = ar(2.0) * hv * w * iD * iD;
+            xx += (double)(mub * (hD * ex - k2 * dx));
+            xy += (double)(mub * (hD * ey - k2 * dy));
+            xz += (double)(mub * (hD * ez - k2 * dz));
+            vx += (double)(mub * hD * dx);
+            vy += (double)(mub * hD * dy);
+            vz += (double)(mub * hD * dz);
+            rb += (double)(Ug * ac0 * mu * ar(0.5) * irb * irb);
+            const ar T1 = (cix * dx + ciy * dy + ciz * dz) * gr * mu * irb;
+            sa -= (double)T1;    // times c0 below
+            sc0 -= (double)T1;   // times alpha below
           }
         }
       });
     });
+    pa = sa * c0;
+    pc = sc0 * alpha;
     // EOS: P = p / rho^2, p = B((rho/rho0)^7 - 1) unless the forward clamped
     // it to 0 (then P = 0 and the EOS has no derivative)
     const double rho = (double)rpi.x;
@@ -199,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
       const double dPdc0 = (r6 * ratio - 1.0) * (2.0 * rho0 * c0 / 7.0) / (rho * rho);
       pc += pb * dPdc0;
     }
-    rhob[i] = rb;
+    rhob[i] = (ar)rb;
     const double us = u.unit_scale;   // pair-geometry units -> metres for position adjoints
     xbf[i] = make_double4(us * xx, us * xy, us * xz, 0.0);
     if ((uint64_t)rec.ids[i] < n_moving) {
@@ -216,10 +229,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
 template <int F>
 __global__ __launch_bounds__(kBlock) void k_adj_density(
     uint32_t n, uint64_t n_moving, SphStepRecord rec, Grid g, SphConst kc, SphUnits u,
-    const double* __restrict__ rhob, const double4* __restrict__ xb_in, const double4* __restrict__ xbf,
-    const double4* __restrict__ vb_in, double4* __restrict__ xb_out, double4* __restrict__ vb_out) {
+    const typename AdjForm<F>::ar* __restrict__ rhob, const double4* __restrict__ xb_in,
+    const double4* __restrict__ xbf, const double4* __restrict__ vb_in, double4* __restrict__ xb_out,
+    double4* __restrict__ vb_out) {
   using real = typename SphForm<F>::real;
   using G4 = typename SphForm<F>::G4;
+  using ar = typename AdjForm<F>::ar;
   const G4* gpos = (const G4*)rec.gpos;
   const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -230,8 +245,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(
     return;
   }
   const G4 pi = gpos[i];
-  const double rbi = rhob[i];
-  const real H2 = (real)u.H2, inv_h = (real)u.inv_h;
+  const ar rbi = rhob[i];
+  const real H2 = (real)u.H2;
+  const ar inv_h = (ar)u.inv_h;
   double sx = 0, sy = 0, sz = 0;
   const CellRows cr = decode_cell(rec.skeys[i], g);
   uint32_t r = 0;
@@ -240,7 +256,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(
     ++r;
     walk_bits(m0, m1, [&](const uint32_t* jj) {
       G4 pj[kUnr];
-      double rbj[kUnr];
+      ar rbj[kUnr];
 #pragma unroll
       for (int v = 0; v < kUnr; ++v) {
         const uint32_t j = j0 + (jj[v] == kPad ? jj[0] : jj[v]);
@@ -254,16 +270,15 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(
         form_pair<F>(pi, pj[v], u.cs_u, dxr, dyr, dzr);
         real r2r = dxr * dxr + dyr * dyr + dzr * dzr;
         if (!(r2r < H2 && r2r > real(0.0))) continue;
-        const double rr = sqrt((double)r2r);
-        const double f1 = (double)spline_df((real)(rr * (double)inv_h));
-        const double s = (rbi + rbj[v]) * f1 / rr;
-        sx += s * (double)dxr;
-        sy += s * (double)dyr;
-        sz += s * (double)dzr;
+        const ar rr = sqrt((ar)r2r);
+        const ar s = (rbi + rbj[v]) * spline_df(rr * inv_h) / rr;
+        sx += (double)(s * (ar)dxr);
+        sy += (double)(s * (ar)dyr);
+        sz += (double)(s * (ar)dzr);
       }
     });
   });
-  const double k = kc.msig * (double)inv_h * u.unit_scale;
+  const double k = kc.msig * u.inv_h * u.unit_scale;
   const double4 a = xb_in[i], b = xbf[i];
   xb_out[dst] = make_double4(a.x + b.x + k * sx, a.y + b.y + k * sy, a.z + b.z + k * sz, 0.0);
   vb_out[dst] = vb_in[i];
@@ -295,8 +310,10 @@ static __global__ void k_adj_reduce(uint32_t nparts, const double2* __restrict__
 
 // Device buffers of the reverse sweep.
 struct AdjBufs {
-  double4 *xb, *vb, *xb2, *vb2, *cvec, *xbf;
-  double* rhob;
+  double4 *xb, *vb, *xb2, *vb2;
+  void* cvec;   // AdjForm<F>::C4 per slot
+  double4* xbf;
+  void* rhob;   // AdjForm<F>::ar per slot
   double2* partial;
   double* acc;
 };
@@ -308,15 +325,16 @@ void sph_reverse_step(Ctx& c, const SphStepRecord& rec, AdjBufs& A) {
   const uint32_t grid = xcd_grid(blocks_for(n));
   {
     LaunchScope ls(&c, "adj_prep", 2.0 * 32.0 * n + 32.0 * n);
-    k_adj_prep<<<blocks_for(n), kBlock, 0, c.stream>>>(n, c.n_moving, rec.ids, A.xb, A.vb, A.cvec, c.sph.dt,
-                                                       c.sph.msh);
+    k_adj_prep<F><<<blocks_for(n), kBlock, 0, c.stream>>>(n, c.n_moving, rec.ids, A.xb, A.vb,
+                                                          (typename AdjForm<F>::C4*)A.cvec, c.sph.dt, c.sph.msh);
     PR_HIP(hipGetLastError());
   }
   {
     const double ge = (double)sizeof(typename SphForm<F>::real);
     LaunchScope ls(&c, F == kF64 ? "adj_force_f64" : "adj_force_mixed", (8.0 * ge + 24.0 + 24.0 + 8.0 + 48.0) * n);
     k_adj_force<F><<<grid, kBlock, 0, c.stream>>>(n, c.n_moving, rec, c.grid, c.sph, u, c.desc.alpha, c.desc.c0,
-                                                 c.desc.rho0, A.cvec, A.xbf, A.vb, A.rhob, A.partial);
+                                                 c.desc.rho0, (const typename AdjForm<F>::C4*)A.cvec, A.xbf, A.vb,
+                                                 (typename AdjForm<F>::ar*)A.rhob, A.partial);
     PR_HIP(hipGetLastError());
     k_adj_reduce<<<1, 256, 0, c.stream>>>(grid, A.partial, A.acc);
     PR_HIP(hipGetLastError());
@@ -325,8 +343,9 @@ void sph_reverse_step(Ctx& c, const SphStepRecord& rec, AdjBufs& A) {
     const double ge = (double)sizeof(typename SphForm<F>::real);
     LaunchScope ls(&c, F == kF64 ? "adj_density_f64" : "adj_density_mixed",
                    (3.0 * ge + 8.0 + 24.0 * 3 + 24.0 * 2 + 8.0) * n);
-    k_adj_density<F><<<grid, kBlock, 0, c.stream>>>(n, c.n_moving, rec, c.grid, c.sph, u, A.rhob, A.xb, A.xbf,
-                                                   A.vb, A.xb2, A.vb2);
+    k_adj_density<F><<<grid, kBlock, 0, c.stream>>>(n, c.n_moving, rec, c.grid, c.sph, u,
+                                                   (const typename AdjForm<F>::ar*)A.rhob, A.xb, A.xbf, A.vb,
+                                                   A.xb2, A.vb2);
     PR_HIP(hipGetLastError());
   }
   std::swap(A.xb, A.xb2);
@@ -344,49 +363,63 @@ void adj_seed_gather(Ctx& c, const uint32_t* ids, const double* seed_x, const do
 void adj_grad_scatter(Ctx& c, const uint32_t* ids, const double4* vb, double* grad_v0_host);
 void ensure_adj_buffers(Ctx& c, uint32_t grid_blocks);
 
-// Copy what the reverse sweep needs from the step that just ran.
+// Hand a buffer the step just wrote to the record without copying: the next
+// step only writes into the context side of the swap before reading it.
+inline void take_buffer(DevBuf& ctx_buf, DevBuf& rec_buf) {
+  if (rec_buf.bytes != ctx_buf.bytes) rec_buf.alloc(ctx_buf.bytes);
+  std::swap(ctx_buf.p, rec_buf.p);
+}
+
+// Keep what the reverse sweep needs from the step that just ran.  Taken by
+// swap: the pair geometry (sorted pre-step state or FP32 probes), the
+// permutation, the sorted keys and the neighbour masks, all of which the
+// next step regenerates.  Copied: ids (the next reorder reads them), the cell
+// table, and (rho, p/rho^2), which stays downloadable.
 template <int F>
 void record_sph_step(Ctx& c, StepRecord& r) {
   const size_t n = c.n ? c.n : 1;
-  const size_t g4 = (F == kF64) ? sizeof(double4) : sizeof(float4);
   const size_t r2 = (F == kF64) ? sizeof(double2) : sizeof(float2);
-  r.gpos.alloc(n * g4);
-  r.gvel.alloc(n * g4);
+  const int prev = c.cur ^ 1;   // the step left its sorted input in the other buffer
+  if (F == kMixed) {
+    take_buffer(c.off32, r.gpos);
+    take_buffer(c.vel32, r.gvel);
+  } else {
+    take_buffer(c.pos[prev], r.gpos);
+    take_buffer(c.vel[prev], r.gvel);
+  }
+  take_buffer(c.vals[c.perm_buf], r.perm);
+  take_buffer(c.keys[c.cur_key ^ 1], r.skeys);
+  take_buffer(c.nmask, r.masks);
   r.rp.alloc(n * r2);
   r.ids.alloc(n * 4);
-  r.perm.alloc(n * 4);
-  r.skeys.alloc(n * 4);
   r.lb.alloc(((size_t)c.grid.n_cells + 1) * 4);
-  r.masks.alloc(n * 18 * 8);
-  const int prev = c.cur ^ 1;   // the step left its sorted input in the other buffer
-  const void* gp = (F == kMixed) ? c.off32.p : c.pos[prev].p;
-  const void* gv = (F == kMixed) ? c.vel32.p : c.vel[prev].p;
   auto cp = [&](DevBuf& d, const void* s, size_t b) {
     PR_HIP(hipMemcpyAsync(d.p, s, b, hipMemcpyDeviceToDevice, c.stream));
   };
-  cp(r.gpos, gp, c.n * g4);
-  cp(r.gvel, gv, c.n * g4);
   cp(r.rp, c.rp.p, c.n * r2);
   cp(r.ids, c.ids[c.cur_ids].p, c.n * 4);
-  cp(r.perm, c.vals[c.perm_buf].p, c.n * 4);
-  cp(r.skeys, c.keys[c.cur_key ^ 1].p, c.n * 4);
   cp(r.lb, c.lb.p, ((size_t)c.grid.n_cells + 1) * 4);
-  cp(r.masks, c.nmask.p, c.n * 18 * 8);
 }
 
 template <int F>
 void sph_tape_forward(Ctx& c, uint32_t nsteps, uint32_t every) {
-  c.tape.clear();
-  c.tape.nsteps = nsteps;
-  c.tape.every = every;
+  Tape& T = c.tape;
+  T.clear();
+  T.nsteps = nsteps;
+  T.every = every;
+  // the last segment is recorded on the way, so the reverse sweep starts
+  // without recomputing it
+  const uint32_t last0 = ((nsteps - 1) / every) * every;
+  while (T.rec.size() < every) T.rec.emplace_back(new StepRecord());
   for (uint32_t s = 0; s < nsteps; ++s) {
     if (s % every == 0) {
-      c.tape.ckpt.emplace_back(new Snapshot());
-      snapshot_state(c, *c.tape.ckpt.back());
+      snapshot_state(c, T.next_ckpt());
     }
     sph_step_impl<F>(c);
+    if (s >= last0) record_sph_step<F>(c, *T.rec[s - last0]);
   }
-  c.tape.final_state.reset(new Snapshot());
+  T.last_recorded = true;
+  if (!c.tape.final_state) c.tape.final_state.reset(new Snapshot());
   snapshot_state(c, *c.tape.final_state);
   c.check_flags();
   c.tape.valid = true;
@@ -399,20 +432,23 @@ void sph_adjoint(Ctx& c, const double* seed_x, const double* seed_v, double* gra
   const uint32_t grid = xcd_grid(blocks_for(n));
   ensure_adj_buffers(c, grid);
   AdjBufs A{T.xb.as<double4>(), T.vb.as<double4>(), T.xb2.as<double4>(), T.vb2.as<double4>(),
-            T.cvec.as<double4>(), T.xbf.as<double4>(), T.rhob.as<double>(), T.partial.as<double2>(),
+            T.cvec.p, T.xbf.as<double4>(), T.rhob.p, T.partial.as<double2>(),
             T.acc.as<double>()};
   PR_HIP(hipMemsetAsync(A.acc, 0, 2 * sizeof(double), c.stream));
   adj_seed_gather(c, T.final_state->ids.as<uint32_t>(), seed_x, seed_v);
   const uint32_t nseg = (T.nsteps + T.every - 1) / T.every;
   while (T.rec.size() < T.every) T.rec.emplace_back(new StepRecord());
   for (uint32_t seg = nseg; seg-- > 0;) {
-    restore_state(c, *T.ckpt[seg]);
     const uint32_t s0 = seg * T.every;
     const uint32_t len = (T.nsteps - s0) < T.every ? (T.nsteps - s0) : T.every;
-    for (uint32_t t = 0; t < len; ++t) {
-      sph_step_impl<F>(c);
-      record_sph_step<F>(c, *T.rec[t]);
+    if (!(seg == nseg - 1 && T.last_recorded)) {
+      restore_state(c, *T.ckpt[seg]);
+      for (uint32_t t = 0; t < len; ++t) {
+        sph_step_impl<F>(c);
+        record_sph_step<F>(c, *T.rec[t]);
+      }
     }
+    T.last_recorded = false;   // records are overwritten by the recompute of earlier segments
     for (uint32_t t = len; t-- > 0;) {
       const StepRecord& r = *T.rec[t];
       SphStepRecord sr{r.gpos.p, r.gvel.p, r.rp.p, r.ids.as<uint32_t>(), r.perm.as<uint32_t>(),

@@ -307,6 +307,13 @@ static void upload(Ctx& c, const std::string& name, const double* host, uint64_t
   if (name == "x" || name == "y" || name == "z") {
     c.keys_valid = false;
     c.sorted = false;
+    if (c.desc.model == PRESTIGE_MODEL_DEM) {
+      // new positions invalidate the tangential contact history
+      const size_t hr = c.desc.precision == PRESTIGE_MIXED ? 4 : 8;
+      PR_HIP(hipMemsetAsync(c.hcnt[c.cur_hist].p, 0, c.n * 4, c.stream));
+      PR_HIP(hipMemsetAsync(c.wxi[c.cur_hist].p, 0, c.n * 18 * hr, c.stream));
+      PR_HIP(hipStreamSynchronize(c.stream));
+    }
   }
   c.tape.clear();
 }

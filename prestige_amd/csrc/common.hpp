@@ -142,16 +142,23 @@ struct StepRecord {
 // Adjoint tape: checkpoints every `every` steps of an nsteps forward run.
 struct Tape {
   bool valid = false;
+  bool last_recorded = false;   // rec holds the last segment of the taped forward
   uint32_t nsteps = 0, every = 0;
   std::vector<std::unique_ptr<Snapshot>> ckpt;   // state before step s * every
+  size_t n_ckpt = 0;                             // checkpoints in use (ckpt keeps its buffers)
   std::unique_ptr<Snapshot> final_state;
   std::vector<std::unique_ptr<StepRecord>> rec;  // reused across segments
   DevBuf xb, vb, xb2, vb2, cvec, xbf, rhob, partial, acc, seeds, wb, wb2;
   DevBuf hb[2], wallb[2], slot_of;   // DEM history adjoints (ping-pong), id -> slot
+  // Start a new tape; checkpoint buffers are kept and reused.
   void clear() {
-    ckpt.clear();
-    final_state.reset();
+    n_ckpt = 0;
     valid = false;
+    last_recorded = false;
+  }
+  Snapshot& next_ckpt() {
+    if (n_ckpt == ckpt.size()) ckpt.emplace_back(new Snapshot());
+    return *ckpt[n_ckpt++];
   }
 };
 
