@@ -69,15 +69,34 @@ def run_form(pa, scene_args, precision, steps, warmup):
             "n_total": int(d.n_particles), "kernels": kt}
 
 
-def roofline(kernels):
+def measured_traffic():
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/)."""
+    path = ROOT / "profiles" / "pmc_traffic_wcsph_1M.md"
+    if not path.exists():
+        return {}
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes:"):
+            pairs = [p.strip().split("=") for p in line.split(":", 1)[1].split(",") if "=" in p]
+            return {k: int(v) for k, v in pairs}
+    return {}
+
+
+def roofline(kernels, workload_1m=True):
+    """Dominant kernel: algorithmic bytes per launch / its HIP-event average duration, vs 8 TB/s; traffic
+    is the rocprofv3-measured HBM bytes per launch of the same kernel on the 1M workload."""
     if not kernels:
         return None
     dom = max(kernels, key=lambda k: k["total_ms"])
     avg_s = dom["total_ms"] / dom["launches"] / 1e3
     achieved = dom["bytes_per_launch"] / avg_s / 1e9
-    return {"kernel": dom["kernel"], "bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
-            "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": None,
-            "avg_us": round(avg_s * 1e6, 3), "bytes_per_launch": dom["bytes_per_launch"]}
+    traffic = measured_traffic().get(dom["kernel"]) if workload_1m else None
+    out = {"kernel": dom["kernel"], "bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
+           "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": traffic,
+           "avg_us": round(avg_s * 1e6, 3), "bytes_per_launch": dom["bytes_per_launch"]}
+    if traffic:
+        out["traffic_GB_s"] = round(traffic / avg_s / 1e9, 1)
+        out["traffic_frac"] = round(traffic / avg_s / 1e9 / PEAK_HBM_GBS, 4)
+    return out
 
 
 def kernel_table(kernels):
@@ -199,6 +218,23 @@ def adjoint_dem(pa, steps=500, every=50):
             "dL_dv0_norm": float(np.linalg.norm(gv0))}
 
 
+def copy_probe(nbytes):
+    """hipMemcpyDtoD of nbytes (the runtime's copy kernel): reads and writes exactly nbytes of HBM."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpyDtoD(b, a, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"[bench] copy probe: 3 x {nbytes} bytes device to device")
+
+
 def cpu_baseline(pa, scene_args, budget_s=12.0):
     """FP64 oracle (OpenMP over particles) on the host cores, bounded sample of the same workload."""
     import oracle
@@ -227,6 +263,9 @@ def main():
     ap.add_argument("--precision", choices=list(PREC), default="f64")
     ap.add_argument("--no-forms", action="store_true", help="skip the other forms, accuracy and adjoint runs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="first run a plain device-to-device copy of BYTES (a reference kernel of known "
+                         "traffic for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
@@ -236,6 +275,8 @@ def main():
     import prestige_amd as pa
 
     pa.load_library()
+    if args.calibrate_copy > 0:
+        copy_probe(args.calibrate_copy)
     log(f"[bench] main form {args.precision}: {args.steps} steps after {args.warmup} warm-up")
     main_res = run_form(pa, SCENE_1M, args.precision, args.steps, args.warmup)
     forms = {}
@@ -250,7 +291,7 @@ def main():
         log("[bench] form mixed (4M, dx = 3.15 mm)")
         r = run_form(pa, SCENE_4M, "mixed", min(args.steps, 300), args.warmup)
         forms["mixed_4M"] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "n_fluid": r["n_fluid"],
-                             "n_total": r["n_total"], "roofline": roofline(r["kernels"]),
+                             "n_total": r["n_total"], "roofline": roofline(r["kernels"], workload_1m=False),
                              "kernels": kernel_table(r["kernels"])}
         log("[bench] accuracy of mixed / f32 vs f64 at 4M over 1000 steps")
         forms["mixed_4M"]["error_vs_f64"] = accuracy(pa, SCENE_4M)
