@@ -168,6 +168,12 @@ __device__ inline real spline_df(real q) {
 constexpr int kUnr = PRESTIGE_UNROLL;
 constexpr uint32_t kPad = 0xFFFFFFFFu;
 
+// Threads per block of the two forward gathers.
+#ifndef PRESTIGE_GATHER_BLOCK
+#define PRESTIGE_GATHER_BLOCK 256
+#endif
+constexpr int kGather = PRESTIGE_GATHER_BLOCK;
+
 // Scan one row run [j0, j0 + len) with the FP32 probe: bit b of the 128-bit
 // mask (m0, m1) is set when candidate j0 + b lies inside the probe cutoff.
 // Loads of a batch use clamped indices, so no load is ever out of the run.
@@ -234,7 +240,7 @@ __device__ inline size_t mask_index(uint32_t r, uint32_t w, uint32_t n, uint32_t
 // mask's set bits in the form's precision.
 // ---------------------------------------------------------------------------
 template <int F>
-__global__ __launch_bounds__(kBlock) void k_sph_density(
+__global__ __launch_bounds__(kGather) void k_sph_density(
     uint32_t n, const float4* __restrict__ probe, const typename SphForm<F>::G4* __restrict__ pos,
     const uint32_t* __restrict__ skeys, const uint32_t* __restrict__ lb, Grid g, SphConst kc, SphUnits u,
     uint64_t* __restrict__ masks, typename SphForm<F>::R2* __restrict__ rp,
@@ -327,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void k_sph_density(
 //   neighbours keep reading the pre-step state.
 // ---------------------------------------------------------------------------
 template <int F>
-__global__ __launch_bounds__(kBlock) void k_sph_force(
+__global__ __launch_bounds__(kGather) void k_sph_force(
     uint32_t n, uint64_t n_moving, const typename SphForm<F>::G4* __restrict__ gpos,
     const typename SphForm<F>::G4* __restrict__ gvel, const typename SphForm<F>::R2* __restrict__ rp,
     const uint32_t* __restrict__ ids, const uint32_t* __restrict__ skeys,
@@ -479,7 +485,7 @@ void sph_step_impl(Ctx& c) {
   const G4* gpos = (F == kMixed) ? c.off32.as<G4>() : c.pos[c.cur].as<G4>();
   const G4* gvel = (F == kMixed) ? c.vel32.as<G4>() : c.vel[c.cur].as<G4>();
   const uint32_t* skeys = c.keys[c.cur_key].as<uint32_t>();
-  const uint32_t grid = xcd_grid(blocks_for(n));
+  const uint32_t grid = xcd_grid(blocks_for(n, kGather));
   const double se = (F == kF32) ? 4.0 : 8.0;   // bytes per state scalar
   const double ge = (double)sizeof(real);     // bytes per scalar of the pair arithmetic
   {
@@ -488,7 +494,7 @@ void sph_step_impl(Ctx& c) {
     double per = (F == kMixed ? 12.0 : 3.0 * ge) + 3.0 * ge;
     LaunchScope ls(&c, F == kF64 ? "sph_density_f64" : (F == kF32 ? "sph_density_f32" : "sph_density_mixed"),
                    per * n);
-    k_sph_density<F><<<grid, kBlock, 0, c.stream>>>(n, probe, gpos, skeys, c.lb.as<uint32_t>(), c.grid, c.sph, u,
+    k_sph_density<F><<<grid, kGather, 0, c.stream>>>(n, probe, gpos, skeys, c.lb.as<uint32_t>(), c.grid, c.sph, u,
                                                     c.nmask.as<uint64_t>(), c.rp.as<R2>(), c.pres.as<real>(),
                                                     c.flags.as<uint32_t>());
     PR_HIP(hipGetLastError());
@@ -502,7 +508,7 @@ void sph_step_impl(Ctx& c) {
     LaunchScope ls(&c, F == kF64 ? "sph_force_f64" : (F == kF32 ? "sph_force_f32" : "sph_force_mixed"), per * nm);
     const int dst = c.cur ^ 1;
     const int kdst = c.cur_key ^ 1;
-    k_sph_force<F><<<grid, kBlock, 0, c.stream>>>(
+    k_sph_force<F><<<grid, kGather, 0, c.stream>>>(
         n, c.n_moving, gpos, gvel, c.rp.as<R2>(), c.ids[c.cur_ids].as<uint32_t>(), skeys, c.lb.as<uint32_t>(),
         c.nmask.as<uint64_t>(), c.grid, c.sph, u, c.pos[c.cur].as<S4>(), c.vel[c.cur].as<S4>(),
         c.pos[dst].as<S4>(), c.vel[dst].as<S4>(), c.keys[kdst].as<uint32_t>(),
