@@ -1,7 +1,7 @@
 // adjoint_sph.hpp — reverse mode of one WCSPH step, gather form.
 //
 // Forward step (sph_kernels.hpp), per moving particle k with d = x_k - x_j,
-// e = v_k - v_j, r = |d|, q = r/h over its frozen neighbour masks:
+// e = v_k - v_j, r = |d|, q = r/h over its frozen neighbour streams:
 //   rho_k = m sigma sum_j f(q),  p_k = max(B((rho_k/rho0)^7 - 1), 0),
 //   P_k = p_k / rho_k^2,  B = rho0 c0^2 / 7,
 //   a_k = g - (m sigma/h) sum_j S_kj d,  S = Q g(r),  Q = P_k + P_j + Pi,
@@ -39,9 +39,8 @@ struct SphStepRecord {
   const void* rp;         // (rho, p/rho^2): double2 / float2
   const uint32_t* ids;
   const uint32_t* perm;   // slot k of this step <- slot perm[k] of the previous order
-  const uint32_t* skeys;
-  const uint32_t* lb;
-  const uint64_t* masks;
+  const uint2* st;         // neighbour streams of the step
+  const uint32_t* scnt;
 };
 
 template <class real>
@@ -132,70 +131,64 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
     const ar rhoi = (ar)rpi.x, pri = (ar)rpi.y;
     const ar cix = (ar)ci.x, ciy = (ar)ci.y, ciz = (ar)ci.z;
     double xx = 0, xy = 0, xz = 0, vx = 0, vy = 0, vz = 0, pb = 0, rb = 0, sa = 0, sc0 = 0;
-    const CellRows cr = decode_cell(rec.skeys[i], g);
-    uint32_t r = 0;
-    for_each_run(cr, g, rec.lb, [&](uint32_t j0, uint32_t, int, int) {
-      const uint64_t m0 = rec.masks[mask_index(r, 0, n, i)], m1 = rec.masks[mask_index(r, 1, n, i)];
-      ++r;
-      walk_bits(m0, m1, [&](const uint32_t* jj) {
-        G4 pj[kUnr], vj[kUnr];
-        R2 rpj[kUnr];
-        C4 cj[kUnr];
+    walk_stream(rec.st, n, i, rec.scnt[i], [&](const uint32_t* jj) {
+      G4 pj[kUnr], vj[kUnr];
+      R2 rpj[kUnr];
+      C4 cj[kUnr];
 #pragma unroll
-        for (int v = 0; v < kUnr; ++v) {
-          const uint32_t j = j0 + (jj[v] == kPad ? jj[0] : jj[v]);
-          pj[v] = gpos[j];
-          vj[v] = gvel[j];
-          rpj[v] = rp[j];
-          cj[v] = cvec[j];
-        }
+      for (int v = 0; v < kUnr; ++v) {
+        const uint32_t j = jj[v] == kPad ? jj[0] : jj[v];
+        pj[v] = gpos[j];
+        vj[v] = gvel[j];
+        rpj[v] = rp[j];
+        cj[v] = cvec[j];
+      }
 #pragma unroll
-        for (int v = 0; v < kUnr; ++v) {
-          if (jj[v] == kPad) continue;
-          real dxr, dyr, dzr;
-          form_pair<F>(pi, pj[v], u.cs_u, dxr, dyr, dzr);
-          real r2r = dxr * dxr + dyr * dyr + dzr * dzr;
-          if (!(r2r < H2 && r2r > real(0.0))) continue;
-          const ar dx = (ar)dxr, dy = (ar)dyr, dz = (ar)dzr, r2 = (ar)r2r;
-          const ar rr = sqrt(r2), irr = ar(1.0) / rr, q = rr * inv_h;
-          const ar f1 = spline_df(q), f2 = spline_d2f(q);
-          const ar gr = f1 * irr;
-          const ar gp = (f2 * inv_h - gr) * irr;   // d(f'(q)/r)/dr
-          const ar ex = (ar)vi.x - (ar)vj[v].x, ey = (ar)vi.y - (ar)vj[v].y, ez = (ar)vi.z - (ar)vj[v].z;
-          const ar w = ex * dx + ey * dy + ez * dz;
-          const ar D = r2 + eps2, iD = ar(1.0) / D;
-          const ar rbar = ar(0.5) * (rhoi + (ar)rpj[v].x), irb = ar(1.0) / rbar;
-          ar mu = ar(0.0), Pi = ar(0.0);
-          if (w < ar(0.0)) {
-            mu = hv * w * iD;
-            Pi = -ac0 * mu * irb;
-          }
-          const ar Q = pri + (ar)rpj[v].y + Pi;
-          const ar S = Q * gr;
-          const ar Cx = cix - (ar)cj[v].x, Cy = ciy - (ar)cj[v].y, Cz = ciz - (ar)cj[v].z;
-          const ar U = Cx * dx + Cy * dy + Cz * dz;
-          const ar Ug = U * gr;
-          const ar t = U * Q * gp * irr;
-          xx += (double)(S * Cx + t * dx);
-          xy += (double)(S * Cy + t * dy);
-          xz += (double)(S * Cz + t * dz);
-          pb += (double)Ug;
-          if (w < ar(0.0)) {
-            const ar mub = -Ug * ac0 * irb;
-            const ar hD = hv * iD, k2 = ar(2.0) * hv * w * iD * iD;
-            xx += (double)(mub * (hD * ex - k2 * dx));
-            xy += (double)(mub * (hD * ey - k2 * dy));
-            xz += (double)(mub * (hD * ez - k2 * dz));
-            vx += (double)(mub * hD * dx);
-            vy += (double)(mub * hD * dy);
-            vz += (double)(mub * hD * dz);
-            rb += (double)(Ug * ac0 * mu * ar(0.5) * irb * irb);
-            const ar T1 = (cix * dx + ciy * dy + ciz * dz) * gr * mu * irb;
-            sa -= (double)T1;    // times c0 below
-            sc0 -= (double)T1;   // times alpha below
-          }
+      for (int v = 0; v < kUnr; ++v) {
+        if (jj[v] == kPad) continue;
+        real dxr, dyr, dzr;
+        form_pair<F>(pi, pj[v], u.cs_u, dxr, dyr, dzr);
+        real r2r = dxr * dxr + dyr * dyr + dzr * dzr;
+        if (!(r2r < H2 && r2r > real(0.0))) continue;
+        const ar dx = (ar)dxr, dy = (ar)dyr, dz = (ar)dzr, r2 = (ar)r2r;
+        const ar rr = sqrt(r2), irr = ar(1.0) / rr, q = rr * inv_h;
+        const ar f1 = spline_df(q), f2 = spline_d2f(q);
+        const ar gr = f1 * irr;
+        const ar gp = (f2 * inv_h - gr) * irr;   // d(f'(q)/r)/dr
+        const ar ex = (ar)vi.x - (ar)vj[v].x, ey = (ar)vi.y - (ar)vj[v].y, ez = (ar)vi.z - (ar)vj[v].z;
+        const ar w = ex * dx + ey * dy + ez * dz;
+        const ar D = r2 + eps2, iD = ar(1.0) / D;
+        const ar rbar = ar(0.5) * (rhoi + (ar)rpj[v].x), irb = ar(1.0) / rbar;
+        ar mu = ar(0.0), Pi = ar(0.0);
+        if (w < ar(0.0)) {
+          mu = hv * w * iD;
+          Pi = -ac0 * mu * irb;
         }
-      });
+        const ar Q = pri + (ar)rpj[v].y + Pi;
+        const ar S = Q * gr;
+        const ar Cx = cix - (ar)cj[v].x, Cy = ciy - (ar)cj[v].y, Cz = ciz - (ar)cj[v].z;
+        const ar U = Cx * dx + Cy * dy + Cz * dz;
+        const ar Ug = U * gr;
+        const ar t = U * Q * gp * irr;
+        xx += (double)(S * Cx + t * dx);
+        xy += (double)(S * Cy + t * dy);
+        xz += (double)(S * Cz + t * dz);
+        pb += (double)Ug;
+        if (w < ar(0.0)) {
+          const ar mub = -Ug * ac0 * irb;
+          const ar hD = hv * iD, k2 = ar(2.0) * hv * w * iD * iD;
+          xx += (double)(mub * (hD * ex - k2 * dx));
+          xy += (double)(mub * (hD * ey - k2 * dy));
+          xz += (double)(mub * (hD * ez - k2 * dz));
+          vx += (double)(mub * hD * dx);
+          vy += (double)(mub * hD * dy);
+          vz += (double)(mub * hD * dz);
+          rb += (double)(Ug * ac0 * mu * ar(0.5) * irb * irb);
+          const ar T1 = (cix * dx + ciy * dy + ciz * dz) * gr * mu * irb;
+          sa -= (double)T1;    // times c0 below
+          sc0 -= (double)T1;   // times alpha below
+        }
+      }
     });
     pa = sa * c0;
     pc = sc0 * alpha;
@@ -249,34 +242,28 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(
   const real H2 = (real)u.H2;
   const ar inv_h = (ar)u.inv_h;
   double sx = 0, sy = 0, sz = 0;
-  const CellRows cr = decode_cell(rec.skeys[i], g);
-  uint32_t r = 0;
-  for_each_run(cr, g, rec.lb, [&](uint32_t j0, uint32_t, int, int) {
-    const uint64_t m0 = rec.masks[mask_index(r, 0, n, i)], m1 = rec.masks[mask_index(r, 1, n, i)];
-    ++r;
-    walk_bits(m0, m1, [&](const uint32_t* jj) {
-      G4 pj[kUnr];
-      ar rbj[kUnr];
+  walk_stream(rec.st, n, i, rec.scnt[i], [&](const uint32_t* jj) {
+    G4 pj[kUnr];
+    ar rbj[kUnr];
 #pragma unroll
-      for (int v = 0; v < kUnr; ++v) {
-        const uint32_t j = j0 + (jj[v] == kPad ? jj[0] : jj[v]);
-        pj[v] = gpos[j];
-        rbj[v] = rhob[j];
-      }
+    for (int v = 0; v < kUnr; ++v) {
+      const uint32_t j = jj[v] == kPad ? jj[0] : jj[v];
+      pj[v] = gpos[j];
+      rbj[v] = rhob[j];
+    }
 #pragma unroll
-      for (int v = 0; v < kUnr; ++v) {
-        if (jj[v] == kPad) continue;
-        real dxr, dyr, dzr;
-        form_pair<F>(pi, pj[v], u.cs_u, dxr, dyr, dzr);
-        real r2r = dxr * dxr + dyr * dyr + dzr * dzr;
-        if (!(r2r < H2 && r2r > real(0.0))) continue;
-        const ar rr = sqrt((ar)r2r);
-        const ar s = (rbi + rbj[v]) * spline_df(rr * inv_h) / rr;
-        sx += (double)(s * (ar)dxr);
-        sy += (double)(s * (ar)dyr);
-        sz += (double)(s * (ar)dzr);
-      }
-    });
+    for (int v = 0; v < kUnr; ++v) {
+      if (jj[v] == kPad) continue;
+      real dxr, dyr, dzr;
+      form_pair<F>(pi, pj[v], u.cs_u, dxr, dyr, dzr);
+      real r2r = dxr * dxr + dyr * dyr + dzr * dzr;
+      if (!(r2r < H2 && r2r > real(0.0))) continue;
+      const ar rr = sqrt((ar)r2r);
+      const ar s = (rbi + rbj[v]) * spline_df(rr * inv_h) / rr;
+      sx += (double)(s * (ar)dxr);
+      sy += (double)(s * (ar)dyr);
+      sz += (double)(s * (ar)dzr);
+    }
   });
   const double k = kc.msig * u.inv_h * u.unit_scale;
   const double4 a = xb_in[i], b = xbf[i];
@@ -372,9 +359,9 @@ inline void take_buffer(DevBuf& ctx_buf, DevBuf& rec_buf) {
 
 // Keep what the reverse sweep needs from the step that just ran.  Taken by
 // swap: the pair geometry (sorted pre-step state or FP32 probes), the
-// permutation, the sorted keys and the neighbour masks, all of which the
-// next step regenerates.  Copied: ids (the next reorder reads them), the cell
-// table, and (rho, p/rho^2), which stays downloadable.
+// permutation and the neighbour streams, all of which the next step
+// regenerates.  Copied: ids (the next reorder reads them) and (rho, p/rho^2),
+// which stays downloadable.
 template <int F>
 void record_sph_step(Ctx& c, StepRecord& r) {
   const size_t n = c.n ? c.n : 1;
@@ -388,17 +375,15 @@ void record_sph_step(Ctx& c, StepRecord& r) {
     take_buffer(c.vel[prev], r.gvel);
   }
   take_buffer(c.vals[c.perm_buf], r.perm);
-  take_buffer(c.keys[c.cur_key ^ 1], r.skeys);
-  take_buffer(c.nmask, r.masks);
+  take_buffer(c.nstream, r.nstream);
+  take_buffer(c.ncount, r.ncount);
   r.rp.alloc(n * r2);
   r.ids.alloc(n * 4);
-  r.lb.alloc(((size_t)c.grid.n_cells + 1) * 4);
   auto cp = [&](DevBuf& d, const void* s, size_t b) {
     PR_HIP(hipMemcpyAsync(d.p, s, b, hipMemcpyDeviceToDevice, c.stream));
   };
   cp(r.rp, c.rp.p, c.n * r2);
   cp(r.ids, c.ids[c.cur_ids].p, c.n * 4);
-  cp(r.lb, c.lb.p, ((size_t)c.grid.n_cells + 1) * 4);
 }
 
 template <int F>
@@ -452,7 +437,7 @@ void sph_adjoint(Ctx& c, const double* seed_x, const double* seed_v, double* gra
     for (uint32_t t = len; t-- > 0;) {
       const StepRecord& r = *T.rec[t];
       SphStepRecord sr{r.gpos.p, r.gvel.p, r.rp.p, r.ids.as<uint32_t>(), r.perm.as<uint32_t>(),
-                       r.skeys.as<uint32_t>(), r.lb.as<uint32_t>(), r.masks.as<uint64_t>()};
+                       r.nstream.as<uint2>(), r.ncount.as<uint32_t>()};
       sph_reverse_step<F>(c, sr, A);
     }
   }

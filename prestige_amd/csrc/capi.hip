@@ -205,13 +205,19 @@ static void allocate(Ctx& c) {
   PR_HIP(hipMemsetAsync(c.rp.p, 0, nn * 2 * es, c.stream));
   PR_HIP(hipMemsetAsync(c.pres.p, 0, nn * es, c.stream));
   PR_HIP(hipMemsetAsync(c.acc.p, 0, nn * e4, c.stream));
-  if (mixed || c.desc.model == PRESTIGE_MODEL_WCSPH) c.off32.alloc(nn * sizeof(float4));
+  if (mixed || c.desc.model == PRESTIGE_MODEL_WCSPH) {
+    // padding entries: the WCSPH scan's last batch may read past the final run
+    c.off32.alloc((nn + kProbePad) * sizeof(float4));
+    PR_HIP(hipMemsetAsync(c.off32.p, 0, (nn + kProbePad) * sizeof(float4), c.stream));
+  }
   if (mixed) {
     c.vel32.alloc(nn * sizeof(float4));
     if (dem) c.om32.alloc(nn * sizeof(float4));
   }
   if (c.desc.model == PRESTIGE_MODEL_WCSPH) {
-    c.nmask.alloc(nn * 18 * 8);
+    c.nstream.alloc(nn * kStreamCap * sizeof(uint2));
+    c.ncount.alloc(nn * 4);
+    PR_HIP(hipMemsetAsync(c.ncount.p, 0, nn * 4, c.stream));
   }
   c.scratch.alloc(nn * 8);
   if (dem) {
@@ -427,6 +433,8 @@ prestige_status prestige_create(const prestige_desc* desc, prestige_ctx** out) {
       throw Error(PRESTIGE_E_INVALID, "unknown precision");
     if (desc->n_particles >= 0xFFFFFFFFull) throw Error(PRESTIGE_E_INVALID, "n_particles must be < 2^32 - 1");
     if (desc->n_moving > desc->n_particles) throw Error(PRESTIGE_E_INVALID, "n_moving > n_particles");
+    if (desc->model == PRESTIGE_MODEL_WCSPH && desc->n_particles > kMaxSphParticles)
+      throw Error(PRESTIGE_E_CAPACITY, "WCSPH holds at most 2^28 - 16 particles per context");
     if (!(desc->cell_size > 0.0) || !std::isfinite(desc->cell_size))
       throw Error(PRESTIGE_E_INVALID, "cell_size must be finite and > 0");
     if (desc->model != PRESTIGE_MODEL_PAIRS && !(desc->dt > 0.0)) throw Error(PRESTIGE_E_INVALID, "dt must be > 0");

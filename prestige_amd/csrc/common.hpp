@@ -133,7 +133,7 @@ struct Snapshot {
 // What the reverse sweep needs from one forward step: the sorted pre-step
 // pair geometry, the gather outputs, and the frozen neighbour structure.
 struct StepRecord {
-  DevBuf gpos, gvel, gom, rp, ids, perm, skeys, lb, masks;
+  DevBuf gpos, gvel, gom, rp, ids, perm, nstream, ncount;
   DevBuf hcnt, hid, hxi, wxi;   // DEM history read by the step
   DevBuf hcnt_out, hid_out;     // DEM contacts the step found (its frozen contact lists)
   DevBuf state_pos, state_vel, state_om;   // DEM FP64 pre-step state (sorted)
@@ -198,10 +198,11 @@ struct Ctx {
   // MIXED form gather copies
   DevBuf off32, vel32, om32;     // float4: offset from cell origin (cell x in .w); velocity
                                  // (mass in .w); DEM angular velocity (radius in .w)
-  // neighbour masks of the last density pass: for each of the (up to) nine
-  // row runs of a slot, 128 bits marking the run's candidates inside the
-  // cutoff; word w of run r of slot i at nmask[(2 r + w) n + i]
-  DevBuf nmask;
+  // neighbour streams of the last density pass: entry k of slot i at
+  // nstream[k n + i] is a uint2 (32-bit word of candidates inside the cutoff,
+  // slot of its bit 0), non-empty words only, in scan order; ncount[i]
+  // entries (at most kStreamCap)
+  DevBuf nstream, ncount;
   // eq1 probe
   DevBuf force, mass, scratch;
   bool have_force = false, have_mass = false;
@@ -235,14 +236,23 @@ __device__ inline uint32_t xcd_tile(uint32_t b, uint32_t grid) {
 }
 
 // The FP32 probe's .w carries the particle's cell coordinates modulo 4 (2
-// bits per axis).  Neighbours sit at a cell difference in {-1, 0, 1}, which
-// the residues recover exactly: ((c_j - c_i + 1) mod 4) - 1.
+// bits per axis: x in bits 6-7, y in 8-9, z in 10-11, bits 0-5 clear).
+// Neighbours sit at a cell difference in {-1, 0, 1}, which the residues
+// recover exactly: ((c_j - c_i + 1) mod 4) - 1.  With bits 0-5 clear, byte 0
+// of w_j - w_i + 64 is ((cx_j - cx_i + 1) mod 4) * 64, which the scan reads
+// with a single byte-to-float conversion.
+constexpr uint32_t kCodeShift = 6;
+constexpr uint32_t kProbePad = 4;                        // probe entries past the last particle
+constexpr uint32_t kMaxSphParticles = (1u << 28) - 16u;  // probe byte offsets stay below 2^32
+constexpr uint32_t kStreamCap = 36;  // neighbour stream entries per slot: 9 runs x 4 words of 32
+
 __device__ inline uint32_t cell_code(uint32_t cx, uint32_t cy, uint32_t cz) {
-  return (cx & 3u) | ((cy & 3u) << 2) | ((cz & 3u) << 4);
+  return ((cx & 3u) | ((cy & 3u) << 2) | ((cz & 3u) << 4)) << kCodeShift;
 }
 
 __device__ inline int cell_shift(uint32_t wi, uint32_t wj, int axis) {
-  uint32_t t = ((wj >> (2 * axis)) - (wi >> (2 * axis)) + 1u) & 3u;
+  const uint32_t s = kCodeShift + 2u * (uint32_t)axis;
+  uint32_t t = ((wj >> s) - (wi >> s) + 1u) & 3u;
   return (int)t - 1;
 }
 

@@ -8,21 +8,25 @@
 //
 // Two passes per step, split where the fused bodies would read rho[j] and
 // p[j] that the density body writes:
-//   density  for each of the nine row runs of the particle's cell: (a) scan
-//            the run's candidates with a compact FP32 probe of the positions
-//            (16 B per particle) and mark every candidate inside the cutoff
-//            plus a relative margin of 2^-10 in a 128-bit run mask, stored
-//            with one coalesced write per word; (b) walk the mask's set bits
-//            in the form's own precision, keep the exact r < 2h pairs and sum
-//            the density.  Then the EOS.
-//   force    walks the same masks: momentum + viscosity, symplectic Euler and
-//            the next step's cell key.
+//   density  (a) scan the candidates of the nine row runs of the particle's
+//            cell with a compact FP32 probe of the positions (16 B per
+//            particle), marking every candidate inside the cutoff plus a
+//            relative margin of 2^-10 in 32-bit words; each non-empty word is
+//            appended to the particle's neighbour stream as (word, slot of
+//            bit 0); (b) walk the stream's set bits in the form's own
+//            precision, keep the exact r < 2h pairs and sum the density.
+//            Then the EOS.
+//   force    walks the same stream: momentum + viscosity, symplectic Euler
+//            and the next step's cell key.
 // Scanning in FP32 halves the bytes each candidate costs, and the expensive
-// pair bodies only ever run over the set bits, where every lane of a
-// wavefront has work, instead of inside the 15 %-hit candidate loop.  Runs
-// in order and bits in ascending order are the scan order, so FP64 sums are
-// formed in exactly the order of the host oracle; the masks are also the
-// frozen neighbour sets the adjoint reuses.
+// pair bodies only run over the set bits.  The stream is one flat sequence
+// per particle, so a lane moves from one run to the next without waiting for
+// the other lanes of its wavefront to finish their run: the wavefront's walk
+// costs the largest total neighbour count of its lanes, not the sum over the
+// runs of each run's largest count.  Runs in order, words in order and bits
+// in ascending order are the scan order, so FP64 sums are formed in exactly
+// the order of the host oracle; the streams are also the frozen neighbour
+// sets the adjoint reuses.
 //
 // Forms:
 //   kF64   state and pair arithmetic in FP64, positions absolute.  The
@@ -159,13 +163,26 @@ __device__ inline real spline_df(real q) {
   return real(-0.75) * t * t;
 }
 
-// Candidate loads kept in flight per lane.  The gathers are latency-bound
-// (dependent loads, little arithmetic per byte), so each lane issues a batch
-// of kUnr independent loads before it uses any of them.
-#ifndef PRESTIGE_UNROLL
-#define PRESTIGE_UNROLL 2
+// Loads kept in flight per lane.  The gathers are latency-bound (dependent
+// loads, little arithmetic per byte), so each lane issues a batch of
+// independent loads before it uses any of them: kScanUnr probe candidates in
+// the density scan, kDensUnr neighbours in the density walk, kUnr neighbours
+// in the stream walks of the force pass and the adjoint.
+#ifndef PRESTIGE_SCAN_UNROLL
+#define PRESTIGE_SCAN_UNROLL 4
 #endif
+#ifndef PRESTIGE_DENSITY_UNROLL
+#define PRESTIGE_DENSITY_UNROLL 2
+#endif
+#ifndef PRESTIGE_UNROLL
+#define PRESTIGE_UNROLL 4
+#endif
+constexpr int kScanUnr = PRESTIGE_SCAN_UNROLL;
+constexpr int kDensUnr = PRESTIGE_DENSITY_UNROLL;
 constexpr int kUnr = PRESTIGE_UNROLL;
+static_assert(kScanUnr == 1 || kScanUnr == 2 || kScanUnr == 4, "scan batch divides 32");
+static_assert((uint32_t)kScanUnr <= kProbePad + 1u, "probe padding covers a scan batch's over-read");
+static_assert(kDensUnr >= 1 && kUnr >= 1, "walk batches hold at least one neighbour");
 constexpr uint32_t kPad = 0xFFFFFFFFu;
 
 // Threads per block of the two forward gathers.
@@ -174,43 +191,86 @@ constexpr uint32_t kPad = 0xFFFFFFFFu;
 #endif
 constexpr int kGather = PRESTIGE_GATHER_BLOCK;
 
-// Scan one row run [j0, j0 + len) with the FP32 probe: bit b of the 128-bit
-// mask (m0, m1) is set when candidate j0 + b lies inside the probe cutoff.
-// Loads of a batch use clamped indices, so no load is ever out of the run.
-__device__ inline void scan_run(const float4* __restrict__ probe, const float4& qi, uint32_t wi,
-                                float yi, float zi, float cs_u, float H2p, uint32_t j0, uint32_t len,
-                                uint64_t& m0, uint64_t& m1) {
-  m0 = 0;
-  m1 = 0;
-  for (uint32_t b = 0; b < len; b += kUnr) {
-    float4 q[kUnr];
+// Probe scan constants of one particle and row run: x carries the +1 cell
+// bias of the residue difference, y and z the row's cell shift.
+struct ScanProbe {
+  float xi1;     // o_x + cs/h
+  uint32_t wi1;  // w_i - 64: byte 0 of w_j - wi1 is (shift_x + 1) * 64
+  float yi, zi;  // o_y - dy cs/h, o_z - dz cs/h
+  float cs64;    // cs/h / 64
+  float H2p;     // probe cutoff^2
+};
+
+// Scan cnt <= 32 candidates starting at byte offset voff of the probe: bit b
+// of the result is set when candidate b lies inside the probe cutoff.  The
+// batch index b is the same on every lane of the wavefront, so each load is
+// the scalar base probe + b plus the lane's 32-bit byte offset.  Hits are
+// shifted in from the right (word = 2 word + hit) and the word is
+// bit-reversed once at the end.  A batch may read up to kScanUnr - 1 entries
+// past the run (the probe carries kProbePad entries past the last particle);
+// their bits are cleared.
+__device__ inline uint32_t scan_word(const float4* __restrict__ probe, uint32_t voff, uint32_t cnt,
+                                     const ScanProbe& s) {
+  uint32_t word = 0;
+  for (uint32_t b = 0; b < cnt; b += kScanUnr) {
+    float4 q[kScanUnr];
 #pragma unroll
-    for (int u = 0; u < kUnr; ++u) {
-      uint32_t o = b + (uint32_t)u < len ? b + (uint32_t)u : len - 1;
-      q[u] = probe[j0 + o];
-    }
+    for (int u = 0; u < kScanUnr; ++u)
+      q[u] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(probe + b + (uint32_t)u) + voff);
 #pragma unroll
-    for (int u = 0; u < kUnr; ++u) {
-      float dx = (qi.x - q[u].x) - (float)cell_shift(wi, __float_as_uint(q[u].w), 0) * cs_u;
-      float dy = yi - q[u].y;
-      float dz = zi - q[u].z;
-      float r2 = dx * dx + dy * dy + dz * dz;
-      uint32_t bit = b + (uint32_t)u;
-      uint64_t hit = (r2 < H2p && bit < len) ? 1ull : 0ull;
-      if (bit < 64) m0 |= hit << bit;
-      else m1 |= hit << (bit & 63u);
+    for (int u = 0; u < kScanUnr; ++u) {
+      const float t = (float)((__float_as_uint(q[u].w) - s.wi1) & 0xFFu);
+      const float dx = __builtin_fmaf(-t, s.cs64, s.xi1 - q[u].x);
+      const float dy = s.yi - q[u].y;
+      const float dz = s.zi - q[u].z;
+      const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+      word = word + word + (r2 < s.H2p ? 1u : 0u);
     }
   }
+  // word holds P = cnt rounded up to kScanUnr bits, candidate 0 in bit P - 1
+  const uint32_t P = (cnt + (uint32_t)kScanUnr - 1u) & ~((uint32_t)kScanUnr - 1u);
+  word = __builtin_bitreverse32(word) >> (32u - P);
+  return cnt < 32u ? word & ((1u << cnt) - 1u) : word;
 }
 
-// Visit the set bits of a 128-bit mask in ascending order, kUnr at a time:
-// fn(jj) receives run-relative indices, kPad for unused batch entries.
-template <class Fn>
+// Neighbour stream entries: (word, base | run << kRunShift).  Bit b of word
+// names slot (base + b); run is the index of the row run the word came from
+// (0..8, in the order for_each_run visits the existing rows).  Slots stay
+// below 2^28, so the run tag fits the top four bits.
+constexpr uint32_t kRunShift = 28;
+constexpr uint32_t kBaseMask = (1u << kRunShift) - 1u;
+
+// Scan one row run [j0, j0 + len), len <= 128, with the FP32 probe: bit b of
+// the 128-bit mask (m0, m1) is set when candidate j0 + b lies inside the
+// probe cutoff.  The run's non-empty 32-candidate words are also appended to
+// slot i's neighbour stream (entry k at st[k n + i]).
+__device__ inline void scan_run(const float4* __restrict__ probe, const ScanProbe& s, uint32_t j0,
+                                uint32_t len, uint32_t run, uint2* __restrict__ st, uint32_t n, uint32_t i,
+                                uint32_t& k, uint64_t& m0, uint64_t& m1) {
+  uint32_t wd[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (uint32_t w = 0; w < 4u; ++w) {
+    if (len > 32u * w) {
+      const uint32_t cnt = len - 32u * w < 32u ? len - 32u * w : 32u;
+      wd[w] = scan_word(probe, (j0 + 32u * w) * 16u, cnt, s);
+      if (wd[w]) {
+        st[(size_t)k * n + i] = make_uint2(wd[w], (j0 + 32u * w) | (run << kRunShift));
+        ++k;
+      }
+    }
+  }
+  m0 = (uint64_t)wd[0] | ((uint64_t)wd[1] << 32);
+  m1 = (uint64_t)wd[2] | ((uint64_t)wd[3] << 32);
+}
+
+// Visit the set bits of a 128-bit mask in ascending order, B at a time:
+// fn(jj) receives mask bit indices, kPad for unused batch entries.
+template <int B, class Fn>
 __device__ inline void walk_bits(uint64_t m0, uint64_t m1, Fn&& fn) {
   while (m0 | m1) {
-    uint32_t jj[kUnr];
+    uint32_t jj[B];
 #pragma unroll
-    for (int u = 0; u < kUnr; ++u) {
+    for (int u = 0; u < B; ++u) {
       if (m0) {
         jj[u] = (uint32_t)__builtin_ctzll(m0);
         m0 &= m0 - 1;
@@ -225,25 +285,63 @@ __device__ inline void walk_bits(uint64_t m0, uint64_t m1, Fn&& fn) {
   }
 }
 
-// Neighbour masks: word w of run r of slot i at masks[(2 r + w) n + i].  Runs
-// are numbered in the order for_each_run visits the existing rows of the
-// cell, so every consumer that enumerates the same cell recovers the same
-// runs.
-__device__ inline size_t mask_index(uint32_t r, uint32_t w, uint32_t n, uint32_t i) {
-  return (size_t)(2u * r + w) * n + i;
+// Runs walked in lockstep by a wavefront.  A lane walks the stream entries of
+// run group g = run / kWalkGroup and then waits for the other lanes before
+// group g + 1: neighbouring lanes (mostly the same cell) then load from the
+// same few cache lines.  1 = per row run, 3 = per z plane, 9 = the whole
+// stream at once (fewest idle lanes, least shared lines).
+#ifndef PRESTIGE_WALK_GROUP
+#define PRESTIGE_WALK_GROUP 9
+#endif
+constexpr uint32_t kWalkGroup = PRESTIGE_WALK_GROUP;
+static_assert(kWalkGroup == 1 || kWalkGroup == 3 || kWalkGroup == 9, "run groups split the nine runs evenly");
+
+// Neighbour stream of slot i: cnt entries at st[k n + i], non-empty and in
+// scan order.  Visit the named slots in that order, kUnr at a time: fn(jj)
+// receives slot indices, kPad for unused batch entries (jj[0] is always a
+// slot).  The next entry is loaded while the current one is walked.
+template <class Fn>
+__device__ inline void walk_stream(const uint2* __restrict__ st, uint32_t n, uint32_t i, uint32_t cnt,
+                                   Fn&& fn) {
+  uint2 cur = cnt ? st[i] : make_uint2(0u, 0u);
+  uint2 nxt = cnt > 1u ? st[(size_t)n + i] : make_uint2(0u, 0u);
+  uint32_t k = 1;  // entry index of nxt
+  for (uint32_t g = 0; g < 9u / kWalkGroup; ++g) {
+    // run group of the current entry (groups past the last for an empty one)
+    auto in_group = [&]() { return cur.x != 0u && (cur.y >> kRunShift) / kWalkGroup == g; };
+    while (in_group()) {
+      uint32_t jj[kUnr];
+#pragma unroll
+      for (int u = 0; u < kUnr; ++u) {
+        if (in_group()) {
+          jj[u] = (cur.y & kBaseMask) + (uint32_t)__builtin_ctz(cur.x);
+          cur.x &= cur.x - 1u;
+          if (!cur.x) {
+            cur = nxt;
+            ++k;
+            nxt = k < cnt ? st[(size_t)k * n + i] : make_uint2(0u, 0u);
+          }
+        } else {
+          jj[u] = kPad;
+        }
+      }
+      fn(jj);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
 // density + EOS: rho_i = m sigma sum_j f(|x_ij|/h) over j with |x_ij| < 2h
 // (j == i included), p_i = max(B((rho_i/rho0)^7 - 1), 0), pr_i = p_i / rho_i^2.
-// Per row run: probe scan -> mask (stored for the force pass) -> walk of the
-// mask's set bits in the form's precision.
+// Per row run: probe scan -> run mask, appended to the neighbour stream for
+// the force pass -> walk of the mask's set bits in the form's precision.
 // ---------------------------------------------------------------------------
 template <int F>
 __global__ __launch_bounds__(kGather) void k_sph_density(
-    uint32_t n, const float4* __restrict__ probe, const typename SphForm<F>::G4* __restrict__ pos,
+    uint32_t n, const float4* __restrict__ probe, const typename SphForm<F>::G4* pos,
     const uint32_t* __restrict__ skeys, const uint32_t* __restrict__ lb, Grid g, SphConst kc, SphUnits u,
-    uint64_t* __restrict__ masks, typename SphForm<F>::R2* __restrict__ rp,
+    uint2* __restrict__ st, uint32_t* __restrict__ scnt, typename SphForm<F>::R2* __restrict__ rp,
+    typename SphForm<F>::G4* wpos, typename SphForm<F>::G4* wvel,
     typename SphForm<F>::real* __restrict__ pres, uint32_t* flags) {
   using real = typename SphForm<F>::real;
   using G4 = typename SphForm<F>::G4;
@@ -255,26 +353,25 @@ __global__ __launch_bounds__(kGather) void k_sph_density(
   const G4 pi = pos[i];
   const real H2 = (real)u.H2, inv_h = (real)u.inv_h;
   Sum<real, F == kMixed> acc;
-  uint32_t r = 0;
+  uint32_t k = 0;
   bool overflow = false;
+  uint32_t run = 0;
   for_each_run(cr, g, lb, [&](uint32_t j0, uint32_t j1, int ddy, int ddz) {
     uint32_t len = j1 - j0;
     if (len > 128u) {
       overflow = true;
       len = 128u;
     }
+    const ScanProbe sp{qi.x + u.cs_u, wi - 64u, qi.y - (float)ddy * u.cs_u, qi.z - (float)ddz * u.cs_u,
+                       u.cs_u * (1.0f / 64.0f), u.H2_probe};
     uint64_t m0, m1;
-    scan_run(probe, qi, wi, qi.y - (float)ddy * u.cs_u, qi.z - (float)ddz * u.cs_u, u.cs_u, u.H2_probe, j0,
-             len, m0, m1);
-    masks[mask_index(r, 0, n, i)] = m0;
-    masks[mask_index(r, 1, n, i)] = m1;
-    ++r;
-    walk_bits(m0, m1, [&](const uint32_t* jj) {
-      G4 pj[kUnr];
+    scan_run(probe, sp, j0, len, run++, st, n, i, k, m0, m1);
+    walk_bits<kDensUnr>(m0, m1, [&](const uint32_t* jj) {
+      G4 pj[kDensUnr];
 #pragma unroll
-      for (int v = 0; v < kUnr; ++v) pj[v] = pos[j0 + (jj[v] == kPad ? jj[0] : jj[v])];
+      for (int v = 0; v < kDensUnr; ++v) pj[v] = pos[j0 + (jj[v] == kPad ? jj[0] : jj[v])];
 #pragma unroll
-      for (int v = 0; v < kUnr; ++v) {
+      for (int v = 0; v < kDensUnr; ++v) {
         if (jj[v] == kPad) continue;
         real dx, dy, dz;
         form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
@@ -286,6 +383,7 @@ __global__ __launch_bounds__(kGather) void k_sph_density(
       }
     });
   });
+  scnt[i] = k;
   if (overflow) atomicOr(&flags[FLAG_CAPACITY], 1u);
   if constexpr (F == kF64) {
     double rho = kc.msig * acc.s;
@@ -295,7 +393,10 @@ __global__ __launch_bounds__(kGather) void k_sph_density(
     double r7 = r4 * r2_ * ratio;
     double p = kc.B * (r7 - 1.0);
     if (p < 0.0) p = 0.0;
-    rp[i] = make_double2(rho, p / (rho * rho));
+    const double pr = p / (rho * rho);
+    rp[i] = make_double2(rho, pr);
+    reinterpret_cast<double*>(wpos + i)[3] = rho;
+    reinterpret_cast<double*>(wvel + i)[3] = pr;
     pres[i] = p;
   } else if constexpr (F == kF32) {
     float rho = (float)kc.msig * acc.s;
@@ -305,7 +406,10 @@ __global__ __launch_bounds__(kGather) void k_sph_density(
     float r7 = r4 * r2_ * ratio;
     float p = (float)kc.B * (r7 - 1.0f);
     if (p < 0.0f) p = 0.0f;
-    rp[i] = make_float2(rho, p / (rho * rho));
+    const float pr = p / (rho * rho);
+    rp[i] = make_float2(rho, pr);
+    reinterpret_cast<float*>(wpos + i)[3] = rho;
+    reinterpret_cast<float*>(wvel + i)[3] = pr;
     pres[i] = p;
   } else {
     // EOS in FP64 from the compensated sum: gamma = 7 amplifies the relative
@@ -337,7 +441,7 @@ __global__ __launch_bounds__(kGather) void k_sph_force(
     uint32_t n, uint64_t n_moving, const typename SphForm<F>::G4* __restrict__ gpos,
     const typename SphForm<F>::G4* __restrict__ gvel, const typename SphForm<F>::R2* __restrict__ rp,
     const uint32_t* __restrict__ ids, const uint32_t* __restrict__ skeys,
-    const uint32_t* __restrict__ lb, const uint64_t* __restrict__ masks, Grid g, SphConst kc,
+    const uint2* __restrict__ st, const uint32_t* __restrict__ scnt, Grid g, SphConst kc,
     SphUnits u, const typename SphForm<F>::S4* __restrict__ spos,
     const typename SphForm<F>::S4* __restrict__ svel, typename SphForm<F>::S4* __restrict__ opos,
     typename SphForm<F>::S4* __restrict__ ovel, uint32_t* __restrict__ okeys,
@@ -358,51 +462,54 @@ __global__ __launch_bounds__(kGather) void k_sph_force(
   }
   const G4 pi = gpos[i];
   const G4 vi = gvel[i];
-  const R2 rpi = rp[i];
-  const real pri = rpi.y, rhoi = rpi.x;
+  // (rho, p/rho^2) of the slot: the density pass also left them in the .w
+  // lanes of the FP64 / FP32 gather copies, so a neighbour costs two loads
+  real rhoi, pri;
+  if constexpr (F == kMixed) {
+    const R2 rpi = rp[i];
+    rhoi = rpi.x;
+    pri = rpi.y;
+  } else {
+    rhoi = pi.w;
+    pri = vi.w;
+  }
   const real H2 = (real)u.H2, inv_h = (real)u.inv_h, eps2 = (real)u.eps2, hv = (real)u.hv;
   const real alpha_c0 = (real)kc.alpha_c0;
   Sum<real, F == kMixed> sx, sy, sz;
-  const CellRows cr = decode_cell(skeys[i], g);
-  uint32_t r = 0;
-  for_each_run(cr, g, lb, [&](uint32_t j0, uint32_t j1, int, int) {
-    const uint64_t m0 = masks[mask_index(r, 0, n, i)], m1 = masks[mask_index(r, 1, n, i)];
-    ++r;
-    (void)j1;
-    walk_bits(m0, m1, [&](const uint32_t* jj) {
-      G4 pj[kUnr], vj[kUnr];
-      R2 rpj[kUnr];
+  walk_stream(st, n, i, scnt[i], [&](const uint32_t* jj) {
+    G4 pj[kUnr], vj[kUnr];
+    R2 rpj[kUnr];
 #pragma unroll
-      for (int v = 0; v < kUnr; ++v) {
-        const uint32_t j = j0 + (jj[v] == kPad ? jj[0] : jj[v]);
-        pj[v] = gpos[j];
-        vj[v] = gvel[j];
-        rpj[v] = rp[j];
-      }
+    for (int v = 0; v < kUnr; ++v) {
+      const uint32_t j = jj[v] == kPad ? jj[0] : jj[v];
+      pj[v] = gpos[j];
+      vj[v] = gvel[j];
+      if constexpr (F == kMixed) rpj[v] = rp[j];
+      else rpj[v] = R2{pj[v].w, vj[v].w};
+    }
 #pragma unroll
-      for (int v = 0; v < kUnr; ++v) {
-        if (jj[v] == kPad) continue;
-        real dx, dy, dz;
-        form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
-        real r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < H2 && r2 > real(0.0)) {
-          real rr = sqrt(r2);
-          real q = rr * inv_h;
-          real dw = spline_df(q);
-          real dvx = vi.x - vj[v].x, dvy = vi.y - vj[v].y, dvz = vi.z - vj[v].z;
-          real vdotr = dvx * dx + dvy * dy + dvz * dz;
-          real pij = real(0.0);
-          if (vdotr < real(0.0)) {
-            real muij = hv * vdotr / (r2 + eps2);
-            pij = -alpha_c0 * muij / (real(0.5) * (rhoi + rpj[v].x));
-          }
-          real sc = (pri + rpj[v].y + pij) * (dw / rr);
-          sx.add(sc * dx);
-          sy.add(sc * dy);
-          sz.add(sc * dz);
+    for (int v = 0; v < kUnr; ++v) {
+      if (jj[v] == kPad) continue;
+      real dx, dy, dz;
+      form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
+      real r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 < H2 && r2 > real(0.0)) {
+        real rr = sqrt(r2);
+        real q = rr * inv_h;
+        real dw = spline_df(q);
+        real dvx = vi.x - vj[v].x, dvy = vi.y - vj[v].y, dvz = vi.z - vj[v].z;
+        real vdotr = dvx * dx + dvy * dy + dvz * dz;
+        real pij = real(0.0);
+        if (vdotr < real(0.0)) {
+          real muij = hv * vdotr / (r2 + eps2);
+          pij = -alpha_c0 * muij / (real(0.5) * (rhoi + rpj[v].x));
         }
+        real sc = (pri + rpj[v].y + pij) * (dw / rr);
+        sx.add(sc * dx);
+        sy.add(sc * dy);
+        sz.add(sc * dz);
       }
-    });
+    }
   });
   const S4 xs = spos[i];
   const S4 vs = svel[i];
@@ -449,7 +556,7 @@ SphUnits sph_units(const Ctx& c) {
   // The scan's r^2 rounds differently from the pair arithmetic that defines
   // the neighbour set (another order for the mixed form, FP32 vs FP64 or
   // absolute FP32 coordinates for the others; all within ~1e-6 relative), so
-  // it keeps a 2^-10 relative margin and the mask walk applies the exact test.
+  // it keeps a 2^-10 relative margin and the stream walk applies the exact test.
   const float margin = 1.0f + 1.0f / 1024.0f;
   const double cs_h = c.grid.cs * c.sph.inv_h;
   u.cs_u = (float)cs_h;
@@ -486,6 +593,9 @@ void sph_step_impl(Ctx& c) {
   const G4* gvel = (F == kMixed) ? c.vel32.as<G4>() : c.vel[c.cur].as<G4>();
   const uint32_t* skeys = c.keys[c.cur_key].as<uint32_t>();
   const uint32_t grid = xcd_grid(blocks_for(n, kGather));
+  // .w lanes of the FP64 / FP32 gather copies receive (rho, p/rho^2)
+  G4* wpos = (F == kMixed) ? nullptr : c.pos[c.cur].as<G4>();
+  G4* wvel = (F == kMixed) ? nullptr : c.vel[c.cur].as<G4>();
   const double se = (F == kF32) ? 4.0 : 8.0;   // bytes per state scalar
   const double ge = (double)sizeof(real);     // bytes per scalar of the pair arithmetic
   {
@@ -495,7 +605,8 @@ void sph_step_impl(Ctx& c) {
     LaunchScope ls(&c, F == kF64 ? "sph_density_f64" : (F == kF32 ? "sph_density_f32" : "sph_density_mixed"),
                    per * n);
     k_sph_density<F><<<grid, kGather, 0, c.stream>>>(n, probe, gpos, skeys, c.lb.as<uint32_t>(), c.grid, c.sph, u,
-                                                    c.nmask.as<uint64_t>(), c.rp.as<R2>(), c.pres.as<real>(),
+                                                    c.nstream.as<uint2>(), c.ncount.as<uint32_t>(),
+                                                    c.rp.as<R2>(), wpos, wvel, c.pres.as<real>(),
                                                     c.flags.as<uint32_t>());
     PR_HIP(hipGetLastError());
   }
@@ -509,8 +620,8 @@ void sph_step_impl(Ctx& c) {
     const int dst = c.cur ^ 1;
     const int kdst = c.cur_key ^ 1;
     k_sph_force<F><<<grid, kGather, 0, c.stream>>>(
-        n, c.n_moving, gpos, gvel, c.rp.as<R2>(), c.ids[c.cur_ids].as<uint32_t>(), skeys, c.lb.as<uint32_t>(),
-        c.nmask.as<uint64_t>(), c.grid, c.sph, u, c.pos[c.cur].as<S4>(), c.vel[c.cur].as<S4>(),
+        n, c.n_moving, gpos, gvel, c.rp.as<R2>(), c.ids[c.cur_ids].as<uint32_t>(), skeys,
+        c.nstream.as<uint2>(), c.ncount.as<uint32_t>(), c.grid, c.sph, u, c.pos[c.cur].as<S4>(), c.vel[c.cur].as<S4>(),
         c.pos[dst].as<S4>(), c.vel[dst].as<S4>(), c.keys[kdst].as<uint32_t>(),
         c.keep_acc ? c.acc.as<S4>() : nullptr, c.flags.as<uint32_t>());
     PR_HIP(hipGetLastError());
