@@ -205,11 +205,7 @@ static void allocate(Ctx& c) {
   PR_HIP(hipMemsetAsync(c.rp.p, 0, nn * 2 * es, c.stream));
   PR_HIP(hipMemsetAsync(c.pres.p, 0, nn * es, c.stream));
   PR_HIP(hipMemsetAsync(c.acc.p, 0, nn * e4, c.stream));
-  if (mixed || c.desc.model == PRESTIGE_MODEL_WCSPH) {
-    // padding entries: the WCSPH scan's last batch may read past the final run
-    c.off32.alloc((nn + kProbePad) * sizeof(float4));
-    PR_HIP(hipMemsetAsync(c.off32.p, 0, (nn + kProbePad) * sizeof(float4), c.stream));
-  }
+  if (mixed) c.off32.alloc(nn * sizeof(float4));
   if (mixed) {
     c.vel32.alloc(nn * sizeof(float4));
     if (dem) c.om32.alloc(nn * sizeof(float4));

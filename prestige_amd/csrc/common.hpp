@@ -242,8 +242,7 @@ __device__ inline uint32_t xcd_tile(uint32_t b, uint32_t grid) {
 // of w_j - w_i + 64 is ((cx_j - cx_i + 1) mod 4) * 64, which the scan reads
 // with a single byte-to-float conversion.
 constexpr uint32_t kCodeShift = 6;
-constexpr uint32_t kProbePad = 4;                        // probe entries past the last particle
-constexpr uint32_t kMaxSphParticles = (1u << 28) - 16u;  // probe byte offsets stay below 2^32
+constexpr uint32_t kMaxSphParticles = (1u << 28) - 16u;  // slot indices fit the 28-bit stream base field
 constexpr uint32_t kStreamCap = 36;  // neighbour stream entries per slot: 9 runs x 4 words of 32
 
 __device__ inline uint32_t cell_code(uint32_t cx, uint32_t cy, uint32_t cz) {

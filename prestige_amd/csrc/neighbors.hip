@@ -346,14 +346,14 @@ void neighbors_build(Ctx& c) {
   const bool f32 = c.desc.precision == PRESTIGE_F32;
   {
     const double elem = f32 ? 12.0 : 24.0;   // three coordinates per vector
-    const bool probe = sph || (dem && mixed);
+    const bool probe = mixed;
     double bytes = 4.0 + 2.0 * (2.0 * elem + 4.0) + (dem ? 2.0 * elem + 16.0 : 0.0) + 4.0 * (probe ? 1.0 : 0.0) +
                    (probe ? 16.0 : 0.0) + (mixed ? 16.0 : 0.0) + (mixed && dem ? 16.0 : 0.0);
     LaunchScope ls(&c, "reorder", bytes * n);
     if (sph) {
-      if (f32) launch_reorder<float4, false, true, false>(c, perm, skeys, src, dst);
+      if (f32) launch_reorder<float4, false, false, false>(c, perm, skeys, src, dst);
       else if (mixed) launch_reorder<double4, false, true, true>(c, perm, skeys, src, dst);
-      else launch_reorder<double4, false, true, false>(c, perm, skeys, src, dst);
+      else launch_reorder<double4, false, false, false>(c, perm, skeys, src, dst);
     } else if (dem) {
       if (mixed) launch_reorder<double4, true, true, true>(c, perm, skeys, src, dst);
       else launch_reorder<double4, true, false, false>(c, perm, skeys, src, dst);

@@ -9,13 +9,13 @@
 // Two passes per step, split where the fused bodies would read rho[j] and
 // p[j] that the density body writes:
 //   density  (a) scan the candidates of the nine row runs of the particle's
-//            cell with a compact FP32 probe of the positions (16 B per
-//            particle), marking every candidate inside the cutoff plus a
-//            relative margin of 2^-10 in 32-bit words; each non-empty word is
-//            appended to the particle's neighbour stream as (word, slot of
-//            bit 0); (b) walk the stream's set bits in the form's own
-//            precision, keep the exact r < 2h pairs and sum the density.
-//            Then the EOS.
+//            cell in FP32 (offsets from a point of the wavefront, staged in
+//            LDS once per wavefront and run), marking every candidate inside
+//            the cutoff plus a small relative margin in 32-bit words; each
+//            non-empty word is appended to the particle's neighbour stream
+//            as (word, slot of bit 0); (b) walk the run's set bits in the
+//            form's own precision, keep the exact r < 2h pairs and sum the
+//            density.  Then the EOS.
 //   force    walks the same stream: momentum + viscosity, symplectic Euler
 //            and the next step's cell key.
 // Scanning in FP32 halves the bytes each candidate costs, and the expensive
@@ -67,7 +67,8 @@ struct SphUnits {
   double eps2;     // 0.01 h^2 (0.01 in units of h)
   double hv;       // viscosity mu = hv * (v.r) / (r^2 + eps2): h, or 1 in units of h
   float cs_u;      // cell size in probe units (cs / h)
-  float H2_probe;  // probe cutoff^2 in units of h, with margin for forms that recheck
+  float H2_probe;  // scan cutoff^2 in units of h, with the margin the scan's rounding needs
+  double scan_inv_h;  // 1/h: scan offsets are (x - reference) / h
   double unit_scale;  // d(pair units)/d(metres): 1/h for the mixed form, 1 otherwise
 };
 
@@ -169,7 +170,7 @@ __device__ inline real spline_df(real q) {
 // the density scan, kDensUnr neighbours in the density walk, kUnr neighbours
 // in the stream walks of the force pass and the adjoint.
 #ifndef PRESTIGE_SCAN_UNROLL
-#define PRESTIGE_SCAN_UNROLL 4
+#define PRESTIGE_SCAN_UNROLL 2
 #endif
 #ifndef PRESTIGE_DENSITY_UNROLL
 #define PRESTIGE_DENSITY_UNROLL 2
@@ -181,7 +182,6 @@ constexpr int kScanUnr = PRESTIGE_SCAN_UNROLL;
 constexpr int kDensUnr = PRESTIGE_DENSITY_UNROLL;
 constexpr int kUnr = PRESTIGE_UNROLL;
 static_assert(kScanUnr == 1 || kScanUnr == 2 || kScanUnr == 4, "scan batch divides 32");
-static_assert((uint32_t)kScanUnr <= kProbePad + 1u, "probe padding covers a scan batch's over-read");
 static_assert(kDensUnr >= 1 && kUnr >= 1, "walk batches hold at least one neighbour");
 constexpr uint32_t kPad = 0xFFFFFFFFu;
 
@@ -191,38 +191,35 @@ constexpr uint32_t kPad = 0xFFFFFFFFu;
 #endif
 constexpr int kGather = PRESTIGE_GATHER_BLOCK;
 
-// Probe scan constants of one particle and row run: x carries the +1 cell
-// bias of the residue difference, y and z the row's cell shift.
-struct ScanProbe {
-  float xi1;     // o_x + cs/h
-  uint32_t wi1;  // w_i - 64: byte 0 of w_j - wi1 is (shift_x + 1) * 64
-  float yi, zi;  // o_y - dy cs/h, o_z - dz cs/h
-  float cs64;    // cs/h / 64
-  float H2p;     // probe cutoff^2
+// Candidate coordinates for the scan: offsets from a reference point of the
+// wavefront (the position of its first slot) in units of h, rounded once to
+// FP32.  Every neighbour lies within a cell of its particle, so the scan
+// needs no per-candidate cell bookkeeping: x_i - x_j is one subtraction.
+// The rounding is covered by the probe margin (sph_units), which grows with
+// the largest coordinate magnitude the grid allows.
+struct ScanPoint {
+  float x, y, z;  // this slot's offset from the reference
+  float H2p;      // probe cutoff^2 in units of h
 };
 
-// Scan cnt <= 32 candidates starting at byte offset voff of the probe: bit b
-// of the result is set when candidate b lies inside the probe cutoff.  The
-// batch index b is the same on every lane of the wavefront, so each load is
-// the scalar base probe + b plus the lane's 32-bit byte offset.  Hits are
-// shifted in from the right (word = 2 word + hit) and the word is
-// bit-reversed once at the end.  A batch may read up to kScanUnr - 1 entries
-// past the run (the probe carries kProbePad entries past the last particle);
-// their bits are cleared.
-__device__ inline uint32_t scan_word(const float4* __restrict__ probe, uint32_t voff, uint32_t cnt,
-                                     const ScanProbe& s) {
+// Scan cnt <= 32 candidates k0, k0 + 1, ... whose offsets ld(k) returns:
+// bit b of the result is set when candidate k0 + b lies inside the probe
+// cutoff.  Hits are shifted in from the right (word = 2 word + hit) and the
+// word is bit-reversed once at the end.  A batch may read up to
+// kScanUnr - 1 candidates past cnt; ld must accept them, and their bits are
+// cleared.
+template <class Ld>
+__device__ inline uint32_t scan_word(Ld&& ld, uint32_t k0, uint32_t cnt, const ScanPoint& s) {
   uint32_t word = 0;
   for (uint32_t b = 0; b < cnt; b += kScanUnr) {
     float4 q[kScanUnr];
 #pragma unroll
-    for (int u = 0; u < kScanUnr; ++u)
-      q[u] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(probe + b + (uint32_t)u) + voff);
+    for (int u = 0; u < kScanUnr; ++u) q[u] = ld(k0 + b + (uint32_t)u);
 #pragma unroll
     for (int u = 0; u < kScanUnr; ++u) {
-      const float t = (float)((__float_as_uint(q[u].w) - s.wi1) & 0xFFu);
-      const float dx = __builtin_fmaf(-t, s.cs64, s.xi1 - q[u].x);
-      const float dy = s.yi - q[u].y;
-      const float dz = s.zi - q[u].z;
+      const float dx = s.x - q[u].x;
+      const float dy = s.y - q[u].y;
+      const float dz = s.z - q[u].z;
       const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
       word = word + word + (r2 < s.H2p ? 1u : 0u);
     }
@@ -240,19 +237,21 @@ __device__ inline uint32_t scan_word(const float4* __restrict__ probe, uint32_t 
 constexpr uint32_t kRunShift = 28;
 constexpr uint32_t kBaseMask = (1u << kRunShift) - 1u;
 
-// Scan one row run [j0, j0 + len), len <= 128, with the FP32 probe: bit b of
-// the 128-bit mask (m0, m1) is set when candidate j0 + b lies inside the
-// probe cutoff.  The run's non-empty 32-candidate words are also appended to
-// slot i's neighbour stream (entry k at st[k n + i]).
-__device__ inline void scan_run(const float4* __restrict__ probe, const ScanProbe& s, uint32_t j0,
-                                uint32_t len, uint32_t run, uint2* __restrict__ st, uint32_t n, uint32_t i,
-                                uint32_t& k, uint64_t& m0, uint64_t& m1) {
+// Scan one row run [j0, j0 + len), len <= 128: bit b of the 128-bit mask
+// (m0, m1) is set when candidate j0 + b lies inside the probe cutoff; ld(j)
+// returns the scan offsets of slot j.  The run's non-empty 32-candidate
+// words are also appended to slot i's neighbour stream (entry k at
+// st[k n + i]).
+template <class Ld>
+__device__ inline void scan_run(Ld&& ld, const ScanPoint& s, uint32_t j0, uint32_t len, uint32_t run,
+                                uint2* __restrict__ st, uint32_t n, uint32_t i, uint32_t& k, uint64_t& m0,
+                                uint64_t& m1) {
   uint32_t wd[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
   for (uint32_t w = 0; w < 4u; ++w) {
     if (len > 32u * w) {
       const uint32_t cnt = len - 32u * w < 32u ? len - 32u * w : 32u;
-      wd[w] = scan_word(probe, (j0 + 32u * w) * 16u, cnt, s);
+      wd[w] = scan_word(ld, j0 + 32u * w, cnt, s);
       if (wd[w]) {
         st[(size_t)k * n + i] = make_uint2(wd[w], (j0 + 32u * w) | (run << kRunShift));
         ++k;
@@ -335,54 +334,132 @@ __device__ inline void walk_stream(const uint2* __restrict__ st, uint32_t n, uin
 // (j == i included), p_i = max(B((rho_i/rho0)^7 - 1), 0), pr_i = p_i / rho_i^2.
 // Per row run: probe scan -> run mask, appended to the neighbour stream for
 // the force pass -> walk of the mask's set bits in the form's precision.
+//
+// One wavefront per block.  For each row run the wavefront first copies the
+// union of its lanes' candidate ranges into LDS as FP32 scan offsets (one
+// coalesced pass; the lanes of a wavefront share most candidates), and every
+// lane scans its own range from there.  A union larger than kStageCap slots
+// (a wavefront spread over distant rows) is scanned from global memory.
 // ---------------------------------------------------------------------------
+constexpr int kDensBlock = 64;  // one wavefront: block barriers are wave barriers
+#ifndef PRESTIGE_STAGE_CAP
+#define PRESTIGE_STAGE_CAP 256
+#endif
+constexpr uint32_t kStageCap = PRESTIGE_STAGE_CAP;
+
+__device__ inline uint32_t wave_min(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const uint32_t t = __shfl_xor(v, d, 64);
+    v = t < v ? t : v;
+  }
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+__device__ inline uint32_t wave_max(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const uint32_t t = __shfl_xor(v, d, 64);
+    v = t > v ? t : v;
+  }
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
 template <int F>
-__global__ __launch_bounds__(kGather) void k_sph_density(
-    uint32_t n, const float4* __restrict__ probe, const typename SphForm<F>::G4* pos,
+__global__ __launch_bounds__(kDensBlock) void k_sph_density(
+    uint32_t n, const typename SphForm<F>::S4* __restrict__ spos, const typename SphForm<F>::G4* pos,
     const uint32_t* __restrict__ skeys, const uint32_t* __restrict__ lb, Grid g, SphConst kc, SphUnits u,
     uint2* __restrict__ st, uint32_t* __restrict__ scnt, typename SphForm<F>::R2* __restrict__ rp,
     typename SphForm<F>::G4* wpos, typename SphForm<F>::G4* wvel,
     typename SphForm<F>::real* __restrict__ pres, uint32_t* flags) {
   using real = typename SphForm<F>::real;
   using G4 = typename SphForm<F>::G4;
-  const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const CellRows cr = decode_cell(skeys[i], g);
-  const float4 qi = probe[i];
-  const uint32_t wi = __float_as_uint(qi.w);
-  const G4 pi = pos[i];
+  using S4 = typename SphForm<F>::S4;
+  __shared__ float4 s_q[kStageCap + kScanUnr];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t wbase = xcd_tile(blockIdx.x, gridDim.x) * kDensBlock;
+  const uint32_t i = wbase + lane;
+  const bool live = i < n;
+  // reference point of the wavefront (n >= 1 whenever the kernel runs)
+  const S4 p0 = spos[wbase < n ? wbase : n - 1];
+  const double xr = (double)p0.x, yr = (double)p0.y, zr = (double)p0.z, ih = u.scan_inv_h;
+  auto offset = [&](const S4& p) {
+    return make_float4((float)(((double)p.x - xr) * ih), (float)(((double)p.y - yr) * ih),
+                       (float)(((double)p.z - zr) * ih), 0.0f);
+  };
+  ScanPoint sp{0.0f, 0.0f, 0.0f, u.H2_probe};
+  CellRows cr{};
+  G4 pi{};
+  if (live) {
+    const float4 o = offset(spos[i]);
+    sp.x = o.x;
+    sp.y = o.y;
+    sp.z = o.z;
+    cr = decode_cell(skeys[i], g);
+    pi = pos[i];
+  }
   const real H2 = (real)u.H2, inv_h = (real)u.inv_h;
   Sum<real, F == kMixed> acc;
   uint32_t k = 0;
   bool overflow = false;
   uint32_t run = 0;
-  for_each_run(cr, g, lb, [&](uint32_t j0, uint32_t j1, int ddy, int ddz) {
-    uint32_t len = j1 - j0;
-    if (len > 128u) {
-      overflow = true;
-      len = 128u;
-    }
-    const ScanProbe sp{qi.x + u.cs_u, wi - 64u, qi.y - (float)ddy * u.cs_u, qi.z - (float)ddz * u.cs_u,
-                       u.cs_u * (1.0f / 64.0f), u.H2_probe};
-    uint64_t m0, m1;
-    scan_run(probe, sp, j0, len, run++, st, n, i, k, m0, m1);
-    walk_bits<kDensUnr>(m0, m1, [&](const uint32_t* jj) {
-      G4 pj[kDensUnr];
-#pragma unroll
-      for (int v = 0; v < kDensUnr; ++v) pj[v] = pos[j0 + (jj[v] == kPad ? jj[0] : jj[v])];
-#pragma unroll
-      for (int v = 0; v < kDensUnr; ++v) {
-        if (jj[v] == kPad) continue;
-        real dx, dy, dz;
-        form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
-        real r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < H2) {
-          real q = sqrt(r2) * inv_h;
-          acc.add(spline_f(q));
+  const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
+  for (int ddz = -1; ddz <= 1; ++ddz) {
+    for (int ddy = -1; ddy <= 1; ++ddy) {
+      const int zz = cr.cz + ddz, yy = cr.cy + ddy;
+      const bool row_ok = live && zz >= 0 && zz < nz && yy >= 0 && yy < ny;
+      uint32_t j0 = 0, len = 0;
+      if (row_ok) {
+        const uint32_t row = ((uint32_t)zz * (uint32_t)ny + (uint32_t)yy) * (uint32_t)nx;
+        j0 = lb[row + (uint32_t)cr.x0];
+        len = lb[row + (uint32_t)cr.x1 + 1] - j0;
+        if (len > 128u) {
+          overflow = true;
+          len = 128u;
         }
       }
-    });
-  });
+      // union of the wavefront's candidate ranges of this run
+      const uint32_t base = wave_min(len ? j0 : 0xFFFFFFFFu);
+      const uint32_t top = wave_max(len ? j0 + len : 0u);
+      if (top <= base) {
+        if (row_ok) ++run;
+        continue;
+      }
+      const bool staged = top - base <= kStageCap;
+      if (staged) {
+        for (uint32_t kk = lane; kk < top - base; kk += kDensBlock) s_q[kk] = offset(spos[base + kk]);
+        __syncthreads();
+      }
+      if (len) {
+        uint64_t m0, m1;
+        if (staged) {
+          scan_run([&](uint32_t j) { return s_q[j - base]; }, sp, j0, len, run, st, n, i, k, m0, m1);
+        } else {
+          scan_run([&](uint32_t j) { return offset(spos[j < n ? j : n - 1]); }, sp, j0, len, run, st, n, i, k,
+                   m0, m1);
+        }
+        walk_bits<kDensUnr>(m0, m1, [&](const uint32_t* jj) {
+          G4 pj[kDensUnr];
+#pragma unroll
+          for (int v = 0; v < kDensUnr; ++v) pj[v] = pos[j0 + (jj[v] == kPad ? jj[0] : jj[v])];
+#pragma unroll
+          for (int v = 0; v < kDensUnr; ++v) {
+            if (jj[v] == kPad) continue;
+            real dx, dy, dz;
+            form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
+            real r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < H2) {
+              real q = sqrt(r2) * inv_h;
+              acc.add(spline_f(q));
+            }
+          }
+        });
+      }
+      if (row_ok) ++run;
+      if (staged) __syncthreads();  // the next run's staging overwrites s_q
+    }
+  }
+  if (!live) return;
   scnt[i] = k;
   if (overflow) atomicOr(&flags[FLAG_CAPACITY], 1u);
   if constexpr (F == kF64) {
@@ -553,14 +630,25 @@ __global__ __launch_bounds__(kGather) void k_sph_force(
 template <int F>
 SphUnits sph_units(const Ctx& c) {
   SphUnits u;
-  // The scan's r^2 rounds differently from the pair arithmetic that defines
-  // the neighbour set (another order for the mixed form, FP32 vs FP64 or
-  // absolute FP32 coordinates for the others; all within ~1e-6 relative), so
-  // it keeps a 2^-10 relative margin and the stream walk applies the exact test.
-  const float margin = 1.0f + 1.0f / 1024.0f;
+  // The scan only has to keep every pair the exact test of the walk keeps.
+  // Its FP32 offsets are rounded from values of magnitude at most E (the
+  // grid's extent plus its largest coordinate, in units of h; the second term
+  // covers the absolute FP32 coordinates of the F32 form's own test), so an
+  // r^2 near the cutoff 4 is off by at most about 4 E 2^-24 relative.  The
+  // margin is 2^-10 plus four times that bound; the walk then applies the
+  // exact test.
+  double E = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double lo = c.grid.lo[a], hi = lo + c.grid.dims[a] * c.grid.cs;
+    const double e = (std::fabs(lo) > std::fabs(hi) ? std::fabs(lo) : std::fabs(hi)) + (hi - lo);
+    E = e > E ? e : E;
+  }
+  E *= c.sph.inv_h;
+  const double margin = 1.0 + 1.0 / 1024.0 + 16.0 * E * 0x1.0p-24;
   const double cs_h = c.grid.cs * c.sph.inv_h;
   u.cs_u = (float)cs_h;
-  u.H2_probe = 4.0f * margin;
+  u.H2_probe = (float)(4.0 * margin);
+  u.scan_inv_h = c.sph.inv_h;
   if constexpr (F == kMixed) {
     u.H2 = 4.0;
     u.inv_h = 1.0;
@@ -588,7 +676,6 @@ void sph_step_impl(Ctx& c) {
   const uint32_t n = (uint32_t)c.n;
   if (!n) return;
   const SphUnits u = sph_units<F>(c);
-  const float4* probe = c.off32.as<float4>();
   const G4* gpos = (F == kMixed) ? c.off32.as<G4>() : c.pos[c.cur].as<G4>();
   const G4* gvel = (F == kMixed) ? c.vel32.as<G4>() : c.vel[c.cur].as<G4>();
   const uint32_t* skeys = c.keys[c.cur_key].as<uint32_t>();
@@ -604,10 +691,11 @@ void sph_step_impl(Ctx& c) {
     double per = (F == kMixed ? 12.0 : 3.0 * ge) + 3.0 * ge;
     LaunchScope ls(&c, F == kF64 ? "sph_density_f64" : (F == kF32 ? "sph_density_f32" : "sph_density_mixed"),
                    per * n);
-    k_sph_density<F><<<grid, kGather, 0, c.stream>>>(n, probe, gpos, skeys, c.lb.as<uint32_t>(), c.grid, c.sph, u,
-                                                    c.nstream.as<uint2>(), c.ncount.as<uint32_t>(),
-                                                    c.rp.as<R2>(), wpos, wvel, c.pres.as<real>(),
-                                                    c.flags.as<uint32_t>());
+    const uint32_t dgrid = xcd_grid(blocks_for(n, kDensBlock));
+    k_sph_density<F><<<dgrid, kDensBlock, 0, c.stream>>>(n, c.pos[c.cur].as<S4>(), gpos, skeys, c.lb.as<uint32_t>(),
+                                                        c.grid, c.sph, u, c.nstream.as<uint2>(),
+                                                        c.ncount.as<uint32_t>(), c.rp.as<R2>(), wpos, wvel,
+                                                        c.pres.as<real>(), c.flags.as<uint32_t>());
     PR_HIP(hipGetLastError());
   }
   {
