@@ -78,3 +78,60 @@ def test_full_size_f64_bitexact_two_steps():
     got, ref = _run_pair(d, arrs, 2)
     for k in ("x", "vz", "rho", "p", "az"):
         assert np.array_equal(got[k], ref[k]), k
+
+
+def test_f64_bitexact_far_from_origin():
+    """The scan works on FP32 offsets, whose rounding grows with the coordinate
+    magnitude; its margin grows with it.  A dam break moved ~1 km from the
+    origin must keep exactly the oracle's neighbour sets."""
+    d, arrs = make_scene(pa.SCENE_DAM_BREAK, n_target=8000, dx=0.005)
+    shift = (1000.0, -500.0, 250.0)
+    for a, s in enumerate(shift):
+        d.grid_lo[a] += s
+        d.grid_hi[a] += s
+    arrs = dict(arrs, x=arrs["x"] + shift[0], y=arrs["y"] + shift[1], z=arrs["z"] + shift[2])
+    got, ref = _run_pair(d, arrs, 5)
+    for k in FIELDS:
+        assert np.array_equal(got[k], ref[k]), k
+
+
+def _layered_scene():
+    """A dense slab two cells thick (lattice 3 % tighter than the rest
+    spacing, so part of it is under pressure) under a sparse layer (one
+    particle per four cells): the wavefronts over the sparse layer span many
+    rows, so the union of their row runs below exceeds the LDS stage and those
+    runs are scanned from global memory."""
+    d = pa.desc_init(pa.MODEL_WCSPH)
+    h = 0.012
+    dx = h / 1.2
+    cs = 2.0 * h
+    ncell = (12, 12, 6)
+    d.h, d.cell_size, d.rho0, d.c0, d.alpha = h, cs, 1000.0, 20.0, 0.1
+    d.mass = d.rho0 * dx**3
+    d.dt = 0.25 * h / d.c0
+    d.gravity[:] = (0.0, 0.0, -9.81)
+    for a in range(3):
+        d.grid_lo[a] = 0.0
+        d.grid_hi[a] = ncell[a] * cs
+    g = np.arange(0.5 * dx, ncell[0] * cs, 0.97 * dx)
+    zz = np.arange(cs + 0.5 * dx, 3.0 * cs, 0.97 * dx)
+    X, Y, Z = np.meshgrid(g, g, zz, indexing="ij")
+    dense = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    cx, cy = np.meshgrid(np.arange(0, ncell[0], 2), np.arange(0, ncell[1], 2), indexing="ij")
+    rng = np.random.default_rng(3)
+    sparse = np.stack([(cx.ravel() + rng.uniform(0.1, 0.9, cx.size)) * cs,
+                       (cy.ravel() + rng.uniform(0.1, 0.9, cx.size)) * cs,
+                       np.full(cx.size, 3.4 * cs)], 1)
+    pts = np.concatenate([dense, sparse])
+    d.n_particles = d.n_moving = len(pts)
+    arrs = {"x": pts[:, 0].copy(), "y": pts[:, 1].copy(), "z": pts[:, 2].copy(),
+            "vx": np.zeros(len(pts)), "vy": np.zeros(len(pts)), "vz": np.zeros(len(pts))}
+    return d, arrs
+
+
+def test_f64_bitexact_unstaged_runs():
+    d, arrs = _layered_scene()
+    assert d.n_particles > 4 * 256
+    got, ref = _run_pair(d, arrs, 3)
+    for k in FIELDS:
+        assert np.array_equal(got[k], ref[k]), k
