@@ -246,6 +246,7 @@ static void NAME(sph_force_integrate)(STATE* s) {
   int64_t n = (int64_t)s->n;
   const double H2 = s->H2, inv_h = s->inv_h, eps2 = s->eps2, h = P->h, msh = s->msh;
   const double dt = P->dt;
+  const REAL ach = s->alpha_c0 * h;
 #pragma omp parallel for schedule(dynamic, 1024)
   for (int64_t i = 0; i < n; ++i) {
     if (s->id[i] >= P->n_moving) {
@@ -274,12 +275,19 @@ static void NAME(sph_force_integrate)(STATE* s) {
           }
           REAL dvx = vxi - s->vx[j], dvy = vyi - s->vy[j], dvz = vzi - s->vz[j];
           REAL vdotr = dvx * dx + dvy * dy + dvz * dz;
-          REAL pij = 0.0;
+          /* sc = (pr_i + pr_j + Pi_ij) f'(q) / r with Pi_ij = -alpha c0 h (v.r) /
+           * ((r^2 + eps2) rhobar) for approaching pairs, put over one common
+           * denominator so that every pair costs a single division */
+          REAL num, den;
           if (RE(vdotr) < 0.0) {
-            REAL muij = h * vdotr / (r2 + eps2);
-            pij = -s->alpha_c0 * muij / (0.5 * (rhoi + s->rho[j]));
+            REAL drb = (r2 + eps2) * (0.5 * (rhoi + s->rho[j]));
+            num = ((pri + s->pr[j]) * drb - ach * vdotr) * dw;
+            den = drb * rr;
+          } else {
+            num = (pri + s->pr[j]) * dw;
+            den = rr;
           }
-          REAL sc = (pri + s->pr[j] + pij) * (dw / rr);
+          REAL sc = num / den;
           sx = sx + sc * dx;
           sy = sy + sc * dy;
           sz = sz + sc * dz;

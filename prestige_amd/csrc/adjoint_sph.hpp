@@ -1,7 +1,7 @@
 // adjoint_sph.hpp — reverse mode of one WCSPH step, gather form.
 //
 // Forward step (sph_kernels.hpp), per moving particle k with d = x_k - x_j,
-// e = v_k - v_j, r = |d|, q = r/h over its frozen neighbour streams:
+// e = v_k - v_j, r = |d|, q = r/h over its frozen neighbour masks:
 //   rho_k = m sigma sum_j f(q),  p_k = max(B((rho_k/rho0)^7 - 1), 0),
 //   P_k = p_k / rho_k^2,  B = rho0 c0^2 / 7,
 //   a_k = g - (m sigma/h) sum_j S_kj d,  S = Q g(r),  Q = P_k + P_j + Pi,
@@ -39,8 +39,8 @@ struct SphStepRecord {
   const void* rp;         // (rho, p/rho^2): double2 / float2
   const uint32_t* ids;
   const uint32_t* perm;   // slot k of this step <- slot perm[k] of the previous order
-  const uint2* st;         // neighbour streams of the step
-  const uint32_t* scnt;
+  const uint2* meta;       // row-run masks of the step
+  const uint32_t* words;
 };
 
 template <class real>
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
     const ar rhoi = (ar)rpi.x, pri = (ar)rpi.y;
     const ar cix = (ar)ci.x, ciy = (ar)ci.y, ciz = (ar)ci.z;
     double xx = 0, xy = 0, xz = 0, vx = 0, vy = 0, vz = 0, pb = 0, rb = 0, sa = 0, sc0 = 0;
-    walk_stream(rec.st, n, i, rec.scnt[i], [&](const uint32_t* jj) {
+    walk_runs<kUnr>(rec.meta, rec.words, n, i, [&](const uint32_t* jj) {
       G4 pj[kUnr], vj[kUnr];
       R2 rpj[kUnr];
       C4 cj[kUnr];
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(
   const real H2 = (real)u.H2;
   const ar inv_h = (ar)u.inv_h;
   double sx = 0, sy = 0, sz = 0;
-  walk_stream(rec.st, n, i, rec.scnt[i], [&](const uint32_t* jj) {
+  walk_runs<kUnr>(rec.meta, rec.words, n, i, [&](const uint32_t* jj) {
     G4 pj[kUnr];
     ar rbj[kUnr];
 #pragma unroll
@@ -359,7 +359,7 @@ inline void take_buffer(DevBuf& ctx_buf, DevBuf& rec_buf) {
 
 // Keep what the reverse sweep needs from the step that just ran.  Taken by
 // swap: the pair geometry (sorted pre-step state or FP32 probes), the
-// permutation and the neighbour streams, all of which the next step
+// permutation and the row-run masks, all of which the next step
 // regenerates.  Copied: ids (the next reorder reads them) and (rho, p/rho^2),
 // which stays downloadable.
 template <int F>
@@ -375,8 +375,8 @@ void record_sph_step(Ctx& c, StepRecord& r) {
     take_buffer(c.vel[prev], r.gvel);
   }
   take_buffer(c.vals[c.perm_buf], r.perm);
-  take_buffer(c.nstream, r.nstream);
-  take_buffer(c.ncount, r.ncount);
+  take_buffer(c.nmeta, r.nmeta);
+  take_buffer(c.nword, r.nword);
   r.rp.alloc(n * r2);
   r.ids.alloc(n * 4);
   auto cp = [&](DevBuf& d, const void* s, size_t b) {
@@ -437,7 +437,7 @@ void sph_adjoint(Ctx& c, const double* seed_x, const double* seed_v, double* gra
     for (uint32_t t = len; t-- > 0;) {
       const StepRecord& r = *T.rec[t];
       SphStepRecord sr{r.gpos.p, r.gvel.p, r.rp.p, r.ids.as<uint32_t>(), r.perm.as<uint32_t>(),
-                       r.nstream.as<uint2>(), r.ncount.as<uint32_t>()};
+                       r.nmeta.as<uint2>(), r.nword.as<uint32_t>()};
       sph_reverse_step<F>(c, sr, A);
     }
   }

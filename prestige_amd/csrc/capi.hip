@@ -211,9 +211,8 @@ static void allocate(Ctx& c) {
     if (dem) c.om32.alloc(nn * sizeof(float4));
   }
   if (c.desc.model == PRESTIGE_MODEL_WCSPH) {
-    c.nstream.alloc(nn * kStreamCap * sizeof(uint2));
-    c.ncount.alloc(nn * 4);
-    PR_HIP(hipMemsetAsync(c.ncount.p, 0, nn * 4, c.stream));
+    c.nmeta.alloc(nn * 9 * sizeof(uint2));
+    c.nword.alloc(nn * 36 * sizeof(uint32_t));
   }
   c.scratch.alloc(nn * 8);
   if (dem) {
@@ -429,8 +428,6 @@ prestige_status prestige_create(const prestige_desc* desc, prestige_ctx** out) {
       throw Error(PRESTIGE_E_INVALID, "unknown precision");
     if (desc->n_particles >= 0xFFFFFFFFull) throw Error(PRESTIGE_E_INVALID, "n_particles must be < 2^32 - 1");
     if (desc->n_moving > desc->n_particles) throw Error(PRESTIGE_E_INVALID, "n_moving > n_particles");
-    if (desc->model == PRESTIGE_MODEL_WCSPH && desc->n_particles > kMaxSphParticles)
-      throw Error(PRESTIGE_E_CAPACITY, "WCSPH holds at most 2^28 - 16 particles per context");
     if (!(desc->cell_size > 0.0) || !std::isfinite(desc->cell_size))
       throw Error(PRESTIGE_E_INVALID, "cell_size must be finite and > 0");
     if (desc->model != PRESTIGE_MODEL_PAIRS && !(desc->dt > 0.0)) throw Error(PRESTIGE_E_INVALID, "dt must be > 0");

@@ -133,7 +133,7 @@ struct Snapshot {
 // What the reverse sweep needs from one forward step: the sorted pre-step
 // pair geometry, the gather outputs, and the frozen neighbour structure.
 struct StepRecord {
-  DevBuf gpos, gvel, gom, rp, ids, perm, nstream, ncount;
+  DevBuf gpos, gvel, gom, rp, ids, perm, nmeta, nword;
   DevBuf hcnt, hid, hxi, wxi;   // DEM history read by the step
   DevBuf hcnt_out, hid_out;     // DEM contacts the step found (its frozen contact lists)
   DevBuf state_pos, state_vel, state_om;   // DEM FP64 pre-step state (sorted)
@@ -198,11 +198,10 @@ struct Ctx {
   // MIXED form gather copies
   DevBuf off32, vel32, om32;     // float4: offset from cell origin (cell x in .w); velocity
                                  // (mass in .w); DEM angular velocity (radius in .w)
-  // neighbour streams of the last density pass: entry k of slot i at
-  // nstream[k n + i] is a uint2 (32-bit word of candidates inside the cutoff,
-  // slot of its bit 0), non-empty words only, in scan order; ncount[i]
-  // entries (at most kStreamCap)
-  DevBuf nstream, ncount;
+  // row-run masks of the last density pass: run r of slot i has (first
+  // candidate slot, candidate count) at nmeta[r n + i] (uint2) and its hit
+  // words at nword[(4 r + w) n + i] (uint32, written for 32 w < count)
+  DevBuf nmeta, nword;
   // eq1 probe
   DevBuf force, mass, scratch;
   bool have_force = false, have_mass = false;
@@ -242,8 +241,6 @@ __device__ inline uint32_t xcd_tile(uint32_t b, uint32_t grid) {
 // of w_j - w_i + 64 is ((cx_j - cx_i + 1) mod 4) * 64, which the scan reads
 // with a single byte-to-float conversion.
 constexpr uint32_t kCodeShift = 6;
-constexpr uint32_t kMaxSphParticles = (1u << 28) - 16u;  // slot indices fit the 28-bit stream base field
-constexpr uint32_t kStreamCap = 36;  // neighbour stream entries per slot: 9 runs x 4 words of 32
 
 __device__ inline uint32_t cell_code(uint32_t cx, uint32_t cy, uint32_t cz) {
   return ((cx & 3u) | ((cy & 3u) << 2) | ((cz & 3u) << 4)) << kCodeShift;

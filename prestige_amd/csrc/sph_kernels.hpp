@@ -176,7 +176,7 @@ __device__ inline real spline_df(real q) {
 #define PRESTIGE_DENSITY_UNROLL 2
 #endif
 #ifndef PRESTIGE_UNROLL
-#define PRESTIGE_UNROLL 4
+#define PRESTIGE_UNROLL 2
 #endif
 constexpr int kScanUnr = PRESTIGE_SCAN_UNROLL;
 constexpr int kDensUnr = PRESTIGE_DENSITY_UNROLL;
@@ -230,34 +230,48 @@ __device__ inline uint32_t scan_word(Ld&& ld, uint32_t k0, uint32_t cnt, const S
   return cnt < 32u ? word & ((1u << cnt) - 1u) : word;
 }
 
-// Neighbour stream entries: (word, base | run << kRunShift).  Bit b of word
-// names slot (base + b); run is the index of the row run the word came from
-// (0..8, in the order for_each_run visits the existing rows).  Slots stay
-// below 2^28, so the run tag fits the top four bits.
-constexpr uint32_t kRunShift = 28;
-constexpr uint32_t kBaseMask = (1u << kRunShift) - 1u;
+// Row-run masks of the density pass, kept for the force pass and the
+// adjoint.  Run r = 3 (dz + 1) + (dy + 1) of slot i (the order for_each_run
+// visits rows): meta[r n + i] = (j0, len) of the run's candidate range (len
+// 0 for a row outside the grid), and for 32 w < len word w of its hit mask at
+// words[(4 r + w) n + i]; bit b of word w names slot j0 + 32 w + b.  Words at
+// or past len are not written.
+constexpr uint32_t kRuns = 9;
+
+__device__ inline size_t meta_index(uint32_t r, uint32_t n, uint32_t i) { return (size_t)r * n + i; }
+
+__device__ inline size_t word_index(uint32_t r, uint32_t w, uint32_t n, uint32_t i) {
+  return (size_t)(4u * r + w) * n + i;
+}
 
 // Scan one row run [j0, j0 + len), len <= 128: bit b of the 128-bit mask
 // (m0, m1) is set when candidate j0 + b lies inside the probe cutoff; ld(j)
-// returns the scan offsets of slot j.  The run's non-empty 32-candidate
-// words are also appended to slot i's neighbour stream (entry k at
-// st[k n + i]).
+// returns the scan offsets of slot j.  The run's words are also stored as
+// run r of slot i.
 template <class Ld>
-__device__ inline void scan_run(Ld&& ld, const ScanPoint& s, uint32_t j0, uint32_t len, uint32_t run,
-                                uint2* __restrict__ st, uint32_t n, uint32_t i, uint32_t& k, uint64_t& m0,
-                                uint64_t& m1) {
+__device__ inline void scan_run(Ld&& ld, const ScanPoint& s, uint32_t j0, uint32_t len, uint32_t r,
+                                uint32_t* __restrict__ words, uint32_t n, uint32_t i, uint64_t& m0, uint64_t& m1) {
   uint32_t wd[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
   for (uint32_t w = 0; w < 4u; ++w) {
     if (len > 32u * w) {
       const uint32_t cnt = len - 32u * w < 32u ? len - 32u * w : 32u;
       wd[w] = scan_word(ld, j0 + 32u * w, cnt, s);
-      if (wd[w]) {
-        st[(size_t)k * n + i] = make_uint2(wd[w], (j0 + 32u * w) | (run << kRunShift));
-        ++k;
-      }
+      words[word_index(r, w, n, i)] = wd[w];
     }
   }
+  m0 = (uint64_t)wd[0] | ((uint64_t)wd[1] << 32);
+  m1 = (uint64_t)wd[2] | ((uint64_t)wd[3] << 32);
+}
+
+// Mask of run r of slot i as stored by the density pass.
+__device__ inline void load_run(const uint2* __restrict__ meta, const uint32_t* __restrict__ words, uint32_t r,
+                                uint32_t n, uint32_t i, uint32_t& j0, uint64_t& m0, uint64_t& m1) {
+  const uint2 mt = meta[meta_index(r, n, i)];
+  j0 = mt.x;
+  uint32_t wd[4];
+#pragma unroll
+  for (uint32_t w = 0; w < 4u; ++w) wd[w] = mt.y > 32u * w ? words[word_index(r, w, n, i)] : 0u;
   m0 = (uint64_t)wd[0] | ((uint64_t)wd[1] << 32);
   m1 = (uint64_t)wd[2] | ((uint64_t)wd[3] << 32);
 }
@@ -284,48 +298,22 @@ __device__ inline void walk_bits(uint64_t m0, uint64_t m1, Fn&& fn) {
   }
 }
 
-// Runs walked in lockstep by a wavefront.  A lane walks the stream entries of
-// run group g = run / kWalkGroup and then waits for the other lanes before
-// group g + 1: neighbouring lanes (mostly the same cell) then load from the
-// same few cache lines.  1 = per row run, 3 = per z plane, 9 = the whole
-// stream at once (fewest idle lanes, least shared lines).
-#ifndef PRESTIGE_WALK_GROUP
-#define PRESTIGE_WALK_GROUP 9
-#endif
-constexpr uint32_t kWalkGroup = PRESTIGE_WALK_GROUP;
-static_assert(kWalkGroup == 1 || kWalkGroup == 3 || kWalkGroup == 9, "run groups split the nine runs evenly");
-
-// Neighbour stream of slot i: cnt entries at st[k n + i], non-empty and in
-// scan order.  Visit the named slots in that order, kUnr at a time: fn(jj)
-// receives slot indices, kPad for unused batch entries (jj[0] is always a
-// slot).  The next entry is loaded while the current one is walked.
-template <class Fn>
-__device__ inline void walk_stream(const uint2* __restrict__ st, uint32_t n, uint32_t i, uint32_t cnt,
-                                   Fn&& fn) {
-  uint2 cur = cnt ? st[i] : make_uint2(0u, 0u);
-  uint2 nxt = cnt > 1u ? st[(size_t)n + i] : make_uint2(0u, 0u);
-  uint32_t k = 1;  // entry index of nxt
-  for (uint32_t g = 0; g < 9u / kWalkGroup; ++g) {
-    // run group of the current entry (groups past the last for an empty one)
-    auto in_group = [&]() { return cur.x != 0u && (cur.y >> kRunShift) / kWalkGroup == g; };
-    while (in_group()) {
-      uint32_t jj[kUnr];
+// Visit the stored neighbours of slot i run by run in scan order, B at a
+// time: fn(jj) receives slot indices, kPad for unused batch entries (jj[0]
+// is always a slot).
+template <int B, class Fn>
+__device__ inline void walk_runs(const uint2* __restrict__ meta, const uint32_t* __restrict__ words, uint32_t n,
+                                 uint32_t i, Fn&& fn) {
+  for (uint32_t r = 0; r < kRuns; ++r) {
+    uint32_t j0;
+    uint64_t m0, m1;
+    load_run(meta, words, r, n, i, j0, m0, m1);
+    walk_bits<B>(m0, m1, [&](const uint32_t* jb) {
+      uint32_t jj[B];
 #pragma unroll
-      for (int u = 0; u < kUnr; ++u) {
-        if (in_group()) {
-          jj[u] = (cur.y & kBaseMask) + (uint32_t)__builtin_ctz(cur.x);
-          cur.x &= cur.x - 1u;
-          if (!cur.x) {
-            cur = nxt;
-            ++k;
-            nxt = k < cnt ? st[(size_t)k * n + i] : make_uint2(0u, 0u);
-          }
-        } else {
-          jj[u] = kPad;
-        }
-      }
+      for (int v = 0; v < B; ++v) jj[v] = jb[v] == kPad ? kPad : j0 + jb[v];
       fn(jj);
-    }
+    });
   }
 }
 
@@ -369,7 +357,7 @@ template <int F>
 __global__ __launch_bounds__(kDensBlock) void k_sph_density(
     uint32_t n, const typename SphForm<F>::S4* __restrict__ spos, const typename SphForm<F>::G4* pos,
     const uint32_t* __restrict__ skeys, const uint32_t* __restrict__ lb, Grid g, SphConst kc, SphUnits u,
-    uint2* __restrict__ st, uint32_t* __restrict__ scnt, typename SphForm<F>::R2* __restrict__ rp,
+    uint2* __restrict__ meta, uint32_t* __restrict__ words, typename SphForm<F>::R2* __restrict__ rp,
     typename SphForm<F>::G4* wpos, typename SphForm<F>::G4* wvel,
     typename SphForm<F>::real* __restrict__ pres, uint32_t* flags) {
   using real = typename SphForm<F>::real;
@@ -400,12 +388,11 @@ __global__ __launch_bounds__(kDensBlock) void k_sph_density(
   }
   const real H2 = (real)u.H2, inv_h = (real)u.inv_h;
   Sum<real, F == kMixed> acc;
-  uint32_t k = 0;
   bool overflow = false;
-  uint32_t run = 0;
   const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
   for (int ddz = -1; ddz <= 1; ++ddz) {
     for (int ddy = -1; ddy <= 1; ++ddy) {
+      const uint32_t r = (uint32_t)(3 * (ddz + 1) + (ddy + 1));
       const int zz = cr.cz + ddz, yy = cr.cy + ddy;
       const bool row_ok = live && zz >= 0 && zz < nz && yy >= 0 && yy < ny;
       uint32_t j0 = 0, len = 0;
@@ -418,13 +405,11 @@ __global__ __launch_bounds__(kDensBlock) void k_sph_density(
           len = 128u;
         }
       }
+      if (live) meta[meta_index(r, n, i)] = make_uint2(j0, len);
       // union of the wavefront's candidate ranges of this run
       const uint32_t base = wave_min(len ? j0 : 0xFFFFFFFFu);
       const uint32_t top = wave_max(len ? j0 + len : 0u);
-      if (top <= base) {
-        if (row_ok) ++run;
-        continue;
-      }
+      if (top <= base) continue;
       const bool staged = top - base <= kStageCap;
       if (staged) {
         for (uint32_t kk = lane; kk < top - base; kk += kDensBlock) s_q[kk] = offset(spos[base + kk]);
@@ -433,10 +418,10 @@ __global__ __launch_bounds__(kDensBlock) void k_sph_density(
       if (len) {
         uint64_t m0, m1;
         if (staged) {
-          scan_run([&](uint32_t j) { return s_q[j - base]; }, sp, j0, len, run, st, n, i, k, m0, m1);
+          scan_run([&](uint32_t j) { return s_q[j - base]; }, sp, j0, len, r, words, n, i, m0, m1);
         } else {
-          scan_run([&](uint32_t j) { return offset(spos[j < n ? j : n - 1]); }, sp, j0, len, run, st, n, i, k,
-                   m0, m1);
+          scan_run([&](uint32_t j) { return offset(spos[j < n ? j : n - 1]); }, sp, j0, len, r, words, n, i, m0,
+                   m1);
         }
         walk_bits<kDensUnr>(m0, m1, [&](const uint32_t* jj) {
           G4 pj[kDensUnr];
@@ -455,12 +440,10 @@ __global__ __launch_bounds__(kDensBlock) void k_sph_density(
           }
         });
       }
-      if (row_ok) ++run;
       if (staged) __syncthreads();  // the next run's staging overwrites s_q
     }
   }
   if (!live) return;
-  scnt[i] = k;
   if (overflow) atomicOr(&flags[FLAG_CAPACITY], 1u);
   if constexpr (F == kF64) {
     double rho = kc.msig * acc.s;
@@ -512,13 +495,66 @@ __global__ __launch_bounds__(kDensBlock) void k_sph_density(
 //   v' = v + dt a, x' = x + dt v'.  Fixed particles (id >= n_moving) copy
 //   through.  State is read from (spos, svel) and written to (opos, ovel):
 //   neighbours keep reading the pre-step state.
+//
+// One wavefront per block, walking the row runs in lockstep.  For each run
+// the wavefront first copies the gather data of the union of its lanes' hit
+// ranges into LDS (coalesced; the lanes of a wavefront share most
+// neighbours), then every lane walks its own mask with LDS reads instead of
+// dependent global loads.  A union wider than kForceCap slots is walked from
+// global memory.
 // ---------------------------------------------------------------------------
+constexpr int kForceBlock = 64;  // one wavefront: block barriers are wave barriers
+#ifndef PRESTIGE_FORCE_CAP
+#define PRESTIGE_FORCE_CAP 160
+#endif
+constexpr uint32_t kForceCap = PRESTIGE_FORCE_CAP;
+
+// One momentum pair term: (rx, ry, rz) = sc x_ij for 0 < r < 2h; false
+// outside (then nothing is added).  sc = (pr_i + pr_j + Pi_ij) f'(q) / r with
+// Pi_ij = -alpha c0 h (v.r) / ((r^2 + eps2) rhobar) for approaching pairs,
+// put over one common denominator: a single division per pair.  ach is
+// alpha c0 times hv.
 template <int F>
-__global__ __launch_bounds__(kGather) void k_sph_force(
+__device__ inline bool force_pair(const typename SphForm<F>::G4& pi, const typename SphForm<F>::G4& vi,
+                                  typename SphForm<F>::real rhoi, typename SphForm<F>::real pri,
+                                  const typename SphForm<F>::G4& pj, const typename SphForm<F>::G4& vj,
+                                  typename SphForm<F>::real rhoj, typename SphForm<F>::real prj,
+                                  const SphUnits& u, typename SphForm<F>::real ach,
+                                  typename SphForm<F>::real& rx, typename SphForm<F>::real& ry,
+                                  typename SphForm<F>::real& rz) {
+  using real = typename SphForm<F>::real;
+  const real H2 = (real)u.H2, inv_h = (real)u.inv_h, eps2 = (real)u.eps2;
+  real dx, dy, dz;
+  form_pair<F>(pi, pj, u.cs_u, dx, dy, dz);
+  real r2 = dx * dx + dy * dy + dz * dz;
+  if (!(r2 < H2 && r2 > real(0.0))) return false;
+  real rr = sqrt(r2);
+  real q = rr * inv_h;
+  real dw = spline_df(q);
+  real dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
+  real vdotr = dvx * dx + dvy * dy + dvz * dz;
+  real num, den;
+  if (vdotr < real(0.0)) {
+    real drb = (r2 + eps2) * (real(0.5) * (rhoi + rhoj));
+    num = ((pri + prj) * drb - ach * vdotr) * dw;
+    den = drb * rr;
+  } else {
+    num = (pri + prj) * dw;
+    den = rr;
+  }
+  real sc = num / den;
+  rx = sc * dx;
+  ry = sc * dy;
+  rz = sc * dz;
+  return true;
+}
+
+template <int F>
+__global__ __launch_bounds__(kForceBlock) void k_sph_force(
     uint32_t n, uint64_t n_moving, const typename SphForm<F>::G4* __restrict__ gpos,
     const typename SphForm<F>::G4* __restrict__ gvel, const typename SphForm<F>::R2* __restrict__ rp,
     const uint32_t* __restrict__ ids, const uint32_t* __restrict__ skeys,
-    const uint2* __restrict__ st, const uint32_t* __restrict__ scnt, Grid g, SphConst kc,
+    const uint2* __restrict__ meta, const uint32_t* __restrict__ words, Grid g, SphConst kc,
     SphUnits u, const typename SphForm<F>::S4* __restrict__ spos,
     const typename SphForm<F>::S4* __restrict__ svel, typename SphForm<F>::S4* __restrict__ opos,
     typename SphForm<F>::S4* __restrict__ ovel, uint32_t* __restrict__ okeys,
@@ -528,66 +564,111 @@ __global__ __launch_bounds__(kGather) void k_sph_force(
   using R2 = typename SphForm<F>::R2;
   using S4 = typename SphForm<F>::S4;
   using sreal = typename SphForm<F>::sreal;
-  const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  __shared__ G4 s_p[kForceCap], s_v[kForceCap];
+  __shared__ R2 s_rp[F == kMixed ? kForceCap : 1];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kForceBlock + lane;
+  const bool live = i < n && (uint64_t)ids[i] < n_moving;
+  // (rho, p/rho^2) of a slot: the density pass also left them in the .w
+  // lanes of the FP64 / FP32 gather copies
+  G4 pi{}, vi{};
+  real rhoi = real(0.0), pri = real(0.0);
+  if (live) {
+    pi = gpos[i];
+    vi = gvel[i];
+    if constexpr (F == kMixed) {
+      const R2 rpi = rp[i];
+      rhoi = rpi.x;
+      pri = rpi.y;
+    } else {
+      rhoi = pi.w;
+      pri = vi.w;
+    }
+  }
+  const real alpha_c0 = (real)kc.alpha_c0, ach = alpha_c0 * (real)u.hv;
+  Sum<real, F == kMixed> sx, sy, sz;
+  // pair bodies over a batch of neighbours fetched by get(j, p, v, rho, pr)
+  auto body = [&](const uint32_t* jj, auto&& get) {
+    G4 pj[kUnr], vj[kUnr];
+    real rhoj[kUnr], prj[kUnr];
+#pragma unroll
+    for (int v = 0; v < kUnr; ++v) get(jj[v] == kPad ? jj[0] : jj[v], pj[v], vj[v], rhoj[v], prj[v]);
+#pragma unroll
+    for (int v = 0; v < kUnr; ++v) {
+      if (jj[v] == kPad) continue;
+      real rx, ry, rz;
+      if (force_pair<F>(pi, vi, rhoi, pri, pj[v], vj[v], rhoj[v], prj[v], u, ach, rx, ry, rz)) {
+        sx.add(rx);
+        sy.add(ry);
+        sz.add(rz);
+      }
+    }
+  };
+  for (uint32_t r = 0; r < kRuns; ++r) {
+    uint32_t j0 = 0;
+    uint64_t m0 = 0, m1 = 0;
+    if (live) load_run(meta, words, r, n, i, j0, m0, m1);
+    // union of the lanes' hit ranges
+    uint32_t a = 0xFFFFFFFFu, b = 0u;
+    if (m0 | m1) {
+      a = j0 + (m0 ? (uint32_t)__builtin_ctzll(m0) : 64u + (uint32_t)__builtin_ctzll(m1));
+      b = j0 + (m1 ? 128u - (uint32_t)__builtin_clzll(m1) : 64u - (uint32_t)__builtin_clzll(m0));
+    }
+    const uint32_t base = wave_min(a), top = wave_max(b);
+    if (top <= base) continue;
+    if (top - base <= kForceCap) {
+      for (uint32_t k = lane; k < top - base; k += kForceBlock) {
+        s_p[k] = gpos[base + k];
+        s_v[k] = gvel[base + k];
+        if constexpr (F == kMixed) s_rp[k] = rp[base + k];
+      }
+      __syncthreads();
+      walk_bits<kUnr>(m0, m1, [&](const uint32_t* jb) {
+        uint32_t jj[kUnr];
+#pragma unroll
+        for (int v = 0; v < kUnr; ++v) jj[v] = jb[v] == kPad ? kPad : j0 + jb[v] - base;
+        body(jj, [&](uint32_t k, G4& p, G4& v, real& rho, real& pr) {
+          p = s_p[k];
+          v = s_v[k];
+          if constexpr (F == kMixed) {
+            const R2 t = s_rp[k];
+            rho = t.x;
+            pr = t.y;
+          } else {
+            rho = p.w;
+            pr = v.w;
+          }
+        });
+      });
+      __syncthreads();  // the next run's staging overwrites the stage
+    } else {
+      walk_bits<kUnr>(m0, m1, [&](const uint32_t* jb) {
+        uint32_t jj[kUnr];
+#pragma unroll
+        for (int v = 0; v < kUnr; ++v) jj[v] = jb[v] == kPad ? kPad : j0 + jb[v];
+        body(jj, [&](uint32_t j, G4& p, G4& v, real& rho, real& pr) {
+          p = gpos[j];
+          v = gvel[j];
+          if constexpr (F == kMixed) {
+            const R2 t = rp[j];
+            rho = t.x;
+            pr = t.y;
+          } else {
+            rho = p.w;
+            pr = v.w;
+          }
+        });
+      });
+    }
+  }
   if (i >= n) return;
-  if ((uint64_t)ids[i] >= n_moving) {
+  if (!live) {
     opos[i] = spos[i];
     ovel[i] = svel[i];
     okeys[i] = skeys[i];
     if (oacc) oacc[i] = S4{0, 0, 0, 0};
     return;
   }
-  const G4 pi = gpos[i];
-  const G4 vi = gvel[i];
-  // (rho, p/rho^2) of the slot: the density pass also left them in the .w
-  // lanes of the FP64 / FP32 gather copies, so a neighbour costs two loads
-  real rhoi, pri;
-  if constexpr (F == kMixed) {
-    const R2 rpi = rp[i];
-    rhoi = rpi.x;
-    pri = rpi.y;
-  } else {
-    rhoi = pi.w;
-    pri = vi.w;
-  }
-  const real H2 = (real)u.H2, inv_h = (real)u.inv_h, eps2 = (real)u.eps2, hv = (real)u.hv;
-  const real alpha_c0 = (real)kc.alpha_c0;
-  Sum<real, F == kMixed> sx, sy, sz;
-  walk_stream(st, n, i, scnt[i], [&](const uint32_t* jj) {
-    G4 pj[kUnr], vj[kUnr];
-    R2 rpj[kUnr];
-#pragma unroll
-    for (int v = 0; v < kUnr; ++v) {
-      const uint32_t j = jj[v] == kPad ? jj[0] : jj[v];
-      pj[v] = gpos[j];
-      vj[v] = gvel[j];
-      if constexpr (F == kMixed) rpj[v] = rp[j];
-      else rpj[v] = R2{pj[v].w, vj[v].w};
-    }
-#pragma unroll
-    for (int v = 0; v < kUnr; ++v) {
-      if (jj[v] == kPad) continue;
-      real dx, dy, dz;
-      form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
-      real r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 < H2 && r2 > real(0.0)) {
-        real rr = sqrt(r2);
-        real q = rr * inv_h;
-        real dw = spline_df(q);
-        real dvx = vi.x - vj[v].x, dvy = vi.y - vj[v].y, dvz = vi.z - vj[v].z;
-        real vdotr = dvx * dx + dvy * dy + dvz * dz;
-        real pij = real(0.0);
-        if (vdotr < real(0.0)) {
-          real muij = hv * vdotr / (r2 + eps2);
-          pij = -alpha_c0 * muij / (real(0.5) * (rhoi + rpj[v].x));
-        }
-        real sc = (pri + rpj[v].y + pij) * (dw / rr);
-        sx.add(sc * dx);
-        sy.add(sc * dy);
-        sz.add(sc * dz);
-      }
-    }
-  });
   const S4 xs = spos[i];
   const S4 vs = svel[i];
   sreal ax, ay, az;
@@ -693,8 +774,8 @@ void sph_step_impl(Ctx& c) {
                    per * n);
     const uint32_t dgrid = xcd_grid(blocks_for(n, kDensBlock));
     k_sph_density<F><<<dgrid, kDensBlock, 0, c.stream>>>(n, c.pos[c.cur].as<S4>(), gpos, skeys, c.lb.as<uint32_t>(),
-                                                        c.grid, c.sph, u, c.nstream.as<uint2>(),
-                                                        c.ncount.as<uint32_t>(), c.rp.as<R2>(), wpos, wvel,
+                                                        c.grid, c.sph, u, c.nmeta.as<uint2>(),
+                                                        c.nword.as<uint32_t>(), c.rp.as<R2>(), wpos, wvel,
                                                         c.pres.as<real>(), c.flags.as<uint32_t>());
     PR_HIP(hipGetLastError());
   }
@@ -707,9 +788,10 @@ void sph_step_impl(Ctx& c) {
     LaunchScope ls(&c, F == kF64 ? "sph_force_f64" : (F == kF32 ? "sph_force_f32" : "sph_force_mixed"), per * nm);
     const int dst = c.cur ^ 1;
     const int kdst = c.cur_key ^ 1;
-    k_sph_force<F><<<grid, kGather, 0, c.stream>>>(
+    const uint32_t fgrid = xcd_grid(blocks_for(n, kForceBlock));
+    k_sph_force<F><<<fgrid, kForceBlock, 0, c.stream>>>(
         n, c.n_moving, gpos, gvel, c.rp.as<R2>(), c.ids[c.cur_ids].as<uint32_t>(), skeys,
-        c.nstream.as<uint2>(), c.ncount.as<uint32_t>(), c.grid, c.sph, u, c.pos[c.cur].as<S4>(), c.vel[c.cur].as<S4>(),
+        c.nmeta.as<uint2>(), c.nword.as<uint32_t>(), c.grid, c.sph, u, c.pos[c.cur].as<S4>(), c.vel[c.cur].as<S4>(),
         c.pos[dst].as<S4>(), c.vel[dst].as<S4>(), c.keys[kdst].as<uint32_t>(),
         c.keep_acc ? c.acc.as<S4>() : nullptr, c.flags.as<uint32_t>());
     PR_HIP(hipGetLastError());
