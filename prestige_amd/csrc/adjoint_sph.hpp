@@ -55,6 +55,26 @@ __device__ inline real spline_d2f(real q) {
 template <int F> struct AdjForm { using ar = double; using C4 = double4; };
 template <> struct AdjForm<kMixed> { using ar = float; using C4 = float4; };
 
+// Reciprocal and 1/sqrt of the reverse pair scalars: hardware estimates
+// (about 1 ulp) in FP32, where the mixed form's gradient tolerance dwarfs
+// them; IEEE operations in FP64.
+template <class T>
+__device__ inline T adj_rcp(T x) {
+  if constexpr (sizeof(T) == 4) return __builtin_amdgcn_rcpf(x);
+  else return T(1.0) / x;
+}
+
+template <class T>
+__device__ inline void adj_len(T r2, T& rr, T& irr) {
+  if constexpr (sizeof(T) == 4) {
+    irr = __builtin_amdgcn_rsqf(r2);
+    rr = r2 * irr;
+  } else {
+    rr = sqrt(r2);
+    irr = T(1.0) / rr;
+  }
+}
+
 // Reverse of the symplectic-Euler update for one slot: vbt = vbar' + dt xbar',
 // c = -(m sigma/h) dt vbt; fixed particles get zero.
 template <int F>
@@ -79,9 +99,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_prep(uint32_t n, uint64_t n_movi
   cvec[k] = C4{(decltype(C4{}.x))(s * vt.x), (decltype(C4{}.x))(s * vt.y), (decltype(C4{}.x))(s * vt.z), 0};
 }
 
-// Block sum of two doubles into partial[blockIdx] (deterministic).
+// Block sum of two doubles into partial[slot] (deterministic); W wavefronts
+// per block.
+template <uint32_t W>
 __device__ inline void block_sum2(double a, double b, double2* __restrict__ partial, uint32_t slot) {
-  __shared__ double sa[kBlock / 64], sb[kBlock / 64];
+  __shared__ double sa[W], sb[W];
   for (int o = 32; o > 0; o >>= 1) {
     a += __shfl_down(a, (unsigned)o, 64);
     b += __shfl_down(b, (unsigned)o, 64);
@@ -94,7 +116,7 @@ __device__ inline void block_sum2(double a, double b, double2* __restrict__ part
   __syncthreads();
   if (threadIdx.x == 0) {
     double ta = 0.0, tb = 0.0;
-    for (uint32_t k = 0; k < kBlock / 64; ++k) {
+    for (uint32_t k = 0; k < W; ++k) {
       ta += sa[k];
       tb += sb[k];
     }
@@ -105,9 +127,12 @@ __device__ inline void block_sum2(double a, double b, double2* __restrict__ part
 // Force + EOS adjoint (all slots; fixed particles collect Pbar and rhobar as
 // neighbours of moving ones).  Writes xbar of the force rows (metres), adds
 // the viscosity part of vbar, writes rhobar, and block partials of
-// (dalpha, dc0).
+// (dalpha, dc0).  One wavefront per block; each row run's neighbour data are
+// staged in LDS as in the forward force pass.
+constexpr int kAdjBlock = 64;
+
 template <int F>
-__global__ __launch_bounds__(kBlock) void k_adj_force(
+__global__ __launch_bounds__(kAdjBlock) void k_adj_force(
     uint32_t n, uint64_t n_moving, SphStepRecord rec, Grid g, SphConst kc, SphUnits u, double alpha, double c0,
     double rho0, const typename AdjForm<F>::C4* __restrict__ cvec, double4* __restrict__ xbf,
     double4* __restrict__ vb, typename AdjForm<F>::ar* __restrict__ rhob, double2* __restrict__ partial) {
@@ -119,30 +144,43 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
   const G4* gpos = (const G4*)rec.gpos;
   const G4* gvel = (const G4*)rec.gvel;
   const R2* rp = (const R2*)rec.rp;
-  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-  const uint32_t i = tile * blockDim.x + threadIdx.x;
+  __shared__ G4 s_p[kForceCap], s_v[kForceCap];
+  __shared__ R2 s_rp[kForceCap];
+  __shared__ C4 s_c[kForceCap];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kAdjBlock + lane;
+  const bool live = i < n;
   double pa = 0.0, pc = 0.0;
-  if (i < n) {
-    const G4 pi = gpos[i], vi = gvel[i];
-    const R2 rpi = rp[i];
-    const C4 ci = cvec[i];
+  G4 pi{}, vi{};
+  R2 rpi{};
+  C4 ci{};
+  if (live) {
+    pi = gpos[i];
+    vi = gvel[i];
+    rpi = rp[i];
+    ci = cvec[i];
+  }
+  {
     const real H2 = (real)u.H2;
     const ar inv_h = (ar)u.inv_h, eps2 = (ar)u.eps2, hv = (ar)u.hv, ac0 = (ar)kc.alpha_c0;
     const ar rhoi = (ar)rpi.x, pri = (ar)rpi.y;
     const ar cix = (ar)ci.x, ciy = (ar)ci.y, ciz = (ar)ci.z;
-    double xx = 0, xy = 0, xz = 0, vx = 0, vy = 0, vz = 0, pb = 0, rb = 0, sa = 0, sc0 = 0;
-    walk_runs<kUnr>(rec.meta, rec.words, n, i, [&](const uint32_t* jj) {
+    // sums: per row run in the pair precision (a few terms each), then FP64
+    double xx = 0, xy = 0, xz = 0, vx = 0, vy = 0, vz = 0, pb = 0, rb = 0, sa = 0;
+    ar qxx = 0, qxy = 0, qxz = 0, qvx = 0, qvy = 0, qvz = 0, qpb = 0, qrb = 0, qsa = 0;
+    auto end_run = [&] {
+      xx += (double)qxx; xy += (double)qxy; xz += (double)qxz;
+      vx += (double)qvx; vy += (double)qvy; vz += (double)qvz;
+      pb += (double)qpb; rb += (double)qrb; sa += (double)qsa;
+      qxx = qxy = qxz = qvx = qvy = qvz = qpb = qrb = qsa = ar(0.0);
+    };
+    // pair rows over a batch of neighbours fetched by get(j, p, v, rp, c)
+    auto body = [&](const uint32_t* jj, auto&& get) {
       G4 pj[kUnr], vj[kUnr];
       R2 rpj[kUnr];
       C4 cj[kUnr];
 #pragma unroll
-      for (int v = 0; v < kUnr; ++v) {
-        const uint32_t j = jj[v] == kPad ? jj[0] : jj[v];
-        pj[v] = gpos[j];
-        vj[v] = gvel[j];
-        rpj[v] = rp[j];
-        cj[v] = cvec[j];
-      }
+      for (int v = 0; v < kUnr; ++v) get(jj[v] == kPad ? jj[0] : jj[v], pj[v], vj[v], rpj[v], cj[v]);
 #pragma unroll
       for (int v = 0; v < kUnr; ++v) {
         if (jj[v] == kPad) continue;
@@ -151,14 +189,16 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
         real r2r = dxr * dxr + dyr * dyr + dzr * dzr;
         if (!(r2r < H2 && r2r > real(0.0))) continue;
         const ar dx = (ar)dxr, dy = (ar)dyr, dz = (ar)dzr, r2 = (ar)r2r;
-        const ar rr = sqrt(r2), irr = ar(1.0) / rr, q = rr * inv_h;
+        ar rr, irr;
+        adj_len(r2, rr, irr);
+        const ar q = rr * inv_h;
         const ar f1 = spline_df(q), f2 = spline_d2f(q);
         const ar gr = f1 * irr;
         const ar gp = (f2 * inv_h - gr) * irr;   // d(f'(q)/r)/dr
         const ar ex = (ar)vi.x - (ar)vj[v].x, ey = (ar)vi.y - (ar)vj[v].y, ez = (ar)vi.z - (ar)vj[v].z;
         const ar w = ex * dx + ey * dy + ez * dz;
-        const ar D = r2 + eps2, iD = ar(1.0) / D;
-        const ar rbar = ar(0.5) * (rhoi + (ar)rpj[v].x), irb = ar(1.0) / rbar;
+        const ar D = r2 + eps2, iD = adj_rcp(D);
+        const ar rbar = ar(0.5) * (rhoi + (ar)rpj[v].x), irb = adj_rcp(rbar);
         ar mu = ar(0.0), Pi = ar(0.0);
         if (w < ar(0.0)) {
           mu = hv * w * iD;
@@ -170,57 +210,84 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
         const ar U = Cx * dx + Cy * dy + Cz * dz;
         const ar Ug = U * gr;
         const ar t = U * Q * gp * irr;
-        xx += (double)(S * Cx + t * dx);
-        xy += (double)(S * Cy + t * dy);
-        xz += (double)(S * Cz + t * dz);
-        pb += (double)Ug;
+        qxx += S * Cx + t * dx;
+        qxy += S * Cy + t * dy;
+        qxz += S * Cz + t * dz;
+        qpb += Ug;
         if (w < ar(0.0)) {
           const ar mub = -Ug * ac0 * irb;
           const ar hD = hv * iD, k2 = ar(2.0) * hv * w * iD * iD;
-          xx += (double)(mub * (hD * ex - k2 * dx));
-          xy += (double)(mub * (hD * ey - k2 * dy));
-          xz += (double)(mub * (hD * ez - k2 * dz));
-          vx += (double)(mub * hD * dx);
-          vy += (double)(mub * hD * dy);
-          vz += (double)(mub * hD * dz);
-          rb += (double)(Ug * ac0 * mu * ar(0.5) * irb * irb);
-          const ar T1 = (cix * dx + ciy * dy + ciz * dz) * gr * mu * irb;
-          sa -= (double)T1;    // times c0 below
-          sc0 -= (double)T1;   // times alpha below
+          qxx += mub * (hD * ex - k2 * dx);
+          qxy += mub * (hD * ey - k2 * dy);
+          qxz += mub * (hD * ez - k2 * dz);
+          qvx += mub * hD * dx;
+          qvy += mub * hD * dy;
+          qvz += mub * hD * dz;
+          qrb += Ug * ac0 * mu * ar(0.5) * irb * irb;
+          // d/dalpha = c0 sum, d/dc0 = alpha sum (Pi is linear in alpha c0)
+          qsa -= (cix * dx + ciy * dy + ciz * dz) * gr * mu * irb;
         }
       }
-    });
-    pa = sa * c0;
-    pc = sc0 * alpha;
-    // EOS: P = p / rho^2, p = B((rho/rho0)^7 - 1) unless the forward clamped
-    // it to 0 (then P = 0 and the EOS has no derivative)
-    const double rho = (double)rpi.x;
-    const double ratio = rho * kc.inv_rho0;
-    const double r6 = ratio * ratio * ratio * ratio * ratio * ratio;
-    const double p = (double)rpi.y * rho * rho;
-    if ((double)rpi.y > 0.0) {
-      const double dpdrho = 7.0 * kc.B * r6 * kc.inv_rho0;
-      const double dPdrho = dpdrho / (rho * rho) - 2.0 * p / (rho * rho * rho);
-      rb += pb * dPdrho;
-      const double dPdc0 = (r6 * ratio - 1.0) * (2.0 * rho0 * c0 / 7.0) / (rho * rho);
-      pc += pb * dPdc0;
-    }
-    rhob[i] = (ar)rb;
-    const double us = u.unit_scale;   // pair-geometry units -> metres for position adjoints
-    xbf[i] = make_double4(us * xx, us * xy, us * xz, 0.0);
-    if ((uint64_t)rec.ids[i] < n_moving) {
-      double4 v0 = vb[i];
-      vb[i] = make_double4(v0.x + vx, v0.y + vy, v0.z + vz, 0.0);
+    };
+    walk_runs_staged<kUnr, kForceCap>(
+        rec.meta, rec.words, n, i, live, lane,
+        [&](uint32_t k, uint32_t j) {
+          s_p[k] = gpos[j];
+          s_v[k] = gvel[j];
+          s_rp[k] = rp[j];
+          s_c[k] = cvec[j];
+        },
+        [&](const uint32_t* jj) {
+          body(jj, [&](uint32_t k, G4& p, G4& v, R2& r, C4& cc) {
+            p = s_p[k];
+            v = s_v[k];
+            r = s_rp[k];
+            cc = s_c[k];
+          });
+        },
+        [&](const uint32_t* jj) {
+          body(jj, [&](uint32_t j, G4& p, G4& v, R2& r, C4& cc) {
+            p = gpos[j];
+            v = gvel[j];
+            r = rp[j];
+            cc = cvec[j];
+          });
+        },
+        end_run);
+    if (live) {
+      pa = sa * c0;
+      pc = sa * alpha;
+      // EOS: P = p / rho^2, p = B((rho/rho0)^7 - 1) unless the forward clamped
+      // it to 0 (then P = 0 and the EOS has no derivative)
+      const double rho = (double)rpi.x;
+      const double ratio = rho * kc.inv_rho0;
+      const double r6 = ratio * ratio * ratio * ratio * ratio * ratio;
+      const double p = (double)rpi.y * rho * rho;
+      if ((double)rpi.y > 0.0) {
+        const double dpdrho = 7.0 * kc.B * r6 * kc.inv_rho0;
+        const double dPdrho = dpdrho / (rho * rho) - 2.0 * p / (rho * rho * rho);
+        rb += pb * dPdrho;
+        const double dPdc0 = (r6 * ratio - 1.0) * (2.0 * rho0 * c0 / 7.0) / (rho * rho);
+        pc += pb * dPdc0;
+      }
+      rhob[i] = (ar)rb;
+      const double us = u.unit_scale;   // pair-geometry units -> metres for position adjoints
+      xbf[i] = make_double4(us * xx, us * xy, us * xz, 0.0);
+      if ((uint64_t)rec.ids[i] < n_moving) {
+        double4 v0 = vb[i];
+        vb[i] = make_double4(v0.x + vx, v0.y + vy, v0.z + vz, 0.0);
+      }
     }
   }
-  block_sum2(pa, pc, partial, blockIdx.x);
+  block_sum2<kAdjBlock / 64>(pa, pc, partial, blockIdx.x);
 }
 
 // Density adjoint + assembly: xbar_k = xbar'_k + force part + density part
 // (moving particles; fixed ones 0), then both adjoints move to the previous
-// step's slot order through the frozen permutation.
+// step's slot order through the frozen permutation.  One wavefront per
+// block, neighbour data staged per row run.
 template <int F>
-__global__ __launch_bounds__(kBlock) void k_adj_density(
+__global__ __launch_bounds__(kAdjBlock) void k_adj_density(
     uint32_t n, uint64_t n_moving, SphStepRecord rec, Grid g, SphConst kc, SphUnits u,
     const typename AdjForm<F>::ar* __restrict__ rhob, const double4* __restrict__ xb_in,
     const double4* __restrict__ xbf, const double4* __restrict__ vb_in, double4* __restrict__ xb_out,
@@ -229,28 +296,26 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(
   using G4 = typename SphForm<F>::G4;
   using ar = typename AdjForm<F>::ar;
   const G4* gpos = (const G4*)rec.gpos;
-  const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t dst = rec.perm[i];
-  if ((uint64_t)rec.ids[i] >= n_moving) {
-    xb_out[dst] = make_double4(0, 0, 0, 0);
-    vb_out[dst] = make_double4(0, 0, 0, 0);
-    return;
+  __shared__ G4 s_p[kForceCap];
+  __shared__ ar s_rb[kForceCap];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kAdjBlock + lane;
+  const bool live = i < n && (uint64_t)rec.ids[i] < n_moving;
+  G4 pi{};
+  ar rbi = ar(0.0);
+  if (live) {
+    pi = gpos[i];
+    rbi = rhob[i];
   }
-  const G4 pi = gpos[i];
-  const ar rbi = rhob[i];
   const real H2 = (real)u.H2;
   const ar inv_h = (ar)u.inv_h;
   double sx = 0, sy = 0, sz = 0;
-  walk_runs<kUnr>(rec.meta, rec.words, n, i, [&](const uint32_t* jj) {
+  ar qx = 0, qy = 0, qz = 0;  // per row run, then FP64
+  auto body = [&](const uint32_t* jj, auto&& get) {
     G4 pj[kUnr];
     ar rbj[kUnr];
 #pragma unroll
-    for (int v = 0; v < kUnr; ++v) {
-      const uint32_t j = jj[v] == kPad ? jj[0] : jj[v];
-      pj[v] = gpos[j];
-      rbj[v] = rhob[j];
-    }
+    for (int v = 0; v < kUnr; ++v) get(jj[v] == kPad ? jj[0] : jj[v], pj[v], rbj[v]);
 #pragma unroll
     for (int v = 0; v < kUnr; ++v) {
       if (jj[v] == kPad) continue;
@@ -258,13 +323,45 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(
       form_pair<F>(pi, pj[v], u.cs_u, dxr, dyr, dzr);
       real r2r = dxr * dxr + dyr * dyr + dzr * dzr;
       if (!(r2r < H2 && r2r > real(0.0))) continue;
-      const ar rr = sqrt((ar)r2r);
-      const ar s = (rbi + rbj[v]) * spline_df(rr * inv_h) / rr;
-      sx += (double)(s * (ar)dxr);
-      sy += (double)(s * (ar)dyr);
-      sz += (double)(s * (ar)dzr);
+      ar rr, irr;
+      adj_len((ar)r2r, rr, irr);
+      const ar s = (rbi + rbj[v]) * spline_df(rr * inv_h) * irr;
+      qx += s * (ar)dxr;
+      qy += s * (ar)dyr;
+      qz += s * (ar)dzr;
     }
-  });
+  };
+  walk_runs_staged<kUnr, kForceCap>(
+      rec.meta, rec.words, n, i, live, lane,
+      [&](uint32_t k, uint32_t j) {
+        s_p[k] = gpos[j];
+        s_rb[k] = rhob[j];
+      },
+      [&](const uint32_t* jj) {
+        body(jj, [&](uint32_t k, G4& p, ar& rb) {
+          p = s_p[k];
+          rb = s_rb[k];
+        });
+      },
+      [&](const uint32_t* jj) {
+        body(jj, [&](uint32_t j, G4& p, ar& rb) {
+          p = gpos[j];
+          rb = rhob[j];
+        });
+      },
+      [&] {
+        sx += (double)qx;
+        sy += (double)qy;
+        sz += (double)qz;
+        qx = qy = qz = ar(0.0);
+      });
+  if (i >= n) return;
+  const uint32_t dst = rec.perm[i];
+  if (!live) {
+    xb_out[dst] = make_double4(0, 0, 0, 0);
+    vb_out[dst] = make_double4(0, 0, 0, 0);
+    return;
+  }
   const double k = kc.msig * u.inv_h * u.unit_scale;
   const double4 a = xb_in[i], b = xbf[i];
   xb_out[dst] = make_double4(a.x + b.x + k * sx, a.y + b.y + k * sy, a.z + b.z + k * sz, 0.0);
@@ -309,7 +406,7 @@ template <int F>
 void sph_reverse_step(Ctx& c, const SphStepRecord& rec, AdjBufs& A) {
   const uint32_t n = (uint32_t)c.n;
   const SphUnits u = sph_units<F>(c);
-  const uint32_t grid = xcd_grid(blocks_for(n));
+  const uint32_t grid = xcd_grid(blocks_for(n, kAdjBlock));
   {
     LaunchScope ls(&c, "adj_prep", 2.0 * 32.0 * n + 32.0 * n);
     k_adj_prep<F><<<blocks_for(n), kBlock, 0, c.stream>>>(n, c.n_moving, rec.ids, A.xb, A.vb,
@@ -319,7 +416,7 @@ void sph_reverse_step(Ctx& c, const SphStepRecord& rec, AdjBufs& A) {
   {
     const double ge = (double)sizeof(typename SphForm<F>::real);
     LaunchScope ls(&c, F == kF64 ? "adj_force_f64" : "adj_force_mixed", (8.0 * ge + 24.0 + 24.0 + 8.0 + 48.0) * n);
-    k_adj_force<F><<<grid, kBlock, 0, c.stream>>>(n, c.n_moving, rec, c.grid, c.sph, u, c.desc.alpha, c.desc.c0,
+    k_adj_force<F><<<grid, kAdjBlock, 0, c.stream>>>(n, c.n_moving, rec, c.grid, c.sph, u, c.desc.alpha, c.desc.c0,
                                                  c.desc.rho0, (const typename AdjForm<F>::C4*)A.cvec, A.xbf, A.vb,
                                                  (typename AdjForm<F>::ar*)A.rhob, A.partial);
     PR_HIP(hipGetLastError());
@@ -330,7 +427,7 @@ void sph_reverse_step(Ctx& c, const SphStepRecord& rec, AdjBufs& A) {
     const double ge = (double)sizeof(typename SphForm<F>::real);
     LaunchScope ls(&c, F == kF64 ? "adj_density_f64" : "adj_density_mixed",
                    (3.0 * ge + 8.0 + 24.0 * 3 + 24.0 * 2 + 8.0) * n);
-    k_adj_density<F><<<grid, kBlock, 0, c.stream>>>(n, c.n_moving, rec, c.grid, c.sph, u,
+    k_adj_density<F><<<grid, kAdjBlock, 0, c.stream>>>(n, c.n_moving, rec, c.grid, c.sph, u,
                                                    (const typename AdjForm<F>::ar*)A.rhob, A.xb, A.xbf, A.vb,
                                                    A.xb2, A.vb2);
     PR_HIP(hipGetLastError());
@@ -414,7 +511,7 @@ template <int F>
 void sph_adjoint(Ctx& c, const double* seed_x, const double* seed_v, double* grad_params, double* grad_v0) {
   Tape& T = c.tape;
   const uint32_t n = (uint32_t)c.n;
-  const uint32_t grid = xcd_grid(blocks_for(n));
+  const uint32_t grid = xcd_grid(blocks_for(n, kAdjBlock));
   ensure_adj_buffers(c, grid);
   AdjBufs A{T.xb.as<double4>(), T.vb.as<double4>(), T.xb2.as<double4>(), T.vb2.as<double4>(),
             T.cvec.p, T.xbf.as<double4>(), T.rhob.p, T.partial.as<double2>(),

@@ -298,43 +298,7 @@ __device__ inline void walk_bits(uint64_t m0, uint64_t m1, Fn&& fn) {
   }
 }
 
-// Visit the stored neighbours of slot i run by run in scan order, B at a
-// time: fn(jj) receives slot indices, kPad for unused batch entries (jj[0]
-// is always a slot).
-template <int B, class Fn>
-__device__ inline void walk_runs(const uint2* __restrict__ meta, const uint32_t* __restrict__ words, uint32_t n,
-                                 uint32_t i, Fn&& fn) {
-  for (uint32_t r = 0; r < kRuns; ++r) {
-    uint32_t j0;
-    uint64_t m0, m1;
-    load_run(meta, words, r, n, i, j0, m0, m1);
-    walk_bits<B>(m0, m1, [&](const uint32_t* jb) {
-      uint32_t jj[B];
-#pragma unroll
-      for (int v = 0; v < B; ++v) jj[v] = jb[v] == kPad ? kPad : j0 + jb[v];
-      fn(jj);
-    });
-  }
-}
-
-// ---------------------------------------------------------------------------
-// density + EOS: rho_i = m sigma sum_j f(|x_ij|/h) over j with |x_ij| < 2h
-// (j == i included), p_i = max(B((rho_i/rho0)^7 - 1), 0), pr_i = p_i / rho_i^2.
-// Per row run: probe scan -> run mask, appended to the neighbour stream for
-// the force pass -> walk of the mask's set bits in the form's precision.
-//
-// One wavefront per block.  For each row run the wavefront first copies the
-// union of its lanes' candidate ranges into LDS as FP32 scan offsets (one
-// coalesced pass; the lanes of a wavefront share most candidates), and every
-// lane scans its own range from there.  A union larger than kStageCap slots
-// (a wavefront spread over distant rows) is scanned from global memory.
-// ---------------------------------------------------------------------------
-constexpr int kDensBlock = 64;  // one wavefront: block barriers are wave barriers
-#ifndef PRESTIGE_STAGE_CAP
-#define PRESTIGE_STAGE_CAP 256
-#endif
-constexpr uint32_t kStageCap = PRESTIGE_STAGE_CAP;
-
+// Minimum / maximum over the wavefront, returned as a wave-uniform value.
 __device__ inline uint32_t wave_min(uint32_t v) {
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) {
@@ -352,6 +316,68 @@ __device__ inline uint32_t wave_max(uint32_t v) {
   }
   return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
+
+// Per-run walk of a wavefront (one wavefront per block; every lane calls it,
+// live or not).  For each run the wavefront copies the data of the union of
+// its lanes' hit ranges into LDS with stage(k, j) (slot j -> stage entry k),
+// then each lane walks its own mask B at a time with lds(jj) (jj: stage
+// entries, kPad for unused batch entries, jj[0] always valid).  A union wider
+// than CAP slots is walked with glob(jj) (jj: slots) instead.  Lanes visit
+// their neighbours in scan order either way; end_run() follows each run.
+template <int B, uint32_t CAP, class Stage, class Lds, class Glob, class End>
+__device__ inline void walk_runs_staged(const uint2* __restrict__ meta, const uint32_t* __restrict__ words,
+                                        uint32_t n, uint32_t i, bool live, uint32_t lane, Stage&& stage, Lds&& lds,
+                                        Glob&& glob, End&& end_run) {
+  for (uint32_t r = 0; r < kRuns; ++r) {
+    uint32_t j0 = 0;
+    uint64_t m0 = 0, m1 = 0;
+    if (live) load_run(meta, words, r, n, i, j0, m0, m1);
+    uint32_t a = 0xFFFFFFFFu, b = 0u;
+    if (m0 | m1) {
+      a = j0 + (m0 ? (uint32_t)__builtin_ctzll(m0) : 64u + (uint32_t)__builtin_ctzll(m1));
+      b = j0 + (m1 ? 128u - (uint32_t)__builtin_clzll(m1) : 64u - (uint32_t)__builtin_clzll(m0));
+    }
+    const uint32_t base = wave_min(a), top = wave_max(b);
+    if (top <= base) continue;
+    if (top - base <= CAP) {
+      for (uint32_t k = lane; k < top - base; k += 64u) stage(k, base + k);
+      __syncthreads();
+      walk_bits<B>(m0, m1, [&](const uint32_t* jb) {
+        uint32_t jj[B];
+#pragma unroll
+        for (int v = 0; v < B; ++v) jj[v] = jb[v] == kPad ? kPad : j0 + jb[v] - base;
+        lds(jj);
+      });
+      __syncthreads();  // the next run's staging overwrites the stage
+    } else {
+      walk_bits<B>(m0, m1, [&](const uint32_t* jb) {
+        uint32_t jj[B];
+#pragma unroll
+        for (int v = 0; v < B; ++v) jj[v] = jb[v] == kPad ? kPad : j0 + jb[v];
+        glob(jj);
+      });
+    }
+    end_run();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// density + EOS: rho_i = m sigma sum_j f(|x_ij|/h) over j with |x_ij| < 2h
+// (j == i included), p_i = max(B((rho_i/rho0)^7 - 1), 0), pr_i = p_i / rho_i^2.
+// Per row run: probe scan -> run mask (stored for the force pass) -> walk of
+// the mask's set bits in the form's precision.
+//
+// One wavefront per block.  For each row run the wavefront first copies the
+// union of its lanes' candidate ranges into LDS as FP32 scan offsets (one
+// coalesced pass; the lanes of a wavefront share most candidates), and every
+// lane scans its own range from there.  A union larger than kStageCap slots
+// (a wavefront spread over distant rows) is scanned from global memory.
+// ---------------------------------------------------------------------------
+constexpr int kDensBlock = 64;  // one wavefront: block barriers are wave barriers
+#ifndef PRESTIGE_STAGE_CAP
+#define PRESTIGE_STAGE_CAP 256
+#endif
+constexpr uint32_t kStageCap = PRESTIGE_STAGE_CAP;
 
 template <int F>
 __global__ __launch_bounds__(kDensBlock) void k_sph_density(
@@ -604,29 +630,14 @@ __global__ __launch_bounds__(kForceBlock) void k_sph_force(
       }
     }
   };
-  for (uint32_t r = 0; r < kRuns; ++r) {
-    uint32_t j0 = 0;
-    uint64_t m0 = 0, m1 = 0;
-    if (live) load_run(meta, words, r, n, i, j0, m0, m1);
-    // union of the lanes' hit ranges
-    uint32_t a = 0xFFFFFFFFu, b = 0u;
-    if (m0 | m1) {
-      a = j0 + (m0 ? (uint32_t)__builtin_ctzll(m0) : 64u + (uint32_t)__builtin_ctzll(m1));
-      b = j0 + (m1 ? 128u - (uint32_t)__builtin_clzll(m1) : 64u - (uint32_t)__builtin_clzll(m0));
-    }
-    const uint32_t base = wave_min(a), top = wave_max(b);
-    if (top <= base) continue;
-    if (top - base <= kForceCap) {
-      for (uint32_t k = lane; k < top - base; k += kForceBlock) {
-        s_p[k] = gpos[base + k];
-        s_v[k] = gvel[base + k];
-        if constexpr (F == kMixed) s_rp[k] = rp[base + k];
-      }
-      __syncthreads();
-      walk_bits<kUnr>(m0, m1, [&](const uint32_t* jb) {
-        uint32_t jj[kUnr];
-#pragma unroll
-        for (int v = 0; v < kUnr; ++v) jj[v] = jb[v] == kPad ? kPad : j0 + jb[v] - base;
+  walk_runs_staged<kUnr, kForceCap>(
+      meta, words, n, i, live, lane,
+      [&](uint32_t k, uint32_t j) {
+        s_p[k] = gpos[j];
+        s_v[k] = gvel[j];
+        if constexpr (F == kMixed) s_rp[k] = rp[j];
+      },
+      [&](const uint32_t* jj) {
         body(jj, [&](uint32_t k, G4& p, G4& v, real& rho, real& pr) {
           p = s_p[k];
           v = s_v[k];
@@ -639,13 +650,8 @@ __global__ __launch_bounds__(kForceBlock) void k_sph_force(
             pr = v.w;
           }
         });
-      });
-      __syncthreads();  // the next run's staging overwrites the stage
-    } else {
-      walk_bits<kUnr>(m0, m1, [&](const uint32_t* jb) {
-        uint32_t jj[kUnr];
-#pragma unroll
-        for (int v = 0; v < kUnr; ++v) jj[v] = jb[v] == kPad ? kPad : j0 + jb[v];
+      },
+      [&](const uint32_t* jj) {
         body(jj, [&](uint32_t j, G4& p, G4& v, real& rho, real& pr) {
           p = gpos[j];
           v = gvel[j];
@@ -658,9 +664,8 @@ __global__ __launch_bounds__(kForceBlock) void k_sph_force(
             pr = v.w;
           }
         });
-      });
-    }
-  }
+      },
+      [] {});
   if (i >= n) return;
   if (!live) {
     opos[i] = spos[i];
