@@ -192,13 +192,14 @@ prestige_status prestige_timing_get(prestige_ctx* ctx, uint32_t idx,
 prestige_status prestige_sync(prestige_ctx* ctx);
 
 /* Last error message of this context ("" if none; static storage valid until
- * the next call on ctx).  ctx may be NULL for creation errors. */
+ * the next call on ctx).  ctx may be NULL for the errors of calls without a
+ * context (creation, equation definition and code generation). */
 const char* prestige_last_error(const prestige_ctx* ctx);
 
 /* ---- equation interface (prestige/src/equations/ir.rs:3-14, fuse.rs:14-40,
  *      codegen/simple_cpu.rs:3-22) -------------------------------------- */
 
-/* Describe built-in equation eq_id: name, read and write field names
+/* Describe equation eq_id (built-in or defined): name, read and write field names
  * (comma-separated, sorted; every indexed name is a read, LHS included, as in
  * prestige_macros/src/lib.rs:71-133), the i/j role of every read (one
  * PRESTIGE_ROLE_* mask per read, same order) and the Rust body text.  Buffers
@@ -211,8 +212,10 @@ prestige_status prestige_equation_describe(int32_t eq_id, char* name, size_t nam
 
 /* Fuse equations eq_ids[0..n) in order (fuse.rs:20-31) and split them into
  * gather passes wherever a body reads at [j] a field an earlier body of the
- * same pass writes.  pass_of[k] receives the pass index of eq_ids[k];
- * *n_passes the count. */
+ * same pass writes, or a body with [j] accesses reads a field an earlier
+ * i-only body of the pass writes (i-only bodies run once per i after the j
+ * loop).  pass_of[k] receives the pass index of eq_ids[k]; *n_passes the
+ * count. */
 prestige_status prestige_fuse_schedule(const int32_t* eq_ids, uint32_t n,
                                        uint32_t* pass_of, uint32_t* n_passes);
 
@@ -221,6 +224,41 @@ prestige_status prestige_fuse_schedule(const int32_t* eq_ids, uint32_t n,
  * (without NUL); the text is truncated to cap-1 bytes. */
 prestige_status prestige_generate_simple_cpu(const int32_t* eq_ids, uint32_t n,
                                              char* out, size_t cap, size_t* len);
+
+/* Register an equation (no counterpart in the reference, where #[equation]
+ * runs at compile time; prestige_macros/src/lib.rs:8-58): name is an
+ * identifier, body the block text the macro captures (quote!(#body), e.g.
+ * "{ force [i] += mass [j] ; }", any spacing).  Reads and writes follow the
+ * macro's rule (prestige_macros/src/lib.rs:71-133); each field also gets the
+ * i/j role of its accesses (the index inside the brackets).  *eq_id receives
+ * the new id (>= PRESTIGE_EQ_COUNT).  The registry is process-wide. */
+prestige_status prestige_equation_define(const char* name, const char* body, int32_t* eq_id);
+
+/* HIP source of the fused equations' gather passes: the GPU sibling of
+ * prestige_generate_simple_cpu (prestige/src/codegen/simple_cpu.rs:3-22).
+ * One extern "C" kernel per pass of prestige_fuse_schedule, in loop form
+ * PRESTIGE_LOOP_ALLPAIRS (thread = id i, j over ids 0..n) or
+ * PRESTIGE_LOOP_CELLS (thread = slot i of the last build, j over the nine row
+ * runs of its cell with |x_i - x_j|^2 < cutoff^2).  Bodies may use fields at
+ * [i] and [j], the positions x y z (read only), the descriptor constants h dt
+ * rho0 c0 alpha kn kt mu gx gy gz, the helpers kernel_w(dx, dy, dz, h) (cubic
+ * spline) and tait(rho, rho0, c0), let bindings, if statements and f64
+ * functions in method (a.sqrt(), a.powi(2)) or path (f64::exp(a)) form.
+ * Other constructs (tuples, if expressions, writes through [j] or to x y z)
+ * return PRESTIGE_E_UNSUPPORTED.  Text as prestige_generate_simple_cpu. */
+prestige_status prestige_generate_hip(const int32_t* eq_ids, uint32_t n, int32_t loop,
+                                      char* out, size_t cap, size_t* len);
+
+/* Compile the passes of prestige_generate_hip for this device (hiprtc, kept
+ * per context) and run them in order on the context's particles.  Fields are
+ * FP64 arrays in particle-id order: eq1's force and mass, and named fields
+ * (prestige_field_upload / _download with any identifier that is not a field
+ * of the context's model).  A field the equations write is created
+ * zero-filled when absent; one they only read must have been uploaded
+ * (E_STATE).  cutoff is used by PRESTIGE_LOOP_CELLS, 0 < cutoff <= cell_size.
+ * i-only bodies run once per i after the j loop. */
+prestige_status prestige_run_equations(prestige_ctx* ctx, const int32_t* eq_ids, uint32_t n,
+                                       int32_t loop, double cutoff);
 
 /* ---- scenes (host only; no device needed) -------------------------------
  * Deterministic particle layouts of the benchmark configurations, drawn with

@@ -22,7 +22,7 @@ import numpy as np
 
 __all__ = [
     "Desc", "Context", "PrestigeError", "load_library", "scene", "equation_describe",
-    "fuse_schedule", "generate_simple_cpu", "MODEL_PAIRS", "MODEL_WCSPH", "MODEL_DEM",
+    "fuse_schedule", "generate_simple_cpu", "define_equation", "generate_hip", "MODEL_PAIRS", "MODEL_WCSPH", "MODEL_DEM",
     "F64", "F32", "MIXED", "LOOP_ALLPAIRS", "LOOP_CELLS", "EQ1", "EQ_SPH_DENSITY", "EQ_SPH_EOS",
     "EQ_SPH_MOMENTUM", "EQ_INTEGRATE", "EQ_DEM_CONTACT", "SCENE_DEM_LATTICE", "SCENE_DAM_BREAK",
     "SCENE_DEM_POUR", "ROLE_I", "ROLE_J",
@@ -127,6 +127,10 @@ def _bind(lib):
         "prestige_fuse_schedule": (S, [C.POINTER(C.c_int32), C.c_uint32, _U32, _U32]),
         "prestige_generate_simple_cpu": (S, [C.POINTER(C.c_int32), C.c_uint32, C.c_char_p, C.c_size_t,
                                              C.POINTER(C.c_size_t)]),
+        "prestige_equation_define": (S, [C.c_char_p, C.c_char_p, C.POINTER(C.c_int32)]),
+        "prestige_generate_hip": (S, [C.POINTER(C.c_int32), C.c_uint32, C.c_int32, C.c_char_p, C.c_size_t,
+                                      C.POINTER(C.c_size_t)]),
+        "prestige_run_equations": (S, [_P, C.POINTER(C.c_int32), C.c_uint32, C.c_int32, C.c_double]),
         "prestige_scene_count": (S, [C.c_int32, C.POINTER(SceneParams), C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_uint64)]),
         "prestige_scene_build": (S, [C.c_int32, C.POINTER(SceneParams), C.POINTER(Desc), _D, _D, _D, _D, _D,
@@ -244,6 +248,11 @@ class Context:
     def eval_equation(self, eq_id: int, loop: int, cutoff: float = 0.0) -> None:
         self._c(self._lib.prestige_eval_equation(self._h, eq_id, loop, float(cutoff)))
 
+    def run_equations(self, eq_ids, loop: int, cutoff: float = 0.0) -> None:
+        """Compile (once per context) and run the generated gather passes of the fused equations."""
+        ids = (C.c_int32 * max(1, len(eq_ids)))(*eq_ids)
+        self._c(self._lib.prestige_run_equations(self._h, ids, len(eq_ids), loop, float(cutoff)))
+
     def step(self, nsteps: int = 1) -> None:
         self._c(self._lib.prestige_step(self._h, int(nsteps)))
 
@@ -327,6 +336,25 @@ def fuse_schedule(eq_ids) -> tuple[list, int]:
     npass = C.c_uint32()
     _check(lib.prestige_fuse_schedule(ids, len(eq_ids), out, C.byref(npass)))
     return [int(out[k]) for k in range(len(eq_ids))], int(npass.value)
+
+
+def define_equation(name: str, body: str) -> int:
+    """Register an equation body (the block #[equation] captures); returns its id."""
+    lib = load_library()
+    out = C.c_int32()
+    _check(lib.prestige_equation_define(name.encode(), body.encode(), C.byref(out)))
+    return int(out.value)
+
+
+def generate_hip(eq_ids, loop: int) -> str:
+    """HIP source of the fused equations' gather passes."""
+    lib = load_library()
+    ids = (C.c_int32 * max(1, len(eq_ids)))(*eq_ids)
+    n = C.c_size_t()
+    _check(lib.prestige_generate_hip(ids, len(eq_ids), loop, None, 0, C.byref(n)))
+    buf = C.create_string_buffer(n.value + 1)
+    _check(lib.prestige_generate_hip(ids, len(eq_ids), loop, buf, n.value + 1, C.byref(n)))
+    return buf.value.decode()
 
 
 def generate_simple_cpu(eq_ids) -> str:

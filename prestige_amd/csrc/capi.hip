@@ -2,6 +2,7 @@
 // lifetime, SoA field transfer by the reference's slice names, step and
 // neighbour-build dispatch, timing and error reporting.  No C++ exception
 // crosses this boundary.
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -272,9 +273,33 @@ static FieldRef find_field(Ctx& c, const std::string& name) {
   throw Error(PRESTIGE_E_INVALID, "unknown field '" + name + "'");
 }
 
+// Names of fields that generated equations own: any identifier that is not a
+// field of the context's model (state, eq1's force / mass, the key; WCSPH
+// outputs in WCSPH contexts).
+bool generic_field_name(const Ctx& c, const std::string& name) {
+  if (name.empty() || !(std::isalpha((unsigned char)name[0]) || name[0] == '_')) return false;
+  for (char ch : name)
+    if (!(std::isalnum((unsigned char)ch) || ch == '_')) return false;
+  static const char* model[] = {"x", "y", "z", "vx", "vy", "vz", "wx", "wy", "wz", "radius", "force", "mass", "key"};
+  for (const char* m : model)
+    if (name == m) return false;
+  if (c.desc.model != PRESTIGE_MODEL_PAIRS)
+    for (const char* m : {"rho", "p", "ax", "ay", "az"})
+      if (name == m) return false;
+  return true;
+}
+
 static void upload(Ctx& c, const std::string& name, const double* host, uint64_t n) {
   PR_CHECK(n == c.n, PRESTIGE_E_INVALID, "field length must equal n_particles");
   PR_CHECK(host != nullptr || n == 0, PRESTIGE_E_INVALID, "null data");
+  if (generic_field_name(c, name)) {
+    std::unique_ptr<DevBuf>& b = c.gfields[name];
+    if (!b) b.reset(new DevBuf());
+    b->alloc((n ? n : 1) * 8);
+    if (n) PR_HIP(hipMemcpyAsync(b->p, host, n * 8, hipMemcpyHostToDevice, c.stream));
+    PR_HIP(hipStreamSynchronize(c.stream));
+    return;
+  }
   if (name == "force" || name == "mass") {
     DevBuf& b = name == "force" ? c.force : c.mass;
     b.alloc((n ? n : 1) * 8);
@@ -322,6 +347,13 @@ static void upload(Ctx& c, const std::string& name, const double* host, uint64_t
 static void download(Ctx& c, const std::string& name, double* host, uint64_t n) {
   PR_CHECK(n == c.n, PRESTIGE_E_INVALID, "field length must equal n_particles");
   PR_CHECK(host != nullptr || n == 0, PRESTIGE_E_INVALID, "null output");
+  if (generic_field_name(c, name)) {
+    auto it = c.gfields.find(name);
+    PR_CHECK(it != c.gfields.end(), PRESTIGE_E_STATE, "field '" + name + "' was never uploaded or written");
+    if (n) PR_HIP(hipMemcpyAsync(host, it->second->p, n * 8, hipMemcpyDeviceToHost, c.stream));
+    PR_HIP(hipStreamSynchronize(c.stream));
+    return;
+  }
   FieldRef f = find_field(c, name);
   if (f.id_order) {
     PR_CHECK(name == "force" ? c.have_force : c.have_mass, PRESTIGE_E_STATE, "field '" + name + "' was never uploaded");
@@ -369,6 +401,13 @@ struct prestige_ctx : Ctx {};
 
 namespace {
 thread_local std::string g_create_error;
+}  // namespace
+
+namespace prestige {
+void set_context_free_error(const std::string& msg) { g_create_error = msg; }
+}  // namespace prestige
+
+namespace {
 
 template <class Fn>
 prestige_status guard(prestige_ctx* ctx, Fn&& fn) {
@@ -518,6 +557,11 @@ prestige_status prestige_eval_equation(prestige_ctx* ctx, int32_t eq_id, int32_t
     eq1_eval(*ctx, loop, cutoff);
     ctx->check_flags();
   });
+}
+
+prestige_status prestige_run_equations(prestige_ctx* ctx, const int32_t* eq_ids, uint32_t n, int32_t loop,
+                                      double cutoff) {
+  return guard(ctx, [&] { run_equations(*ctx, eq_ids, n, loop, cutoff); });
 }
 
 prestige_status prestige_step(prestige_ctx* ctx, uint32_t nsteps) {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <memory>
 #include <cstring>
 #include <stdexcept>
@@ -211,6 +212,16 @@ struct Ctx {
   Timing timing;
   Tape tape;
   bool keep_acc = true;
+  // named FP64 fields of generated equations (particle-id order), and the
+  // modules compiled for them, keyed by source text
+  std::map<std::string, std::unique_ptr<DevBuf>> gfields;
+  DevBuf posid;   // positions in id order for the all-pairs form of generated passes
+  struct JitCache {
+    std::map<std::string, hipModule_t> mods;
+    ~JitCache() {
+      for (auto& kv : mods) (void)hipModuleUnload(kv.second);
+    }
+  } jit;
 
   void check_flags();            // syncs; throws on a raised flag
 };
@@ -261,6 +272,8 @@ void sph_step_f32(Ctx& c);
 void sph_step_mixed(Ctx& c);
 void dem_step_f64(Ctx& c);
 void dem_step_mixed(Ctx& c);
+void run_equations(Ctx& c, const int32_t* eq_ids, uint32_t n, int32_t loop, double cutoff);
+bool generic_field_name(const Ctx& c, const std::string& name);
 void scatter_to_ids(Ctx& c, const void* src, int src_kind, int comp, double* d_out);
 void gather_from_ids(Ctx& c, const double* d_in, void* dst, int dst_kind, int comp);
 

@@ -3,40 +3,50 @@
 //   #[equation] read/write inference          prestige_macros/src/lib.rs:64-134
 //   fuse(Vec<EquationIR>) -> FusedEquations   prestige/src/equations/fuse.rs:14-40
 //   generate_simple_cpu(&FusedEquations)      prestige/src/codegen/simple_cpu.rs:3-22
+// and the HIP code generator that is the GPU sibling of generate_simple_cpu.
 //
-// Each built-in equation is stored as the Rust body the #[equation] macro
-// would capture, plus the read/write sets that macro infers from it (every
+// Each equation is the Rust body the #[equation] macro captures
+// (quote!(#body) text), the read/write sets that macro infers from it (every
 // `X[..]` is a read, LHS included; `X[..] =` and `X[..] op=` are writes) and
-// the i/j role of each read that the macro does not record.  The pass
-// scheduler is the part the reference leaves out: fused bodies run per (i, j)
-// pair in order, so a body that reads `F[j]` of a field an earlier body of
-// the same pass writes would see partial sums; such an equation starts a new
-// gather pass.  A body that only writes fields earlier bodies read at [j]
-// stays in the pass, because the device passes write their outputs to a
-// second buffer.  i-only bodies (no [j] read) commute to the end of the j
-// loop, which is why the EOS fuses into the density pass.
+// the i/j role of each field, which the macro does not record: the index
+// inside the brackets.  Built-in equations are listed below with their sets
+// written out; prestige_equation_define registers further bodies and infers
+// the sets with the same rule.
+//
+// Pass scheduling is the part the reference leaves out: fused bodies run per
+// (i, j) pair in order, so a body that reads `F[j]` of a field an earlier body
+// of the same pass writes would see partial sums; such an equation starts a
+// new gather pass.  A body that only writes fields earlier bodies read at [j]
+// stays in the pass: generated passes give such a field a separate output
+// array.  i-only bodies (no [j] access) commute to the end of the j loop and
+// run once per i, which is why the EOS fuses into the density pass; a body
+// with [j] accesses that reads a field an earlier i-only body of the pass
+// writes starts a new pass, so it sees the final value.
 #include <algorithm>
+#include <cctype>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <set>
 #include <string>
 #include <vector>
 
 #include "../../include/prestige_hip.h"
+#include "codegen.hpp"
 
+namespace prestige {
 namespace {
 
 struct Eq {
-  const char* name;
-  const char* body;                                   // as captured by quote!(#body)
-  std::vector<std::pair<std::string, unsigned>> reads;   // name, role bits
+  std::string name;
+  std::string body;                                     // as captured by quote!(#body)
+  std::vector<std::pair<std::string, unsigned>> reads;  // name, role bits
   std::vector<std::string> writes;
 };
 
-const std::vector<Eq>& table() {
-  static const std::vector<Eq> t = {
-      {"eq1", "{ force [i] += mass [j] ; }",
-       {{"force", PRESTIGE_ROLE_I}, {"mass", PRESTIGE_ROLE_J}},
-       {"force"}},
+std::vector<Eq> builtins() {
+  return {
+      {"eq1", "{ force [i] += mass [j] ; }", {{"force", PRESTIGE_ROLE_I}, {"mass", PRESTIGE_ROLE_J}}, {"force"}},
       {"sph_density",
        "{ rho [i] += mass [j] * kernel_w (x [i] - x [j] , y [i] - y [j] , z [i] - z [j] , h) ; }",
        {{"mass", PRESTIGE_ROLE_J},
@@ -45,8 +55,7 @@ const std::vector<Eq>& table() {
         {"y", PRESTIGE_ROLE_I | PRESTIGE_ROLE_J},
         {"z", PRESTIGE_ROLE_I | PRESTIGE_ROLE_J}},
        {"rho"}},
-      {"sph_eos", "{ p [i] = tait (rho [i] , rho0 , c0) ; }",
-       {{"p", PRESTIGE_ROLE_I}, {"rho", PRESTIGE_ROLE_I}},
+      {"sph_eos", "{ p [i] = tait (rho [i] , rho0 , c0) ; }", {{"p", PRESTIGE_ROLE_I}, {"rho", PRESTIGE_ROLE_I}},
        {"p"}},
       {"sph_momentum",
        "{ let f = momentum (x [i] - x [j] , y [i] - y [j] , z [i] - z [j] , vx [i] - vx [j] , "
@@ -102,7 +111,13 @@ const std::vector<Eq>& table() {
         {"z", PRESTIGE_ROLE_I | PRESTIGE_ROLE_J}},
        {"fx", "fy", "fz", "tx", "ty", "tz"}},
   };
-  return t;
+}
+
+// Built-ins first (ids 0 .. PRESTIGE_EQ_COUNT - 1), then defined equations.
+std::mutex g_mutex;
+std::vector<Eq>& registry() {
+  static std::vector<Eq> r = builtins();
+  return r;
 }
 
 void copy_out(const std::string& s, char* dst, size_t cap) {
@@ -115,11 +130,554 @@ void copy_out(const std::string& s, char* dst, size_t cap) {
 bool valid_ids(const int32_t* ids, uint32_t n) {
   if (n && !ids) return false;
   for (uint32_t k = 0; k < n; ++k)
-    if (ids[k] < 0 || ids[k] >= (int32_t)table().size()) return false;
+    if (ids[k] < 0 || ids[k] >= (int32_t)registry().size()) return false;
   return true;
 }
 
+bool is_ident(const std::string& s) {
+  if (s.empty() || !(std::isalpha((unsigned char)s[0]) || s[0] == '_')) return false;
+  for (char ch : s)
+    if (!(std::isalnum((unsigned char)ch) || ch == '_')) return false;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// body tokens
+// ---------------------------------------------------------------------------
+enum TokKind { kIdent, kNumber, kPunct };
+struct Tok {
+  TokKind kind;
+  std::string s;
+};
+
+std::vector<Tok> tokenize(const std::string& b) {
+  static const char* multi[] = {"<<=", ">>=", "+=", "-=", "*=", "/=", "%=", "^=", "&=", "|=", "==", "!=",
+                                "<=",  ">=",  "&&", "||", "::", "->", "<<", ">>", ".."};
+  std::vector<Tok> t;
+  size_t k = 0;
+  while (k < b.size()) {
+    const char ch = b[k];
+    if (std::isspace((unsigned char)ch)) {
+      ++k;
+      continue;
+    }
+    if (std::isalpha((unsigned char)ch) || ch == '_') {
+      size_t e = k;
+      while (e < b.size() && (std::isalnum((unsigned char)b[e]) || b[e] == '_')) ++e;
+      t.push_back({kIdent, b.substr(k, e - k)});
+      k = e;
+      continue;
+    }
+    if (std::isdigit((unsigned char)ch)) {
+      // digits [. digits] [e[+-]digits] [f64 | f32]; underscores dropped; a
+      // '.' not followed by a digit ends the number (a method call or field)
+      std::string num;
+      size_t e = k;
+      while (e < b.size() && (std::isdigit((unsigned char)b[e]) || b[e] == '_')) {
+        if (b[e] != '_') num += b[e];
+        ++e;
+      }
+      if (e + 1 < b.size() && b[e] == '.' && std::isdigit((unsigned char)b[e + 1])) {
+        num += '.';
+        ++e;
+        while (e < b.size() && (std::isdigit((unsigned char)b[e]) || b[e] == '_')) {
+          if (b[e] != '_') num += b[e];
+          ++e;
+        }
+      } else if (e < b.size() && b[e] == '.' && (e + 1 >= b.size() || !std::isalpha((unsigned char)b[e + 1]))) {
+        num += ".0";  // "1." is a float literal
+        ++e;
+      }
+      if (e < b.size() && (b[e] == 'e' || b[e] == 'E')) {
+        size_t f = e + 1;
+        if (f < b.size() && (b[f] == '+' || b[f] == '-')) ++f;
+        if (f < b.size() && std::isdigit((unsigned char)b[f])) {
+          num += b.substr(e, f - e);
+          e = f;
+          while (e < b.size() && std::isdigit((unsigned char)b[e])) num += b[e++];
+        }
+      }
+      for (const char* suf : {"_f64", "_f32", "f64", "f32"}) {
+        const size_t L = std::strlen(suf);
+        if (b.compare(e, L, suf) == 0) {
+          e += L;
+          break;
+        }
+      }
+      t.push_back({kNumber, num});
+      k = e;
+      continue;
+    }
+    bool found = false;
+    for (const char* m : multi) {
+      const size_t L = std::strlen(m);
+      if (b.compare(k, L, m) == 0) {
+        t.push_back({kPunct, m});
+        k += L;
+        found = true;
+        break;
+      }
+    }
+    if (found) continue;
+    if (std::strchr("{}[](),;.+-*/%<>=!&|^:", ch)) {
+      t.push_back({kPunct, std::string(1, ch)});
+      ++k;
+      continue;
+    }
+    throw GenError{PRESTIGE_E_INVALID, std::string("unexpected character '") + ch + "' in equation body"};
+  }
+  return t;
+}
+
+bool is_assign_op(const std::string& s) {
+  return s == "=" || s == "+=" || s == "-=" || s == "*=" || s == "/=" || s == "%=" || s == "^=" ||
+         s == "&=" || s == "|=" || s == "<<=" || s == ">>=";
+}
+
+// Index expression "X [ ... ]" at token k: returns the position of the
+// matching ']' or 0 when k does not start one.
+size_t index_close(const std::vector<Tok>& t, size_t k) {
+  if (k + 1 >= t.size() || t[k].kind != kIdent || t[k + 1].s != "[") return 0;
+  int depth = 0;
+  for (size_t m = k + 1; m < t.size(); ++m) {
+    if (t[m].s == "[") ++depth;
+    if (t[m].s == "]" && --depth == 0) return m;
+  }
+  throw GenError{PRESTIGE_E_INVALID, "unbalanced '[' in equation body"};
+}
+
+// The macro's rule (prestige_macros/src/lib.rs:71-133): every indexed path is
+// a read, an indexed path on the left of `=` or `op=` is a write; plus the
+// role of each access: the index inside the brackets (i, j, or other).
+void infer(const std::vector<Tok>& t, Eq& e) {
+  std::map<std::string, unsigned> roles;
+  std::set<std::string> writes;
+  for (size_t k = 0; k < t.size(); ++k) {
+    const size_t c = index_close(t, k);
+    if (!c) continue;
+    unsigned role = 0;
+    if (c == k + 3 && t[k + 2].s == "i") role = PRESTIGE_ROLE_I;
+    if (c == k + 3 && t[k + 2].s == "j") role = PRESTIGE_ROLE_J;
+    roles[t[k].s] |= role;
+    if (c + 1 < t.size() && is_assign_op(t[c + 1].s)) writes.insert(t[k].s);
+  }
+  e.reads.assign(roles.begin(), roles.end());
+  e.writes.assign(writes.begin(), writes.end());
+}
+
+// ---------------------------------------------------------------------------
+// HIP translation of one body
+// ---------------------------------------------------------------------------
+bool is_pos(const std::string& s) { return s == "x" || s == "y" || s == "z"; }
+
+// Descriptor constants a body may name.
+const std::set<std::string>& const_names() {
+  static const std::set<std::string> s = {"h", "dt", "rho0", "c0", "alpha", "kn", "kt", "mu", "gx", "gy", "gz"};
+  return s;
+}
+
+// Functions a body may call (Rust method or path form is rewritten to these).
+const std::map<std::string, std::string>& fn_map() {
+  static const std::map<std::string, std::string> m = {
+      {"sqrt", "sqrt"}, {"abs", "fabs"}, {"exp", "exp"},     {"ln", "log"},     {"log10", "log10"},
+      {"log2", "log2"}, {"powi", "pow"}, {"powf", "pow"},    {"sin", "sin"},    {"cos", "cos"},
+      {"tan", "tan"},   {"min", "fmin"}, {"max", "fmax"},    {"floor", "floor"}, {"ceil", "ceil"},
+      {"hypot", "hypot"}, {"atan2", "atan2"}};
+  return m;
+}
+
+// Device helpers provided to generated code.
+const std::set<std::string>& helper_names() {
+  static const std::set<std::string> s = {"kernel_w", "tait"};
+  return s;
+}
+
+size_t match_close(const std::vector<Tok>& t, size_t open) {
+  const std::string o = t[open].s;
+  const std::string c = o == "(" ? ")" : (o == "[" ? "]" : "}");
+  int depth = 0;
+  for (size_t m = open; m < t.size(); ++m) {
+    if (t[m].s == o) ++depth;
+    if (t[m].s == c && --depth == 0) return m;
+  }
+  throw GenError{PRESTIGE_E_INVALID, "unbalanced '" + o + "' in equation body"};
+}
+
+size_t match_open(const std::vector<Tok>& t, size_t close) {
+  const std::string c = t[close].s;
+  const std::string o = c == ")" ? "(" : (c == "]" ? "[" : "{");
+  int depth = 0;
+  for (size_t m = close + 1; m-- > 0;) {
+    if (t[m].s == c) ++depth;
+    if (t[m].s == o && --depth == 0) return m;
+  }
+  throw GenError{PRESTIGE_E_INVALID, "unbalanced '" + c + "' in equation body"};
+}
+
+// Rewrite Rust-only syntax into the C subset: `recv.m(args)` -> `f(recv,
+// args)`, `f64::m(..)` -> `f(..)`, `std::f64::consts::PI` -> literal,
+// `x as f64` -> `x`, `let [mut] v [: T] =` -> `double v =`, `if c {` ->
+// `if (c) {`, and a `;` before a closing brace where Rust allows none.
+std::vector<Tok> rewrite(std::vector<Tok> t, const std::string& eq) {
+  auto bad = [&](const std::string& what) {
+    return GenError{PRESTIGE_E_UNSUPPORTED, "equation '" + eq + "': " + what + " has no HIP translation"};
+  };
+  // paths
+  for (size_t k = 0; k + 2 < t.size(); ++k) {
+    if (t[k].s == "std" && t[k + 1].s == "::") {
+      t.erase(t.begin() + (long)k, t.begin() + (long)k + 2);
+      --k;
+      continue;
+    }
+    if ((t[k].s == "f64" || t[k].s == "f32") && t[k + 1].s == "::") {
+      if (t[k + 2].s == "consts" && k + 4 < t.size() && t[k + 3].s == "::") {
+        const std::string c = t[k + 4].s;
+        const char* v = c == "PI" ? "3.141592653589793" : (c == "E" ? "2.718281828459045" : nullptr);
+        if (!v) throw bad("constant '" + c + "'");
+        t.erase(t.begin() + (long)k, t.begin() + (long)k + 4);
+        t[k] = {kNumber, v};
+        continue;
+      }
+      if (!fn_map().count(t[k + 2].s)) throw bad("'" + t[k].s + "::" + t[k + 2].s + "'");
+      t.erase(t.begin() + (long)k, t.begin() + (long)k + 2);
+      t[k].s = fn_map().at(t[k].s);
+      continue;
+    }
+  }
+  // casts
+  for (size_t k = 0; k + 1 < t.size(); ++k) {
+    if (t[k].kind == kIdent && t[k].s == "as") {
+      if (t[k + 1].s != "f64" && t[k + 1].s != "f32") throw bad("cast to '" + t[k + 1].s + "'");
+      t.erase(t.begin() + (long)k, t.begin() + (long)k + 2);
+      --k;
+    }
+  }
+  // method calls, innermost receivers first (left to right)
+  for (;;) {
+    size_t dot = 0;
+    bool found = false;
+    for (size_t k = 1; k + 2 < t.size(); ++k) {
+      if (t[k].s == "." && t[k + 1].kind == kIdent && t[k + 2].s == "(") {
+        dot = k;
+        found = true;
+        break;
+      }
+    }
+    if (!found) break;
+    const std::string m = t[dot + 1].s;
+    if (!fn_map().count(m)) throw bad("method '" + m + "'");
+    size_t start = dot - 1;
+    if (t[start].s == ")" || t[start].s == "]") {
+      start = match_open(t, start);
+      if (start > 0 && t[start - 1].kind == kIdent) --start;  // f(..).m() or field[..].m()
+    } else if (t[start].kind != kIdent && t[start].kind != kNumber) {
+      throw bad("method call on '" + t[start].s + "'");
+    }
+    const size_t close = match_close(t, dot + 2);
+    std::vector<Tok> repl;
+    repl.push_back({kIdent, fn_map().at(m)});
+    repl.push_back({kPunct, "("});
+    repl.insert(repl.end(), t.begin() + (long)start, t.begin() + (long)dot);
+    if (close > dot + 3) {
+      repl.push_back({kPunct, ","});
+      repl.insert(repl.end(), t.begin() + (long)dot + 3, t.begin() + (long)close);
+    }
+    repl.push_back({kPunct, ")"});
+    t.erase(t.begin() + (long)start, t.begin() + (long)close + 1);
+    t.insert(t.begin() + (long)start, repl.begin(), repl.end());
+  }
+  // tuple fields have no translation
+  for (size_t k = 0; k + 1 < t.size(); ++k)
+    if (t[k].s == "." && t[k + 1].kind == kNumber) throw bad("tuple field access");
+  // let
+  for (size_t k = 0; k < t.size(); ++k) {
+    if (t[k].s != "let") continue;
+    size_t v = k + 1;
+    if (v < t.size() && t[v].s == "mut") t.erase(t.begin() + (long)v);
+    if (v >= t.size() || t[v].kind != kIdent) throw bad("pattern in let");
+    if (v + 1 < t.size() && t[v + 1].s == ":") {
+      if (v + 2 >= t.size() || (t[v + 2].s != "f64" && t[v + 2].s != "f32")) throw bad("non-float let binding");
+      t.erase(t.begin() + (long)v + 1, t.begin() + (long)v + 3);
+    }
+    t[k].s = "double";
+  }
+  // if statements
+  for (size_t k = 0; k < t.size(); ++k) {
+    if (t[k].s != "if") continue;
+    const std::string prev = k ? t[k - 1].s : "{";
+    if (prev != "{" && prev != ";" && prev != "}" && prev != "else") throw bad("if expression");
+    size_t b = k + 1;
+    int depth = 0;
+    for (; b < t.size(); ++b) {
+      if (t[b].s == "(" || t[b].s == "[") ++depth;
+      if (t[b].s == ")" || t[b].s == "]") --depth;
+      if (t[b].s == "{" && depth == 0) break;
+    }
+    if (b >= t.size()) throw bad("if without block");
+    t.insert(t.begin() + (long)b, {kPunct, ")"});
+    t.insert(t.begin() + (long)k + 1, {kPunct, "("});
+  }
+  // a block's last expression may omit its ';'
+  for (size_t k = 1; k < t.size(); ++k) {
+    if (t[k].s != "}") continue;
+    const std::string& p = t[k - 1].s;
+    if (p != ";" && p != "{" && p != "}") {
+      t.insert(t.begin() + (long)k, {kPunct, ";"});
+      ++k;
+    }
+  }
+  return t;
+}
+
+// Field-access and name mapping of one body inside a generated pass.
+struct PassNames {
+  std::set<std::string> split;  // fields read at [j] and written in this pass
+};
+
+std::string emit_body(const Eq& e, const PassNames& pn, std::set<std::string>* consts_used) {
+  auto bad = [&](const std::string& what) {
+    return GenError{PRESTIGE_E_UNSUPPORTED, "equation '" + e.name + "': " + what};
+  };
+  std::vector<Tok> t = rewrite(tokenize(e.body), e.name);
+  std::set<std::string> locals;
+  for (size_t k = 0; k + 1 < t.size(); ++k)
+    if (t[k].s == "double" && t[k + 1].kind == kIdent) locals.insert(t[k + 1].s);
+  std::string out;
+  for (size_t k = 0; k < t.size(); ++k) {
+    const Tok& tk = t[k];
+    std::string piece = tk.s;
+    if (tk.kind == kIdent) {
+      const size_t c = index_close(t, k);
+      if (c) {
+        if (c != k + 3 || (t[k + 2].s != "i" && t[k + 2].s != "j"))
+          throw bad("field '" + tk.s + "' indexed by something other than i or j");
+        const bool at_i = t[k + 2].s == "i";
+        const bool write = c + 1 < t.size() && is_assign_op(t[c + 1].s);
+        if (write && !at_i) throw bad("writes '" + tk.s + "[j]' (gather passes write through [i] only)");
+        if (is_pos(tk.s)) {
+          if (write) throw bad("writes the position '" + tk.s + "' that drives the neighbour search");
+          piece = std::string("(double)p") + (at_i ? "i." : "j.") + tk.s;
+        } else if (at_i) {
+          piece = tk.s + "_i";
+        } else {
+          piece = "f_" + tk.s + (pn.split.count(tk.s) ? "_in" : "") + "[jd]";
+        }
+        out += piece + " ";
+        k = c;
+        continue;
+      }
+      if (tk.s == "i") piece = "me";
+      else if (tk.s == "j") piece = "jd";
+      else if (const_names().count(tk.s)) {
+        if (consts_used) consts_used->insert(tk.s);
+      } else if (tk.s == "double" || tk.s == "if" || tk.s == "else" || tk.s == "true" || tk.s == "false" ||
+                 locals.count(tk.s) || helper_names().count(tk.s)) {
+      } else {
+        bool fn = false;
+        for (const auto& kv : fn_map())
+          if (kv.second == tk.s) fn = true;
+        if (!fn) throw bad("unknown name '" + tk.s + "'");
+      }
+    }
+    out += piece + " ";
+  }
+  return out;
+}
+
+bool has_j(const Eq& e, const std::vector<Tok>& t) {
+  for (const auto& r : e.reads)
+    if (r.second & PRESTIGE_ROLE_J) return true;
+  for (const auto& tk : t)
+    if (tk.kind == kIdent && tk.s == "j") return true;
+  return false;
+}
+
+// Pass index of every equation of the fused list (the schedule rule above).
+std::vector<uint32_t> schedule(const int32_t* ids, uint32_t n) {
+  std::vector<uint32_t> pass(n);
+  uint32_t p = 0;
+  std::set<std::string> written, written_ionly;
+  for (uint32_t k = 0; k < n; ++k) {
+    const Eq& e = registry()[(size_t)ids[k]];
+    const bool jb = has_j(e, tokenize(e.body));
+    bool hazard = false;
+    for (const auto& rd : e.reads) {
+      if ((rd.second & PRESTIGE_ROLE_J) && written.count(rd.first)) hazard = true;
+      if (jb && written_ionly.count(rd.first)) hazard = true;
+    }
+    if (hazard) {
+      ++p;
+      written.clear();
+      written_ionly.clear();
+    }
+    for (const auto& w : e.writes) {
+      written.insert(w);
+      if (!jb) written_ionly.insert(w);
+    }
+    pass[k] = p;
+  }
+  return pass;
+}
+
+const char* kPreamble = R"(// Device helpers available to equation bodies.
+// kernel_w: cubic spline W(r, h) = sigma f(r / h), sigma = 1 / (pi h^3),
+// support 2h (Monaghan & Lattanzio 1985).
+__device__ inline double kernel_w(double dx, double dy, double dz, double h) {
+  const double q = sqrt(dx * dx + dy * dy + dz * dz) / h;
+  const double s = 1.0 / (3.141592653589793 * h * h * h);
+  if (q < 1.0) return s * (1.0 - 1.5 * q * q + 0.75 * q * q * q);
+  if (q < 2.0) {
+    const double t = 2.0 - q;
+    return s * 0.25 * t * t * t;
+  }
+  return 0.0;
+}
+// tait: p = max(rho0 c0^2 / 7 ((rho / rho0)^7 - 1), 0)
+__device__ inline double tait(double rho, double rho0, double c0) {
+  const double r = rho / rho0, r2 = r * r, r4 = r2 * r2;
+  const double p = rho0 * c0 * c0 / 7.0 * (r4 * r2 * r - 1.0);
+  return p > 0.0 ? p : 0.0;
+}
+struct prestige_consts {
+  double h, dt, rho0, c0, alpha, kn, kt, mu, gx, gy, gz;
+};
+)";
+
+GenProgram build_program(const int32_t* ids, uint32_t n, int32_t loop, bool f32_pos) {
+  if (!valid_ids(ids, n)) throw GenError{PRESTIGE_E_INVALID, "unknown equation id"};
+  if (loop != PRESTIGE_LOOP_ALLPAIRS && loop != PRESTIGE_LOOP_CELLS) throw GenError{PRESTIGE_E_INVALID, "unknown loop"};
+  const std::vector<uint32_t> pass = schedule(ids, n);
+  const uint32_t npass = n ? pass.back() + 1 : 0;
+  GenProgram prog;
+  std::string& s = prog.source;
+  s += "// Gather passes generated by prestige_generate_hip from the fused equations";
+  for (uint32_t k = 0; k < n; ++k) s += std::string(k ? ", " : " ") + registry()[(size_t)ids[k]].name;
+  s += ".\n// Loop form: ";
+  s += loop == PRESTIGE_LOOP_CELLS ? "cell iterator (slot i over the nine row runs of its cell, |x_i - x_j|^2 < c2)"
+                                   : "all pairs (id i, j over ids 0..n)";
+  s += ".\n// Fields are FP64 arrays in particle-id order; [i] fields live in registers for the\n"
+       "// whole j loop and are stored once at the end.\n";
+  s += "#if !defined(__HIPCC_RTC__)\n#include <hip/hip_runtime.h>\n#endif\n";
+  s += std::string("typedef ") + (f32_pos ? "float4" : "double4") + " prestige_pos;\n";
+  s += kPreamble;
+  for (uint32_t p = 0; p < npass; ++p) {
+    std::vector<const Eq*> jb, ib;
+    std::map<std::string, GenField> fields;  // ordered by name
+    std::set<std::string> at_i, read_j, written;
+    for (uint32_t k = 0; k < n; ++k) {
+      if (pass[k] != p) continue;
+      const Eq& e = registry()[(size_t)ids[k]];
+      (has_j(e, tokenize(e.body)) ? jb : ib).push_back(&e);
+      for (const auto& rd : e.reads) {
+        if (is_pos(rd.first)) continue;
+        GenField& f = fields[rd.first];
+        f.name = rd.first;
+        f.read = true;
+        if (rd.second & PRESTIGE_ROLE_I) at_i.insert(rd.first);
+        if (rd.second & PRESTIGE_ROLE_J) read_j.insert(rd.first);
+      }
+      for (const auto& w : e.writes) {
+        if (is_pos(w)) continue;
+        fields[w].name = w;
+        fields[w].written = true;
+        written.insert(w);
+        at_i.insert(w);
+      }
+    }
+    PassNames pn;
+    for (auto& kv : fields) {
+      kv.second.split = kv.second.written && read_j.count(kv.first);
+      if (kv.second.split) pn.split.insert(kv.first);
+    }
+    std::set<std::string> consts;
+    std::string jcode, icode;
+    for (const Eq* e : jb) jcode += "      { " + emit_body(*e, pn, &consts) + "}\n";
+    for (const Eq* e : ib) icode += "  { " + emit_body(*e, pn, &consts) + "}\n";
+    GenPass gp;
+    gp.kernel = "prestige_pass_" + std::to_string(p);
+    std::string params;
+    for (auto& kv : fields) {
+      const GenField& f = kv.second;
+      gp.fields.push_back(f);
+      if (f.split)
+        params += ",\n    const double* __restrict__ f_" + f.name + "_in, double* __restrict__ f_" + f.name + "_out";
+      else if (f.written)
+        params += ",\n    double* __restrict__ f_" + f.name;
+      else
+        params += ",\n    const double* __restrict__ f_" + f.name;
+    }
+    s += "\n// pass " + std::to_string(p) + ": fields";
+    for (auto& kv : fields) s += " " + kv.first + (kv.second.split ? "(in,out)" : "");
+    s += "\n";
+    if (loop == PRESTIGE_LOOP_CELLS)
+      s += "extern \"C\" __global__ void __launch_bounds__(256) " + gp.kernel +
+           "(\n    unsigned n, const prestige_pos* __restrict__ pos, const unsigned* __restrict__ ids,\n"
+           "    const unsigned* __restrict__ skeys, const unsigned* __restrict__ lb, int nx, int ny, int nz,\n"
+           "    double c2, prestige_consts k" + params + ") {\n";
+    else
+      s += "extern \"C\" __global__ void __launch_bounds__(256) " + gp.kernel +
+           "(\n    unsigned n, const prestige_pos* __restrict__ pos, prestige_consts k" + params + ") {\n";
+    s += "#pragma clang fp contract(off)\n";
+    if (loop == PRESTIGE_LOOP_CELLS) {
+      s += "  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;\n  if (i >= n) return;\n";
+      s += "  const unsigned me = ids[i];\n  const prestige_pos pi = pos[i];\n";
+    } else {
+      s += "  const unsigned me = blockIdx.x * blockDim.x + threadIdx.x;\n  if (me >= n) return;\n";
+      s += "  const prestige_pos pi = pos[me];\n";
+    }
+    for (const auto& c : consts) s += "  const double " + c + " = k." + c + ";\n";
+    for (const auto& f : at_i) {
+      const bool sp = pn.split.count(f) > 0;
+      s += "  double " + f + "_i = f_" + f + (sp ? "_in" : "") + "[me];\n";
+    }
+    if (!jb.empty()) {
+      if (loop == PRESTIGE_LOOP_CELLS) {
+        s += "  const unsigned key = skeys[i];\n"
+             "  const int cx = (int)(key % (unsigned)nx), cy = (int)((key / (unsigned)nx) % (unsigned)ny),\n"
+             "            cz = (int)(key / ((unsigned)nx * (unsigned)ny));\n"
+             "  const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < nx - 1 ? cx + 1 : nx - 1;\n"
+             "  for (int dz = -1; dz <= 1; ++dz) {\n"
+             "    const int zz = cz + dz;\n    if (zz < 0 || zz >= nz) continue;\n"
+             "    for (int dy = -1; dy <= 1; ++dy) {\n"
+             "      const int yy = cy + dy;\n      if (yy < 0 || yy >= ny) continue;\n"
+             "      const unsigned row = ((unsigned)zz * (unsigned)ny + (unsigned)yy) * (unsigned)nx;\n"
+             "      const unsigned j1 = lb[row + (unsigned)x1 + 1];\n"
+             "      for (unsigned js = lb[row + (unsigned)x0]; js < j1; ++js) {\n"
+             "      const prestige_pos pj = pos[js];\n"
+             "      const double ex = (double)pi.x - (double)pj.x, ey = (double)pi.y - (double)pj.y,\n"
+             "                   ez = (double)pi.z - (double)pj.z;\n"
+             "      if (!(ex * ex + ey * ey + ez * ez < c2)) continue;\n"
+             "      const unsigned jd = ids[js];\n";
+        s += jcode;
+        s += "      }\n    }\n  }\n";
+      } else {
+        s += "  for (unsigned jd = 0; jd < n; ++jd) {\n      const prestige_pos pj = pos[jd];\n";
+        s += jcode;
+        s += "  }\n";
+      }
+    }
+    s += icode;
+    for (const auto& f : written) {
+      const bool sp = pn.split.count(f) > 0;
+      s += "  f_" + f + (sp ? "_out" : "") + "[me] = " + f + "_i;\n";
+    }
+    s += "}\n";
+    prog.passes.push_back(gp);
+  }
+  return prog;
+}
+
 }  // namespace
+
+GenProgram generate_hip_program(const int32_t* eq_ids, uint32_t n, int32_t loop, bool f32_pos) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  return build_program(eq_ids, n, loop, f32_pos);
+}
+
+}  // namespace prestige
+
+using prestige::GenError;
+using prestige::registry;
 
 extern "C" {
 
@@ -127,9 +685,10 @@ prestige_status prestige_equation_describe(int32_t eq_id, char* name, size_t nam
                                            size_t reads_cap, char* writes, size_t writes_cap,
                                            uint8_t* read_roles, size_t roles_cap, char* body,
                                            size_t body_cap) {
-  if (eq_id < 0 || eq_id >= (int32_t)table().size()) return PRESTIGE_E_INVALID;
-  const Eq& e = table()[(size_t)eq_id];
-  copy_out(e.name, name, name_cap);
+  std::lock_guard<std::mutex> lk(prestige::g_mutex);
+  if (eq_id < 0 || eq_id >= (int32_t)registry().size()) return PRESTIGE_E_INVALID;
+  const prestige::Eq& e = registry()[(size_t)eq_id];
+  prestige::copy_out(e.name, name, name_cap);
   std::string r, w;
   for (size_t k = 0; k < e.reads.size(); ++k) {
     if (k) r += ",";
@@ -140,50 +699,88 @@ prestige_status prestige_equation_describe(int32_t eq_id, char* name, size_t nam
     if (k) w += ",";
     w += e.writes[k];
   }
-  copy_out(r, reads, reads_cap);
-  copy_out(w, writes, writes_cap);
-  copy_out(e.body, body, body_cap);
+  prestige::copy_out(r, reads, reads_cap);
+  prestige::copy_out(w, writes, writes_cap);
+  prestige::copy_out(e.body, body, body_cap);
   return PRESTIGE_OK;
+}
+
+prestige_status prestige_equation_define(const char* name, const char* body, int32_t* eq_id) {
+  if (!name || !body || !eq_id || !prestige::is_ident(name)) {
+    prestige::set_context_free_error("equation name must be an identifier; body and eq_id non-null");
+    return PRESTIGE_E_INVALID;
+  }
+  try {
+    prestige::Eq e;
+    e.name = name;
+    e.body = body;
+    const auto t = prestige::tokenize(e.body);
+    if (t.empty() || t.front().s != "{" || prestige::match_close(t, 0) != t.size() - 1) {
+      prestige::set_context_free_error("an equation body is one block: { ... }");
+      return PRESTIGE_E_INVALID;
+    }
+    prestige::infer(t, e);
+    std::lock_guard<std::mutex> lk(prestige::g_mutex);
+    registry().push_back(e);
+    *eq_id = (int32_t)registry().size() - 1;
+    return PRESTIGE_OK;
+  } catch (const GenError& g) {
+    prestige::set_context_free_error(g.msg);
+    return g.code;
+  }
 }
 
 prestige_status prestige_fuse_schedule(const int32_t* eq_ids, uint32_t n, uint32_t* pass_of, uint32_t* n_passes) {
-  if (!valid_ids(eq_ids, n) || (n && !pass_of) || !n_passes) return PRESTIGE_E_INVALID;
-  uint32_t pass = 0;
-  std::set<std::string> written;   // fields written so far in the current pass
-  bool any = false;
-  for (uint32_t k = 0; k < n; ++k) {
-    const Eq& e = table()[(size_t)eq_ids[k]];
-    bool hazard = false;
-    for (const auto& rd : e.reads)
-      if ((rd.second & PRESTIGE_ROLE_J) && written.count(rd.first)) hazard = true;
-    if (hazard) {
-      ++pass;
-      written.clear();
-    }
-    for (const auto& wr : e.writes) written.insert(wr);
-    pass_of[k] = pass;
-    any = true;
+  std::lock_guard<std::mutex> lk(prestige::g_mutex);
+  if (!prestige::valid_ids(eq_ids, n) || (n && !pass_of) || !n_passes) {
+    prestige::set_context_free_error("unknown equation id or null output");
+    return PRESTIGE_E_INVALID;
   }
-  *n_passes = any ? pass + 1 : 0;
-  return PRESTIGE_OK;
+  try {
+    const std::vector<uint32_t> p = prestige::schedule(eq_ids, n);
+    for (uint32_t k = 0; k < n; ++k) pass_of[k] = p[k];
+    *n_passes = n ? p.back() + 1 : 0;
+    return PRESTIGE_OK;
+  } catch (const GenError& g) {
+    prestige::set_context_free_error(g.msg);
+    return g.code;
+  }
 }
 
 prestige_status prestige_generate_simple_cpu(const int32_t* eq_ids, uint32_t n, char* out, size_t cap, size_t* len) {
-  if (!valid_ids(eq_ids, n)) return PRESTIGE_E_INVALID;
+  std::lock_guard<std::mutex> lk(prestige::g_mutex);
+  if (!prestige::valid_ids(eq_ids, n)) {
+    prestige::set_context_free_error("unknown equation id");
+    return PRESTIGE_E_INVALID;
+  }
   // text layout of prestige/src/codegen/simple_cpu.rs:7-19
   std::string code;
   code += "for i in 0..n {\n";
   code += "    for j in 0..n {\n";
   for (uint32_t k = 0; k < n; ++k) {
     code += "        ";
-    code += table()[(size_t)eq_ids[k]].body;
+    code += registry()[(size_t)eq_ids[k]].body;
     code += "\n";
   }
   code += "    }\n";
   code += "}\n";
   if (len) *len = code.size();
-  copy_out(code, out, cap);
+  prestige::copy_out(code, out, cap);
   return PRESTIGE_OK;
+}
+
+prestige_status prestige_generate_hip(const int32_t* eq_ids, uint32_t n, int32_t loop, char* out, size_t cap,
+                                      size_t* len) {
+  std::lock_guard<std::mutex> lk(prestige::g_mutex);
+  try {
+    const prestige::GenProgram p = prestige::build_program(eq_ids, n, loop, false);
+    if (len) *len = p.source.size();
+    prestige::copy_out(p.source, out, cap);
+    return PRESTIGE_OK;
+  } catch (const GenError& g) {
+    prestige::set_context_free_error(g.msg);
+    return g.code;
+  }
 }
 
 }  // extern "C"
