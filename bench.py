@@ -218,6 +218,31 @@ def adjoint_dem(pa, steps=500, every=50):
             "dL_dv0_norm": float(np.linalg.norm(gv0))}
 
 
+def generated_density(pa, scene_args, reps=20):
+    """The built-in density and EOS equation bodies run through the generated path
+    (prestige_run_equations: hiprtc-compiled gather pass over the cell iterator) on the
+    1M dam break, for comparison with the hand-written density kernel."""
+    d, a = pa.scene(pa.SCENE_DAM_BREAK, **scene_args)
+    d.model = pa.MODEL_PAIRS
+    with pa.Context(d) as ctx:
+        for k in ("x", "y", "z"):
+            ctx.upload(k, a[k])
+        ctx.upload("mass", np.full(d.n_particles, d.mass))
+        ctx.neighbors_build()
+        eqs = [pa.EQ_SPH_DENSITY, pa.EQ_SPH_EOS]
+        t0 = time.perf_counter()
+        ctx.run_equations(eqs, pa.LOOP_CELLS, d.cell_size)   # first call compiles
+        t_first = time.perf_counter() - t0
+        ctx.timing_enable(True)
+        for _ in range(reps):
+            ctx.run_equations(eqs, pa.LOOP_CELLS, d.cell_size)
+        kt = [k for k in ctx.timing() if k["kernel"] == "generated_pass"][0]
+    return {"workload": "density + EOS bodies of the equation table, generated, cell iterator, "
+                        f"{d.n_particles} particles",
+            "avg_us": round(1e3 * kt["total_ms"] / kt["launches"], 1), "launches": kt["launches"],
+            "first_call_s": round(t_first, 3)}
+
+
 def copy_probe(nbytes):
     """hipMemcpyDtoD of nbytes (the runtime's copy kernel): reads and writes exactly nbytes of HBM."""
     import ctypes as C
@@ -299,6 +324,8 @@ def main():
         forms["adjoint_wcsph_1M"] = adjoint_wcsph(pa)
         log("[bench] adjoint: DEM pour 100k mixed, 500 steps, K = 50")
         forms["adjoint_dem_100k"] = adjoint_dem(pa)
+        log("[bench] generated density + EOS pass (1M)")
+        forms["generated_density_1M"] = generated_density(pa, SCENE_1M)
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")

@@ -119,3 +119,26 @@ def test_missing_input_field_is_reported():
         with pytest.raises(pa.PrestigeError) as ei:
             ctx.run_equations([e], pa.LOOP_CELLS, d.cell_size)
         assert ei.value.name == "E_STATE" and "'w'" in str(ei.value)
+
+
+def test_generated_density_and_eos_match_the_wcsph_kernel():
+    """The built-in density and EOS bodies, generated and run over the cell
+    iterator, give the hand-written WCSPH density pass's rho and p (the same
+    sums in another association: rho to 1e-13, p to 1e-10 relative, p being
+    B ((rho/rho0)^7 - 1) with ~1e-3 of cancellation)."""
+    d, arrs = make_scene(pa.SCENE_DAM_BREAK, n_target=8000, dx=0.005)
+    with pa.Context(d) as ctx:
+        upload_all(ctx, arrs)
+        ctx.step(1)          # density and EOS of the uploaded positions, then the move
+        rho_ref, p_ref = ctx.download("rho"), ctx.download("p")
+    dp = d.copy()
+    dp.model = pa.MODEL_PAIRS
+    with pa.Context(dp) as ctx:
+        upload_all(ctx, arrs)
+        ctx.upload("mass", np.full(d.n_particles, d.mass))
+        ctx.upload("rho", np.zeros(d.n_particles))
+        ctx.run_equations([pa.EQ_SPH_DENSITY, pa.EQ_SPH_EOS], pa.LOOP_CELLS, d.cell_size)
+        rho, p = ctx.download("rho"), ctx.download("p")
+    assert np.allclose(rho, rho_ref, rtol=1e-13, atol=0.0)
+    scale = np.max(np.abs(p_ref))
+    assert np.max(np.abs(p - p_ref)) <= 1e-10 * scale
