@@ -193,7 +193,7 @@ static void allocate(Ctx& c) {
     c.keys[b].alloc(nn * 4);
     c.vals[b].alloc(nn * 4);
   }
-  const uint64_t tiles = (n + 4095) / 4096 + 1;
+  const uint64_t tiles = sort_tiles(n) + 1;
   c.sort_hist.alloc(256 * tiles * 4);
   c.sort_tot.alloc(256 * 4);
   c.lb.alloc(((size_t)c.grid.n_cells + 1) * 4);

@@ -265,6 +265,7 @@ __device__ inline int cell_shift(uint32_t wi, uint32_t wj, int axis) {
 
 // Per-form kernel entry points (defined in the .hip files).
 void neighbors_build(Ctx& c);
+uint64_t sort_tiles(uint64_t n);   // radix-sort tiles of n keys (histogram rows per digit)
 void keys_from_positions(Ctx& c);
 void eq1_eval(Ctx& c, int loop, double cutoff);
 void sph_step_f64(Ctx& c);

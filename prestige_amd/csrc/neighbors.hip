@@ -60,7 +60,10 @@ void keys_from_positions(Ctx& c) {
 constexpr int kRadixBits = 8;
 constexpr int kBuckets = 1 << kRadixBits;
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
+#ifndef PRESTIGE_SORT_ITEMS
+#define PRESTIGE_SORT_ITEMS 8
+#endif
+constexpr int kSortItems = PRESTIGE_SORT_ITEMS;   // keys per thread and tile
 constexpr int kTile = kSortThreads * kSortItems;
 static_assert(kSortThreads == kBuckets, "one thread per bucket in the scan phases");
 
@@ -199,6 +202,8 @@ __global__ void k_iota(uint32_t n, uint32_t* v) {
   if (i < n) v[i] = i;
 }
 
+uint64_t sort_tiles(uint64_t n) { return (n + (uint64_t)kTile - 1) / (uint64_t)kTile; }
+
 static int key_bits(uint32_t n_cells) {
   int b = 0;
   while (b < 32 && ((uint64_t)1 << b) < (uint64_t)n_cells) ++b;
@@ -211,7 +216,7 @@ static void radix_sort(Ctx& c) {
   const uint32_t n = (uint32_t)c.n;
   const int bits = key_bits(c.grid.n_cells);
   const int passes = (bits + kRadixBits - 1) / kRadixBits;
-  const uint32_t n_tiles = (n + kTile - 1) / kTile;
+  const uint32_t n_tiles = (uint32_t)sort_tiles(n);
   if (passes == 0) {
     k_iota<<<blocks_for(n), kBlock, 0, c.stream>>>(n, c.vals[0].as<uint32_t>());
     PR_HIP(hipGetLastError());
