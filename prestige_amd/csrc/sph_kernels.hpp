@@ -7,25 +7,22 @@
 // atomics.
 //
 // Two passes per step, split where the fused bodies would read rho[j] and
-// p[j] that the density body writes:
-//   density  (a) scan the candidates of the nine row runs of the particle's
-//            cell in FP32 (offsets from a point of the wavefront, staged in
-//            LDS once per wavefront and run), marking every candidate inside
-//            the cutoff plus a small relative margin in 32-bit words; each
-//            non-empty word is appended to the particle's neighbour stream
-//            as (word, slot of bit 0); (b) walk the run's set bits in the
-//            form's own precision, keep the exact r < 2h pairs and sum the
+// p[j] that the density body writes; both run one wavefront per block and
+// visit the nine row runs of a particle's cell in lockstep:
+//   density  (a) scan the run's candidates in FP32 (offsets from a point of
+//            the wavefront, staged in LDS once per wavefront and run),
+//            marking every candidate inside the cutoff plus a small relative
+//            margin in 32-bit words, which are stored with the run's
+//            (first slot, count) for the force pass; (b) walk the set bits in
+//            the form's own precision, keep the exact r < 2h pairs and sum the
 //            density.  Then the EOS.
-//   force    walks the same stream: momentum + viscosity, symplectic Euler
-//            and the next step's cell key.
+//   force    stage the gather data of the wavefront's hits of the run in LDS
+//            and walk the stored masks: momentum + viscosity, then
+//            symplectic Euler and the next step's cell key.
 // Scanning in FP32 halves the bytes each candidate costs, and the expensive
-// pair bodies only run over the set bits.  The stream is one flat sequence
-// per particle, so a lane moves from one run to the next without waiting for
-// the other lanes of its wavefront to finish their run: the wavefront's walk
-// costs the largest total neighbour count of its lanes, not the sum over the
-// runs of each run's largest count.  Runs in order, words in order and bits
-// in ascending order are the scan order, so FP64 sums are formed in exactly
-// the order of the host oracle; the streams are also the frozen neighbour
+// pair bodies only run over the set bits.  Runs in order and bits in
+// ascending order are the scan order, so FP64 sums are formed in exactly the
+// order of the host oracle; the stored masks are also the frozen neighbour
 // sets the adjoint reuses.
 //
 // Forms:
@@ -166,9 +163,9 @@ __device__ inline real spline_df(real q) {
 
 // Loads kept in flight per lane.  The gathers are latency-bound (dependent
 // loads, little arithmetic per byte), so each lane issues a batch of
-// independent loads before it uses any of them: kScanUnr probe candidates in
-// the density scan, kDensUnr neighbours in the density walk, kUnr neighbours
-// in the stream walks of the force pass and the adjoint.
+// independent loads before it uses any of them: kScanUnr candidates in the
+// density scan, kDensUnr neighbours in the density walk, kUnr neighbours in
+// the walks of the force pass and the adjoint.
 #ifndef PRESTIGE_SCAN_UNROLL
 #define PRESTIGE_SCAN_UNROLL 2
 #endif
