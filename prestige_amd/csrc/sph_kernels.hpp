@@ -182,12 +182,6 @@ static_assert(kScanUnr == 1 || kScanUnr == 2 || kScanUnr == 4, "scan batch divid
 static_assert(kDensUnr >= 1 && kUnr >= 1, "walk batches hold at least one neighbour");
 constexpr uint32_t kPad = 0xFFFFFFFFu;
 
-// Threads per block of the two forward gathers.
-#ifndef PRESTIGE_GATHER_BLOCK
-#define PRESTIGE_GATHER_BLOCK 256
-#endif
-constexpr int kGather = PRESTIGE_GATHER_BLOCK;
-
 // Candidate coordinates for the scan: offsets from a reference point of the
 // wavefront (the position of its first slot) in units of h, rounded once to
 // FP32.  Every neighbour lies within a cell of its particle, so the scan
@@ -762,7 +756,6 @@ void sph_step_impl(Ctx& c) {
   const G4* gpos = (F == kMixed) ? c.off32.as<G4>() : c.pos[c.cur].as<G4>();
   const G4* gvel = (F == kMixed) ? c.vel32.as<G4>() : c.vel[c.cur].as<G4>();
   const uint32_t* skeys = c.keys[c.cur_key].as<uint32_t>();
-  const uint32_t grid = xcd_grid(blocks_for(n, kGather));
   // .w lanes of the FP64 / FP32 gather copies receive (rho, p/rho^2)
   G4* wpos = (F == kMixed) ? nullptr : c.pos[c.cur].as<G4>();
   G4* wvel = (F == kMixed) ? nullptr : c.vel[c.cur].as<G4>();
