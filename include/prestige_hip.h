@@ -68,6 +68,12 @@ typedef enum prestige_loop {
   PRESTIGE_LOOP_CELLS = 1     /* j over the 27 cells around i's cell, |x_i-x_j| < cell_size */
 } prestige_loop;
 
+/* Particle output formats (prestige_write_particles). */
+typedef enum prestige_output_format {
+  PRESTIGE_OUT_VTK = 0,   /* legacy VTK POLYDATA, binary (big-endian FP64) */
+  PRESTIGE_OUT_CSV = 1    /* text, one row per particle, 17 significant digits */
+} prestige_output_format;
+
 /* Built-in equations (prestige/src/lib.rs:7-12 holds the reference's one, eq1). */
 typedef enum prestige_equation_id {
   PRESTIGE_EQ1 = 0,           /* force[i] += mass[j]  (prestige/src/lib.rs:10) */
@@ -259,6 +265,24 @@ prestige_status prestige_generate_hip(const int32_t* eq_ids, uint32_t n, int32_t
  * i-only bodies run once per i after the j loop. */
 prestige_status prestige_run_equations(prestige_ctx* ctx, const int32_t* eq_ids, uint32_t n,
                                        int32_t loop, double cutoff);
+
+/* ---- output (the VTK / CSV particle and time-series files the reference's
+ *      .gitignore:4,6 anticipates; no writer exists there) ----------------- */
+
+/* Write the particles to path in particle-id order: the positions (x, y, z)
+ * and the comma-separated fields (any name prestige_field_download accepts;
+ * NULL or "" for none), with the simulation time.  VTK: points, one vertex
+ * cell per particle, the fields as POINT_DATA scalars and the time as field
+ * data TIME.  CSV: a "# time" line, a header "id,x,y,z,<fields>" and one row
+ * per particle. */
+prestige_status prestige_write_particles(prestige_ctx* ctx, const char* path, int32_t format,
+                                         const char* fields, double time);
+
+/* Write a ParaView collection file (.pvd) that lists files[k] (paths as
+ * written into the file) at times[k], k < n: the time series of particle
+ * outputs.  Host only. */
+prestige_status prestige_write_series(const char* path, const char* const* files, const double* times,
+                                      uint32_t n);
 
 /* ---- scenes (host only; no device needed) -------------------------------
  * Deterministic particle layouts of the benchmark configurations, drawn with

@@ -22,7 +22,8 @@ import numpy as np
 
 __all__ = [
     "Desc", "Context", "PrestigeError", "load_library", "scene", "equation_describe",
-    "fuse_schedule", "generate_simple_cpu", "define_equation", "generate_hip", "MODEL_PAIRS", "MODEL_WCSPH", "MODEL_DEM",
+    "fuse_schedule", "generate_simple_cpu", "define_equation", "generate_hip", "write_series", "OUT_VTK",
+    "OUT_CSV", "MODEL_PAIRS", "MODEL_WCSPH", "MODEL_DEM",
     "F64", "F32", "MIXED", "LOOP_ALLPAIRS", "LOOP_CELLS", "EQ1", "EQ_SPH_DENSITY", "EQ_SPH_EOS",
     "EQ_SPH_MOMENTUM", "EQ_INTEGRATE", "EQ_DEM_CONTACT", "SCENE_DEM_LATTICE", "SCENE_DAM_BREAK",
     "SCENE_DEM_POUR", "ROLE_I", "ROLE_J",
@@ -33,6 +34,7 @@ LIB_PATH = PKG / "libprestige_hip.so"
 
 ABI_VERSION = 1
 MODEL_PAIRS, MODEL_WCSPH, MODEL_DEM = 0, 1, 2
+OUT_VTK, OUT_CSV = 0, 1
 F64, F32, MIXED = 0, 1, 2
 LOOP_ALLPAIRS, LOOP_CELLS = 0, 1
 EQ1, EQ_SPH_DENSITY, EQ_SPH_EOS, EQ_SPH_MOMENTUM, EQ_INTEGRATE, EQ_DEM_CONTACT = range(6)
@@ -131,6 +133,8 @@ def _bind(lib):
         "prestige_generate_hip": (S, [C.POINTER(C.c_int32), C.c_uint32, C.c_int32, C.c_char_p, C.c_size_t,
                                       C.POINTER(C.c_size_t)]),
         "prestige_run_equations": (S, [_P, C.POINTER(C.c_int32), C.c_uint32, C.c_int32, C.c_double]),
+        "prestige_write_particles": (S, [_P, C.c_char_p, C.c_int32, C.c_char_p, C.c_double]),
+        "prestige_write_series": (S, [C.c_char_p, C.POINTER(C.c_char_p), _D, C.c_uint32]),
         "prestige_scene_count": (S, [C.c_int32, C.POINTER(SceneParams), C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_uint64)]),
         "prestige_scene_build": (S, [C.c_int32, C.POINTER(SceneParams), C.POINTER(Desc), _D, _D, _D, _D, _D,
@@ -248,6 +252,11 @@ class Context:
     def eval_equation(self, eq_id: int, loop: int, cutoff: float = 0.0) -> None:
         self._c(self._lib.prestige_eval_equation(self._h, eq_id, loop, float(cutoff)))
 
+    def write_particles(self, path, fmt: int = 0, fields=(), time: float = 0.0) -> None:
+        """Write positions and fields (id order) as legacy binary VTK (fmt OUT_VTK) or CSV (OUT_CSV)."""
+        self._c(self._lib.prestige_write_particles(self._h, str(path).encode(), fmt, ",".join(fields).encode(),
+                                                   float(time)))
+
     def run_equations(self, eq_ids, loop: int, cutoff: float = 0.0) -> None:
         """Compile (once per context) and run the generated gather passes of the fused equations."""
         ids = (C.c_int32 * max(1, len(eq_ids)))(*eq_ids)
@@ -355,6 +364,14 @@ def generate_hip(eq_ids, loop: int) -> str:
     buf = C.create_string_buffer(n.value + 1)
     _check(lib.prestige_generate_hip(ids, len(eq_ids), loop, buf, n.value + 1, C.byref(n)))
     return buf.value.decode()
+
+
+def write_series(path, files, times) -> None:
+    """ParaView collection (.pvd) listing files[k] at times[k]."""
+    lib = load_library()
+    fs = (C.c_char_p * max(1, len(files)))(*[str(f).encode() for f in files])
+    ts = np.ascontiguousarray(times, dtype=np.float64)
+    _check(lib.prestige_write_series(str(path).encode(), fs, _dptr(ts) if len(ts) else None, len(files)))
 
 
 def generate_simple_cpu(eq_ids) -> str:

@@ -520,6 +520,115 @@ prestige_status prestige_field_download(prestige_ctx* ctx, const char* name, dou
   });
 }
 
+prestige_status prestige_write_particles(prestige_ctx* ctx, const char* path, int32_t format,
+                                         const char* fields, double time) {
+  return guard(ctx, [&] {
+    PR_CHECK(path != nullptr, PRESTIGE_E_INVALID, "null path");
+    PR_CHECK(format == PRESTIGE_OUT_VTK || format == PRESTIGE_OUT_CSV, PRESTIGE_E_INVALID, "unknown format");
+    std::vector<std::string> names;
+    if (fields) {
+      std::string cur;
+      for (const char* q = fields;; ++q) {
+        if (*q == ',' || *q == '\0') {
+          if (!cur.empty()) names.push_back(cur);
+          cur.clear();
+          if (*q == '\0') break;
+        } else if (*q != ' ') {
+          cur += *q;
+        }
+      }
+    }
+    const uint64_t n = ctx->n;
+    std::vector<double> xyz[3];
+    for (int a = 0; a < 3; ++a) {
+      xyz[a].resize(n);
+      download(*ctx, a == 0 ? "x" : (a == 1 ? "y" : "z"), xyz[a].data(), n);
+    }
+    std::vector<std::vector<double>> data(names.size());
+    for (size_t k = 0; k < names.size(); ++k) {
+      data[k].resize(n);
+      download(*ctx, names[k], data[k].data(), n);
+    }
+    FILE* f = std::fopen(path, format == PRESTIGE_OUT_VTK ? "wb" : "w");
+    PR_CHECK(f != nullptr, PRESTIGE_E_INVALID, std::string("cannot open '") + path + "' for writing");
+    bool ok = true;
+    if (format == PRESTIGE_OUT_CSV) {
+      ok &= std::fprintf(f, "# time %.17g\nid,x,y,z", time) > 0;
+      for (const auto& nm : names) ok &= std::fprintf(f, ",%s", nm.c_str()) > 0;
+      ok &= std::fputc('\n', f) != EOF;
+      for (uint64_t i = 0; i < n && ok; ++i) {
+        ok &= std::fprintf(f, "%llu,%.17g,%.17g,%.17g", (unsigned long long)i, xyz[0][i], xyz[1][i], xyz[2][i]) > 0;
+        for (const auto& col : data) ok &= std::fprintf(f, ",%.17g", col[i]) > 0;
+        ok &= std::fputc('\n', f) != EOF;
+      }
+    } else {
+      // legacy VTK binary data are big-endian
+      auto be64 = [](double v) {
+        uint64_t u;
+        std::memcpy(&u, &v, 8);
+        return __builtin_bswap64(u);
+      };
+      auto put = [&](const void* p, size_t b) { ok &= std::fwrite(p, 1, b, f) == b; };
+      std::vector<uint64_t> buf;
+      ok &= std::fprintf(f, "# vtk DataFile Version 3.0\nprestige particles, time %.17g\nBINARY\n"
+                            "DATASET POLYDATA\nFIELD FieldData 1\nTIME 1 1 double\n", time) > 0;
+      const uint64_t tb = be64(time);
+      put(&tb, 8);
+      ok &= std::fprintf(f, "\nPOINTS %llu double\n", (unsigned long long)n) > 0;
+      buf.resize(3 * n);
+      for (uint64_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) buf[3 * i + a] = be64(xyz[a][i]);
+      put(buf.data(), buf.size() * 8);
+      ok &= std::fprintf(f, "\nVERTICES %llu %llu\n", (unsigned long long)n, (unsigned long long)(2 * n)) > 0;
+      std::vector<uint32_t> cells(2 * n);
+      for (uint64_t i = 0; i < n; ++i) {
+        cells[2 * i] = __builtin_bswap32(1u);
+        cells[2 * i + 1] = __builtin_bswap32((uint32_t)i);
+      }
+      put(cells.data(), cells.size() * 4);
+      ok &= std::fprintf(f, "\nPOINT_DATA %llu\n", (unsigned long long)n) > 0;
+      buf.resize(n);
+      for (size_t k = 0; k < names.size(); ++k) {
+        ok &= std::fprintf(f, "SCALARS %s double 1\nLOOKUP_TABLE default\n", names[k].c_str()) > 0;
+        for (uint64_t i = 0; i < n; ++i) buf[i] = be64(data[k][i]);
+        put(buf.data(), n * 8);
+        ok &= std::fputc('\n', f) != EOF;
+      }
+    }
+    ok &= std::fclose(f) == 0;
+    PR_CHECK(ok, PRESTIGE_E_INVALID, std::string("write to '") + path + "' failed");
+  });
+}
+
+prestige_status prestige_write_series(const char* path, const char* const* files, const double* times,
+                                      uint32_t n) {
+  if (!path || (n && (!files || !times))) {
+    g_create_error = "null path or list";
+    return PRESTIGE_E_INVALID;
+  }
+  FILE* f = std::fopen(path, "w");
+  if (!f) {
+    g_create_error = std::string("cannot open '") + path + "' for writing";
+    return PRESTIGE_E_INVALID;
+  }
+  bool ok = std::fprintf(f, "<?xml version=\"1.0\"?>\n<VTKFile type=\"Collection\" version=\"0.1\">\n"
+                            "  <Collection>\n") > 0;
+  for (uint32_t k = 0; k < n; ++k) {
+    if (!files[k]) {
+      ok = false;
+      break;
+    }
+    ok &= std::fprintf(f, "    <DataSet timestep=\"%.17g\" part=\"0\" file=\"%s\"/>\n", times[k], files[k]) > 0;
+  }
+  ok &= std::fprintf(f, "  </Collection>\n</VTKFile>\n") > 0;
+  ok &= std::fclose(f) == 0;
+  if (!ok) {
+    g_create_error = std::string("write to '") + path + "' failed";
+    return PRESTIGE_E_INVALID;
+  }
+  return PRESTIGE_OK;
+}
+
 prestige_status prestige_neighbors_build(prestige_ctx* ctx) {
   return guard(ctx, [&] {
     neighbors_build(*ctx);

@@ -173,3 +173,13 @@ def test_no_cpu_fallback_without_device():
     with pytest.raises(pa.PrestigeError) as ei:
         pa.Context(_small_desc())
     assert ei.value.name in ("E_HIP", "E_OOM")
+
+
+def test_write_series_collection(tmp_path):
+    """The time-series collection lists each particle file at its time."""
+    import xml.etree.ElementTree as ET
+    p = tmp_path / "run.pvd"
+    pa.write_series(p, ["out_0000.vtk", "out_0001.vtk"], [0.0, 2.5e-4])
+    root = ET.parse(p).getroot()
+    ds = root.find("Collection").findall("DataSet")
+    assert [(d.get("file"), float(d.get("timestep"))) for d in ds] == [("out_0000.vtk", 0.0), ("out_0001.vtk", 2.5e-4)]
