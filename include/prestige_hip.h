@@ -284,6 +284,23 @@ prestige_status prestige_write_particles(prestige_ctx* ctx, const char* path, in
 prestige_status prestige_write_series(const char* path, const char* const* files, const double* times,
                                       uint32_t n);
 
+/* ---- boundary geometry from Gmsh meshes (the *.msh files .gitignore:5
+ *      anticipates; no reader exists there).  Host only. ----------------- */
+
+/* Boundary particles of the surface mesh in the ASCII MSH 2.x / 4.x file
+ * path, at spacing dx: the triangles and quadrangles' coplanar patches are
+ * filled with a square dx lattice (points at least dx/2 inside), the feature
+ * lines between patches and along open rims are split into equal steps of at
+ * most dx, and their corners are kept; a box whose sides are multiples of dx
+ * gives exactly the lattice points of its surface.  layers >= 1 surfaces are
+ * written, layer l offset by l*dx against (side = -1) or along (side = +1)
+ * the face normals (b - a) x (c - a): mesh a tank with normals into the fluid
+ * and pass side = -1 for walls behind it.  With x, y and z all NULL only *n
+ * is set; otherwise the cap-sized arrays receive the *n points (layer 0
+ * first).  Append them as fixed particles (ids >= n_moving). */
+prestige_status prestige_mesh_boundary(const char* path, double dx, uint32_t layers, int32_t side,
+                                       double* x, double* y, double* z, uint64_t cap, uint64_t* n);
+
 /* ---- scenes (host only; no device needed) -------------------------------
  * Deterministic particle layouts of the benchmark configurations, drawn with
  * SplitMix64.  Arrays are caller-owned, sized from prestige_scene_count. */

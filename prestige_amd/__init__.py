@@ -22,7 +22,7 @@ import numpy as np
 
 __all__ = [
     "Desc", "Context", "PrestigeError", "load_library", "scene", "equation_describe",
-    "fuse_schedule", "generate_simple_cpu", "define_equation", "generate_hip", "write_series", "OUT_VTK",
+    "fuse_schedule", "generate_simple_cpu", "define_equation", "generate_hip", "write_series", "mesh_boundary", "OUT_VTK",
     "OUT_CSV", "MODEL_PAIRS", "MODEL_WCSPH", "MODEL_DEM",
     "F64", "F32", "MIXED", "LOOP_ALLPAIRS", "LOOP_CELLS", "EQ1", "EQ_SPH_DENSITY", "EQ_SPH_EOS",
     "EQ_SPH_MOMENTUM", "EQ_INTEGRATE", "EQ_DEM_CONTACT", "SCENE_DEM_LATTICE", "SCENE_DAM_BREAK",
@@ -135,6 +135,8 @@ def _bind(lib):
         "prestige_run_equations": (S, [_P, C.POINTER(C.c_int32), C.c_uint32, C.c_int32, C.c_double]),
         "prestige_write_particles": (S, [_P, C.c_char_p, C.c_int32, C.c_char_p, C.c_double]),
         "prestige_write_series": (S, [C.c_char_p, C.POINTER(C.c_char_p), _D, C.c_uint32]),
+        "prestige_mesh_boundary": (S, [C.c_char_p, C.c_double, C.c_uint32, C.c_int32, _D, _D, _D, C.c_uint64,
+                                       C.POINTER(C.c_uint64)]),
         "prestige_scene_count": (S, [C.c_int32, C.POINTER(SceneParams), C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_uint64)]),
         "prestige_scene_build": (S, [C.c_int32, C.POINTER(SceneParams), C.POINTER(Desc), _D, _D, _D, _D, _D,
@@ -372,6 +374,19 @@ def write_series(path, files, times) -> None:
     fs = (C.c_char_p * max(1, len(files)))(*[str(f).encode() for f in files])
     ts = np.ascontiguousarray(times, dtype=np.float64)
     _check(lib.prestige_write_series(str(path).encode(), fs, _dptr(ts) if len(ts) else None, len(files)))
+
+
+def mesh_boundary(path, dx: float, layers: int = 1, side: int = -1) -> np.ndarray:
+    """Boundary particles sampled from the surface of a Gmsh .msh file at
+    spacing dx: an (n, 3) float64 array, layer 0 first, later layers l*dx
+    against (side=-1) or along (side=+1) the face normals."""
+    lib = load_library()
+    n = C.c_uint64()
+    _check(lib.prestige_mesh_boundary(str(path).encode(), dx, layers, side, None, None, None, 0, C.byref(n)))
+    xyz = [np.zeros(max(1, n.value), np.float64) for _ in range(3)]
+    _check(lib.prestige_mesh_boundary(str(path).encode(), dx, layers, side, *[_dptr(a) for a in xyz], n.value,
+                                      C.byref(n)))
+    return np.stack([a[:n.value] for a in xyz], 1)
 
 
 def generate_simple_cpu(eq_ids) -> str:
