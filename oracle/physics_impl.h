@@ -11,7 +11,9 @@
  * inner, j == i visited, bodies in order, writes only through [i]) with the
  * j range replaced by the 27 cells around i: for dz in -1..1, dy in -1..1 the
  * three cells cx-1..cx+1 of that row are one contiguous run of the key-sorted
- * order, visited in ascending slot order.
+ * order, visited in ascending slot order.  DEM visits the runs row by row
+ * (dz outer, dy inner); the WCSPH passes visit opposite rows one after the
+ * other (sph_runs), the order in which the device walks them.
  */
 
 typedef struct STATE {
@@ -201,13 +203,42 @@ static int NAME(runs)(const STATE* s, uint64_t i, uint32_t* rlo, uint32_t* rhi) 
 }
 
 /* Summation density over all particles, then Tait EOS (gamma = 7), p >= 0. */
+#ifndef ORC_SPH_ORDER
+#define ORC_SPH_ORDER
+/* WCSPH run order: run r = 3 (dz + 1) + (dy + 1) is followed by the
+ * opposite row 8 - r, the centre row last. */
+static const int orc_sph_order[9] = {0, 8, 1, 7, 2, 6, 3, 5, 4};
+#endif
+
+/* The nine runs of slot i in the WCSPH order (rows outside the grid left out). */
+static int NAME(sph_runs)(const STATE* s, uint64_t i, uint32_t* rlo, uint32_t* rhi) {
+  const int32_t* D = s->P.dims;
+  uint32_t k = s->key[i];
+  int32_t cx = (int32_t)(k % (uint32_t)D[0]);
+  int32_t cy = (int32_t)((k / (uint32_t)D[0]) % (uint32_t)D[1]);
+  int32_t cz = (int32_t)(k / ((uint32_t)D[0] * (uint32_t)D[1]));
+  int32_t x0 = cx > 0 ? cx - 1 : 0;
+  int32_t x1 = cx < D[0] - 1 ? cx + 1 : D[0] - 1;
+  int nr = 0;
+  for (int t = 0; t < 9; ++t) {
+    const int r = orc_sph_order[t];
+    int32_t zz = cz + r / 3 - 1, yy = cy + r % 3 - 1;
+    if (zz < 0 || zz >= D[2] || yy < 0 || yy >= D[1]) continue;
+    uint32_t row = ((uint32_t)zz * (uint32_t)D[1] + (uint32_t)yy) * (uint32_t)D[0];
+    rlo[nr] = s->lb[row + (uint32_t)x0];
+    rhi[nr] = s->lb[row + (uint32_t)x1 + 1];
+    ++nr;
+  }
+  return nr;
+}
+
 static void NAME(density)(STATE* s) {
   int64_t n = (int64_t)s->n;
   const double H2 = s->H2, inv_h = s->inv_h;
 #pragma omp parallel for schedule(dynamic, 1024)
   for (int64_t i = 0; i < n; ++i) {
     uint32_t rlo[9], rhi[9];
-    int nr = NAME(runs)(s, (uint64_t)i, rlo, rhi);
+    int nr = NAME(sph_runs)(s, (uint64_t)i, rlo, rhi);
     REAL xi = s->x[i], yi = s->y[i], zi = s->z[i];
     REAL acc = 0.0;
     for (int r = 0; r < nr; ++r) {
@@ -254,7 +285,7 @@ static void NAME(sph_force_integrate)(STATE* s) {
       continue;
     }
     uint32_t rlo[9], rhi[9];
-    int nr = NAME(runs)(s, (uint64_t)i, rlo, rhi);
+    int nr = NAME(sph_runs)(s, (uint64_t)i, rlo, rhi);
     REAL xi = s->x[i], yi = s->y[i], zi = s->z[i];
     REAL vxi = s->vx[i], vyi = s->vy[i], vzi = s->vz[i];
     REAL pri = s->pr[i], rhoi = s->rho[i];

@@ -123,6 +123,50 @@ __device__ inline void for_each_run(const CellRows& cr, const Grid& g, const uin
   }
 }
 
+// Correctly rounded square root of a positive normal double (pair
+// distances 0 < r^2 < 4 h^2; x = 0 would give NaN, and coordinates that
+// differ by less than 1e-154 are outside the domain): the hardware
+// reciprocal square root refined by the coupled Newton iteration for
+// g ~ sqrt(x), hh ~ 1/(2 sqrt(x)) and two remainder corrections
+// g += (x - g^2) hh (Markstein, IA-64 and Elementary Functions, 2000, ch. 8).
+// sqrt() adds exponent scaling for tiny or huge x and special-value
+// selects, which leave such x unchanged; without them the sequence is 7
+// instructions shorter and gives the same bits.
+__device__ inline double sqrt_pos(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y;
+  double hh = y * 0.5;
+  const double r = __builtin_fma(-hh, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  hh = __builtin_fma(hh, r, hh);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, hh, g);
+  d = __builtin_fma(-g, g, x);
+  return __builtin_fma(d, hh, g);
+}
+__device__ inline float sqrt_pos(float x) { return sqrtf(x); }
+
+// Correctly rounded a / b for a positive normal b and an a whose quotient
+// stays normal (the pair terms: b = r or r^2-scaled density, a bounded):
+// reciprocal by two Newton steps from the hardware estimate, then the
+// quotient a r corrected once by its remainder a - b q (Markstein 2000,
+// ch. 8).  The division operator adds exponent pre-scaling (div_scale,
+// div_fmas) and special-value repair (div_fixup), which leave such operands
+// unchanged.  One difference: a zero a yields +0 where the operator gives
+// -0 for a = -0; the pair sums start at +0 and never become -0, so adding
+// either zero leaves them bit-identical.
+__device__ inline double div_pos(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  const double q = a * r;
+  const double rem = __builtin_fma(-b, q, a);
+  return __builtin_fma(rem, r, q);
+}
+__device__ inline float div_pos(float a, float b) { return a / b; }
+
 // Mixed-form pair vector x_i - x_j from two probes (units of h).
 __device__ inline void probe_pair(const float4& qi, const float4& qj, float cs_u, float& dx, float& dy,
                                   float& dz) {
@@ -267,8 +311,34 @@ __device__ inline void load_run(const uint2* __restrict__ meta, const uint32_t* 
   m1 = (uint64_t)wd[2] | ((uint64_t)wd[3] << 32);
 }
 
-// Visit the set bits of a 128-bit mask in ascending order, B at a time:
-// fn(jj) receives mask bit indices, kPad for unused batch entries.
+// Visit the set bits of the W-word mask m (bit b of word w is index 64 w + b)
+// in ascending order, B at a time: fn(jj) receives bit indices, kPad for
+// unused batch entries.
+template <int B, int W, class Fn>
+__device__ inline void walk_bits_w(uint64_t (&m)[W], Fn&& fn) {
+  auto any = [&] {
+    uint64_t o = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) o |= m[w];
+    return o != 0;
+  };
+  while (any()) {
+    uint32_t jj[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      jj[u] = kPad;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        if (jj[u] == kPad && m[w]) {
+          jj[u] = 64u * (uint32_t)w + (uint32_t)__builtin_ctzll(m[w]);
+          m[w] &= m[w] - 1;
+        }
+      }
+    }
+    fn(jj);
+  }
+}
+
 template <int B, class Fn>
 __device__ inline void walk_bits(uint64_t m0, uint64_t m1, Fn&& fn) {
   while (m0 | m1) {
@@ -287,6 +357,17 @@ __device__ inline void walk_bits(uint64_t m0, uint64_t m1, Fn&& fn) {
     }
     fn(jj);
   }
+}
+
+// Order of the row runs in both WCSPH passes: each run is followed by the
+// opposite row (run 8 - r), the centre row last, and a wavefront walks each
+// pair as one stream.  Lanes near one corner of their cell have many hits in
+// one row of a pair and few in the other, so the pair's per-lane totals are
+// far more even than either row's, and lockstep walks idle fewer lanes.
+constexpr uint32_t kGroups = 5;  // {0, 8}, {1, 7}, {2, 6}, {3, 5}, {4}
+__device__ inline uint32_t sph_run(uint32_t k) {
+  // k-th run of the order 0, 8, 1, 7, 2, 6, 3, 5, 4
+  return (k & 1u) ? 8u - (k >> 1) : (k >> 1);
 }
 
 // Minimum / maximum over the wavefront, returned as a wave-uniform value.
@@ -308,45 +389,81 @@ __device__ inline uint32_t wave_max(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
 
-// Per-run walk of a wavefront (one wavefront per block; every lane calls it,
-// live or not).  For each run the wavefront copies the data of the union of
-// its lanes' hit ranges into LDS with stage(k, j) (slot j -> stage entry k),
-// then each lane walks its own mask B at a time with lds(jj) (jj: stage
-// entries, kPad for unused batch entries, jj[0] always valid).  A union wider
-// than CAP slots is walked with glob(jj) (jj: slots) instead.  Lanes visit
-// their neighbours in scan order either way; end_run() follows each run.
+// Walk of a wavefront over its lanes' stored masks (one wavefront per block;
+// every lane calls it, live or not), run groups in the WCSPH order.  For each
+// group the wavefront copies the gather data of the union of its lanes' hit
+// ranges of the group's runs into LDS with stage(k, j) (slot j -> stage
+// entry k), then each lane walks its own masks B at a time with lds(jj) (jj:
+// stage entries, kPad for unused batch entries, jj[0] always valid).  A group
+// whose unions do not fit CAP entries together is walked run by run, and a
+// run whose union alone exceeds CAP with glob(jj) (jj: slots).  Lanes visit
+// their neighbours in group order and scan order either way; end_run()
+// follows each group.
 template <int B, uint32_t CAP, class Stage, class Lds, class Glob, class End>
 __device__ inline void walk_runs_staged(const uint2* __restrict__ meta, const uint32_t* __restrict__ words,
                                         uint32_t n, uint32_t i, bool live, uint32_t lane, Stage&& stage, Lds&& lds,
                                         Glob&& glob, End&& end_run) {
-  for (uint32_t r = 0; r < kRuns; ++r) {
-    uint32_t j0 = 0;
-    uint64_t m0 = 0, m1 = 0;
-    if (live) load_run(meta, words, r, n, i, j0, m0, m1);
-    uint32_t a = 0xFFFFFFFFu, b = 0u;
-    if (m0 | m1) {
-      a = j0 + (m0 ? (uint32_t)__builtin_ctzll(m0) : 64u + (uint32_t)__builtin_ctzll(m1));
-      b = j0 + (m1 ? 128u - (uint32_t)__builtin_clzll(m1) : 64u - (uint32_t)__builtin_clzll(m0));
+  for (uint32_t grp = 0; grp < kGroups; ++grp) {
+    const uint32_t nrun = grp + 1u == kGroups ? 1u : 2u;
+    uint64_t m[4] = {0u, 0u, 0u, 0u};
+    uint32_t j0[2] = {0u, 0u}, base[2] = {0u, 0u}, span[2] = {0u, 0u};
+#pragma unroll
+    for (uint32_t t = 0; t < 2u; ++t) {
+      if (t >= nrun) break;
+      if (live) load_run(meta, words, sph_run(2u * grp + t), n, i, j0[t], m[2u * t], m[2u * t + 1u]);
+      const uint64_t lo = m[2u * t], hi = m[2u * t + 1u];
+      uint32_t a = 0xFFFFFFFFu, b = 0u;
+      if (lo | hi) {
+        a = j0[t] + (lo ? (uint32_t)__builtin_ctzll(lo) : 64u + (uint32_t)__builtin_ctzll(hi));
+        b = j0[t] + (hi ? 128u - (uint32_t)__builtin_clzll(hi) : 64u - (uint32_t)__builtin_clzll(lo));
+      }
+      base[t] = wave_min(a);
+      const uint32_t top = wave_max(b);
+      span[t] = top > base[t] ? top - base[t] : 0u;
     }
-    const uint32_t base = wave_min(a), top = wave_max(b);
-    if (top <= base) continue;
-    if (top - base <= CAP) {
-      for (uint32_t k = lane; k < top - base; k += 64u) stage(k, base + k);
+    const uint32_t total = span[0] + span[1];
+    if (total == 0u) {
+      end_run();
+      continue;
+    }
+    if (total <= CAP) {
+      // both runs' unions side by side: run 0 at [0, span0), run 1 after it
+      for (uint32_t k = lane; k < total; k += 64u)
+        stage(k, k < span[0] ? base[0] + k : base[1] + (k - span[0]));
       __syncthreads();
-      walk_bits<B>(m0, m1, [&](const uint32_t* jb) {
+      walk_bits_w<B>(m, [&](const uint32_t* jb) {
         uint32_t jj[B];
 #pragma unroll
-        for (int v = 0; v < B; ++v) jj[v] = jb[v] == kPad ? kPad : j0 + jb[v] - base;
+        for (int v = 0; v < B; ++v) {
+          const uint32_t q = jb[v];
+          jj[v] = q == kPad ? kPad : (q < 128u ? j0[0] + q - base[0] : span[0] + j0[1] + (q - 128u) - base[1]);
+        }
         lds(jj);
       });
-      __syncthreads();  // the next run's staging overwrites the stage
+      __syncthreads();  // the next group's staging overwrites the stage
     } else {
-      walk_bits<B>(m0, m1, [&](const uint32_t* jb) {
-        uint32_t jj[B];
 #pragma unroll
-        for (int v = 0; v < B; ++v) jj[v] = jb[v] == kPad ? kPad : j0 + jb[v];
-        glob(jj);
-      });
+      for (uint32_t t = 0; t < 2u; ++t) {
+        if (span[t] == 0u) continue;
+        if (span[t] <= CAP) {
+          for (uint32_t k = lane; k < span[t]; k += 64u) stage(k, base[t] + k);
+          __syncthreads();
+          walk_bits<B>(m[2u * t], m[2u * t + 1u], [&](const uint32_t* jb) {
+            uint32_t jj[B];
+#pragma unroll
+            for (int v = 0; v < B; ++v) jj[v] = jb[v] == kPad ? kPad : j0[t] + jb[v] - base[t];
+            lds(jj);
+          });
+          __syncthreads();
+        } else {
+          walk_bits<B>(m[2u * t], m[2u * t + 1u], [&](const uint32_t* jb) {
+            uint32_t jj[B];
+#pragma unroll
+            for (int v = 0; v < B; ++v) jj[v] = jb[v] == kPad ? kPad : j0[t] + jb[v];
+            glob(jj);
+          });
+        }
+      }
     }
     end_run();
   }
@@ -407,58 +524,62 @@ __global__ __launch_bounds__(kDensBlock) void k_sph_density(
   Sum<real, F == kMixed> acc;
   bool overflow = false;
   const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
-  for (int ddz = -1; ddz <= 1; ++ddz) {
-    for (int ddy = -1; ddy <= 1; ++ddy) {
-      const uint32_t r = (uint32_t)(3 * (ddz + 1) + (ddy + 1));
-      const int zz = cr.cz + ddz, yy = cr.cy + ddy;
-      const bool row_ok = live && zz >= 0 && zz < nz && yy >= 0 && yy < ny;
-      uint32_t j0 = 0, len = 0;
-      if (row_ok) {
-        const uint32_t row = ((uint32_t)zz * (uint32_t)ny + (uint32_t)yy) * (uint32_t)nx;
-        j0 = lb[row + (uint32_t)cr.x0];
-        len = lb[row + (uint32_t)cr.x1 + 1] - j0;
-        if (len > 128u) {
-          overflow = true;
-          len = 128u;
-        }
+  // runs in the WCSPH order, each scanned and walked before the next (a
+  // group's masks held together would cost registers, and this pass's walk
+  // reads global memory, which wants occupancy; the two-level loop also
+  // compiles to fewer registers than one loop of nine)
+  for (uint32_t kz = 0; kz < 3u; ++kz)
+  for (uint32_t ky = 0; ky < 3u; ++ky) {
+    const uint32_t r = sph_run(3u * kz + ky);
+    const int zz = cr.cz + (int)(r / 3u) - 1, yy = cr.cy + (int)(r % 3u) - 1;
+    const bool row_ok = live && zz >= 0 && zz < nz && yy >= 0 && yy < ny;
+    uint32_t j0 = 0, len = 0;
+    if (row_ok) {
+      const uint32_t row = ((uint32_t)zz * (uint32_t)ny + (uint32_t)yy) * (uint32_t)nx;
+      j0 = lb[row + (uint32_t)cr.x0];
+      len = lb[row + (uint32_t)cr.x1 + 1] - j0;
+      if (len > 128u) {
+        overflow = true;
+        len = 128u;
       }
-      if (live) meta[meta_index(r, n, i)] = make_uint2(j0, len);
-      // union of the wavefront's candidate ranges of this run
-      const uint32_t base = wave_min(len ? j0 : 0xFFFFFFFFu);
-      const uint32_t top = wave_max(len ? j0 + len : 0u);
-      if (top <= base) continue;
-      const bool staged = top - base <= kStageCap;
-      if (staged) {
-        for (uint32_t kk = lane; kk < top - base; kk += kDensBlock) s_q[kk] = offset(spos[base + kk]);
-        __syncthreads();
-      }
-      if (len) {
-        uint64_t m0, m1;
-        if (staged) {
-          scan_run([&](uint32_t j) { return s_q[j - base]; }, sp, j0, len, r, words, n, i, m0, m1);
-        } else {
-          scan_run([&](uint32_t j) { return offset(spos[j < n ? j : n - 1]); }, sp, j0, len, r, words, n, i, m0,
-                   m1);
-        }
-        walk_bits<kDensUnr>(m0, m1, [&](const uint32_t* jj) {
-          G4 pj[kDensUnr];
-#pragma unroll
-          for (int v = 0; v < kDensUnr; ++v) pj[v] = pos[j0 + (jj[v] == kPad ? jj[0] : jj[v])];
-#pragma unroll
-          for (int v = 0; v < kDensUnr; ++v) {
-            if (jj[v] == kPad) continue;
-            real dx, dy, dz;
-            form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
-            real r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 < H2) {
-              real q = sqrt(r2) * inv_h;
-              acc.add(spline_f(q));
-            }
-          }
-        });
-      }
-      if (staged) __syncthreads();  // the next run's staging overwrites s_q
     }
+    if (live) meta[meta_index(r, n, i)] = make_uint2(j0, len);
+    // union of the wavefront's candidate ranges of this run
+    const uint32_t base = wave_min(len ? j0 : 0xFFFFFFFFu);
+    const uint32_t top = wave_max(len ? j0 + len : 0u);
+    if (top <= base) continue;
+    const bool staged = top - base <= kStageCap;
+    if (staged) {
+      for (uint32_t kk = lane; kk < top - base; kk += kDensBlock) s_q[kk] = offset(spos[base + kk]);
+      __syncthreads();
+    }
+    if (len) {
+      uint64_t m0, m1;
+      if (staged) {
+        scan_run([&](uint32_t j) { return s_q[j - base]; }, sp, j0, len, r, words, n, i, m0, m1);
+      } else {
+        scan_run([&](uint32_t j) { return offset(spos[j < n ? j : n - 1]); }, sp, j0, len, r, words, n, i, m0,
+                 m1);
+      }
+      walk_bits<kDensUnr>(m0, m1, [&](const uint32_t* jj) {
+        G4 pj[kDensUnr];
+#pragma unroll
+        for (int v = 0; v < kDensUnr; ++v) pj[v] = pos[j0 + (jj[v] == kPad ? jj[0] : jj[v])];
+#pragma unroll
+        for (int v = 0; v < kDensUnr; ++v) {
+          if (jj[v] == kPad) continue;
+          real dx, dy, dz;
+          form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
+          real r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 < H2) {
+            // the self pair (and coincident slots) has r2 = 0, q = 0
+            real q = r2 > real(0.0) ? sqrt_pos(r2) * inv_h : real(0.0);
+            acc.add(spline_f(q));
+          }
+        }
+      });
+    }
+    if (staged) __syncthreads();  // the next run's staging overwrites s_q
   }
   if (!live) return;
   if (overflow) atomicOr(&flags[FLAG_CAPACITY], 1u);
@@ -545,7 +666,7 @@ __device__ inline bool force_pair(const typename SphForm<F>::G4& pi, const typen
   form_pair<F>(pi, pj, u.cs_u, dx, dy, dz);
   real r2 = dx * dx + dy * dy + dz * dz;
   if (!(r2 < H2 && r2 > real(0.0))) return false;
-  real rr = sqrt(r2);
+  real rr = sqrt_pos(r2);
   real q = rr * inv_h;
   real dw = spline_df(q);
   real dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
@@ -559,7 +680,7 @@ __device__ inline bool force_pair(const typename SphForm<F>::G4& pi, const typen
     num = (pri + prj) * dw;
     den = rr;
   }
-  real sc = num / den;
+  real sc = div_pos(num, den);
   rx = sc * dx;
   ry = sc * dy;
   rz = sc * dz;
