@@ -265,6 +265,38 @@ __device__ inline uint32_t scan_word(Ld&& ld, uint32_t k0, uint32_t cnt, const S
   return cnt < 32u ? word & ((1u << cnt) - 1u) : word;
 }
 
+// word = 2 word + hit in one instruction: the wavefront's hit mask is the
+// carry-in of an add with carry, each lane taking its own bit.
+__device__ inline uint32_t shift_in(uint32_t w, bool hit) {
+  const uint64_t m = __builtin_amdgcn_ballot_w64(hit);
+  uint32_t r;
+  asm("v_addc_co_u32 %0, vcc, %1, %1, %2" : "=v"(r) : "v"(w), "s"(m) : "vcc");
+  return r;
+}
+
+// scan_word for LDS-staged candidates that carry |q|^2 in .w: |s - q|^2 <
+// H2p is tested as q.w - 2 s.q < H2p - |s|^2, three FMAs per candidate with
+// a = -2 s and t = H2p - |s|^2 precomputed.  The rounding of this form grows
+// with the offsets' magnitude M, so t carries a margin in M (density kernel).
+template <class Ld>
+__device__ inline uint32_t scan_word_dot(Ld&& ld, uint32_t k0, uint32_t cnt, float ax, float ay, float az,
+                                         float t) {
+  uint32_t word = 0;
+  for (uint32_t b = 0; b < cnt; b += kScanUnr) {
+    float4 q[kScanUnr];
+#pragma unroll
+    for (int u = 0; u < kScanUnr; ++u) q[u] = ld(k0 + b + (uint32_t)u);
+#pragma unroll
+    for (int u = 0; u < kScanUnr; ++u) {
+      const float v = __builtin_fmaf(ax, q[u].x, __builtin_fmaf(ay, q[u].y, __builtin_fmaf(az, q[u].z, q[u].w)));
+      word = shift_in(word, v < t);
+    }
+  }
+  const uint32_t P = (cnt + (uint32_t)kScanUnr - 1u) & ~((uint32_t)kScanUnr - 1u);
+  word = __builtin_bitreverse32(word) >> (32u - P);
+  return cnt < 32u ? word & ((1u << cnt) - 1u) : word;
+}
+
 // Row-run masks of the density pass, kept for the force pass and the
 // adjoint.  Run r = 3 (dz + 1) + (dy + 1) of slot i (the order for_each_run
 // visits rows): meta[r n + i] = (j0, len) of the run's candidate range (len
@@ -280,18 +312,18 @@ __device__ inline size_t word_index(uint32_t r, uint32_t w, uint32_t n, uint32_t
 }
 
 // Scan one row run [j0, j0 + len), len <= 128: bit b of the 128-bit mask
-// (m0, m1) is set when candidate j0 + b lies inside the probe cutoff; ld(j)
-// returns the scan offsets of slot j.  The run's words are also stored as
-// run r of slot i.
-template <class Ld>
-__device__ inline void scan_run(Ld&& ld, const ScanPoint& s, uint32_t j0, uint32_t len, uint32_t r,
-                                uint32_t* __restrict__ words, uint32_t n, uint32_t i, uint64_t& m0, uint64_t& m1) {
+// (m0, m1) is set when candidate j0 + b lies inside the probe cutoff;
+// wf(k0, cnt) scans cnt <= 32 candidates from slot k0 into a word.  The run's
+// words are also stored as run r of slot i.
+template <class Wf>
+__device__ inline void scan_run(Wf&& wf, uint32_t j0, uint32_t len, uint32_t r, uint32_t* __restrict__ words,
+                                uint32_t n, uint32_t i, uint64_t& m0, uint64_t& m1) {
   uint32_t wd[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
   for (uint32_t w = 0; w < 4u; ++w) {
     if (len > 32u * w) {
       const uint32_t cnt = len - 32u * w < 32u ? len - 32u * w : 32u;
-      wd[w] = scan_word(ld, j0 + 32u * w, cnt, s);
+      wd[w] = wf(j0 + 32u * w, cnt);
       words[word_index(r, w, n, i)] = wd[w];
     }
   }
@@ -388,6 +420,10 @@ __device__ inline uint32_t wave_max(uint32_t v) {
   }
   return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
+
+// Maximum of a non-negative float over the wavefront (their bit patterns
+// order like the values).
+__device__ inline float wave_max_pos(float v) { return __uint_as_float(wave_max(__float_as_uint(v))); }
 
 // Walk of a wavefront over its lanes' stored masks (one wavefront per block;
 // every lane calls it, live or not), run groups in the WCSPH order.  For each
@@ -520,6 +556,10 @@ __global__ __launch_bounds__(kDensBlock) void k_sph_density(
     cr = decode_cell(skeys[i], g);
     pi = pos[i];
   }
+  // dot-form scan of staged runs: a = -2 s, and |s|^2 for the threshold
+  const float ax = -2.0f * sp.x, ay = -2.0f * sp.y, az = -2.0f * sp.z;
+  const float ss = __builtin_fmaf(sp.x, sp.x, __builtin_fmaf(sp.y, sp.y, sp.z * sp.z));
+  const float ss_max = wave_max_pos(ss);
   const real H2 = (real)u.H2, inv_h = (real)u.inv_h;
   Sum<real, F == kMixed> acc;
   bool overflow = false;
@@ -549,17 +589,37 @@ __global__ __launch_bounds__(kDensBlock) void k_sph_density(
     const uint32_t top = wave_max(len ? j0 + len : 0u);
     if (top <= base) continue;
     const bool staged = top - base <= kStageCap;
+    float t = 0.0f;
     if (staged) {
-      for (uint32_t kk = lane; kk < top - base; kk += kDensBlock) s_q[kk] = offset(spos[base + kk]);
+      float mq = 0.0f;
+      for (uint32_t kk = lane; kk < top - base; kk += kDensBlock) {
+        float4 q = offset(spos[base + kk]);
+        q.w = __builtin_fmaf(q.x, q.x, __builtin_fmaf(q.y, q.y, q.z * q.z));
+        s_q[kk] = q;
+        mq = q.w > mq ? q.w : mq;
+      }
+      // Rounding of the dot form with |s|, |q| <= M: |q|^2 (3 roundings),
+      // the three FMAs (partial sums up to 3 M^2), |s|^2 and the threshold,
+      // and the FP32 offsets themselves (2 u M per component on a pair vector
+      // of length < 2) stay below u (22 M^2 + 14 M + 5); the margin is
+      // 32 u (M^2 + M + 1), u = 2^-24, on top of the common probe margin.
+      const float m2 = fmaxf(ss_max, wave_max_pos(mq));
+      const float h2p = u.H2_probe + 0x1.0p-19f * (m2 + __builtin_sqrtf(m2) + 1.0f);
+      t = h2p - ss;
       __syncthreads();
     }
     if (len) {
       uint64_t m0, m1;
       if (staged) {
-        scan_run([&](uint32_t j) { return s_q[j - base]; }, sp, j0, len, r, words, n, i, m0, m1);
+        scan_run([&](uint32_t k0, uint32_t cnt) {
+                   return scan_word_dot([&](uint32_t j) { return s_q[j - base]; }, k0, cnt, ax, ay, az, t);
+                 },
+                 j0, len, r, words, n, i, m0, m1);
       } else {
-        scan_run([&](uint32_t j) { return offset(spos[j < n ? j : n - 1]); }, sp, j0, len, r, words, n, i, m0,
-                 m1);
+        scan_run([&](uint32_t k0, uint32_t cnt) {
+                   return scan_word([&](uint32_t j) { return offset(spos[j < n ? j : n - 1]); }, k0, cnt, sp);
+                 },
+                 j0, len, r, words, n, i, m0, m1);
       }
       walk_bits<kDensUnr>(m0, m1, [&](const uint32_t* jj) {
         G4 pj[kDensUnr];
