@@ -50,17 +50,27 @@ def _context(pa, scene_id, scene_args, precision):
     return d, arrs, ctx
 
 
+KERNEL_STEPS = 200   # steps whose kernels are timed with HIP events, after the timed region
+
+
 def run_form(pa, scene_args, precision, steps, warmup):
+    """Throughput over `steps` timed steps with nothing recorded between the
+    kernels (a HIP event pair around every launch stretches the stream by
+    about 0.15 ms per step), then each kernel's average duration from HIP
+    events on the library's stream over the next min(steps, KERNEL_STEPS)
+    steps of the same run."""
     d, _, ctx = _context(pa, pa.SCENE_DAM_BREAK, scene_args, precision)
     with ctx:
         ctx.step(warmup)
-        ctx.sync()
-        ctx.timing_enable(True)   # per-kernel HIP events on the library's stream
         ctx.sync()
         t0 = time.perf_counter()
         ctx.step(steps)
         ctx.sync()
         t1 = time.perf_counter()
+        ctx.timing_enable(True)   # per-kernel HIP events on the library's stream
+        ctx.sync()
+        ctx.step(min(steps, KERNEL_STEPS))
+        ctx.sync()
         kt = ctx.timing()
         ctx.timing_enable(False)
     sec = t1 - t0
