@@ -309,34 +309,44 @@ static void launch_reorder(Ctx& c, const uint32_t* perm, const uint32_t* skeys, 
   PR_HIP(hipGetLastError());
 }
 
-// lb[c] = first slot with key >= c.  Slot k (k >= 1) fills the cells
-// (key[k-1], key[k]]; cells up to key[0] and after key[n-1] are filled by
-// k_cell_lb_ends.  Every cell is written exactly once; no clearing pass.
-__global__ __launch_bounds__(kBlock) void k_cell_lb(uint32_t n, const uint32_t* __restrict__ skeys,
-                                                    uint32_t* __restrict__ lb) {
-  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k == 0 || k >= n) return;
-  uint32_t a = skeys[k - 1], b = skeys[k];
-  for (uint32_t c = a + 1; c <= b; ++c) lb[c] = k;
-}
+// lb[c] = first slot with key >= c, for c = 0 .. n_cells.  Thread k = 0 .. n
+// owns the cells [lo, hi) whose lower bound is k: lo = key[k-1] + 1 (0 for
+// k = 0), hi = key[k] + 1 (n_cells + 1 for k = n), so every cell is written
+// exactly once and no clearing pass is needed.  A lane writes a range of up
+// to kOwnCells cells itself; longer ranges (the empty stretches of a row
+// beside the walls) are written by the whole wavefront, 64 cells per store.
+constexpr uint32_t kOwnCells = 4;
 
-__global__ __launch_bounds__(kBlock) void k_cell_lb_ends(uint32_t n, uint32_t n_cells,
-                                                         const uint32_t* __restrict__ skeys,
-                                                         uint32_t* __restrict__ lb) {
-  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c > n_cells) return;
-  if (n == 0) { lb[c] = 0; return; }
-  uint32_t first = skeys[0], last = skeys[n - 1];
-  if (c <= first) lb[c] = 0;
-  else if (c > last) lb[c] = n;
+__global__ __launch_bounds__(kBlock) void k_cell_lb(uint32_t n, uint32_t n_cells,
+                                                    const uint32_t* __restrict__ skeys,
+                                                    uint32_t* __restrict__ lb) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t lo = 0, hi = 0;
+  if (k <= n) {
+    lo = k == 0 ? 0u : skeys[k - 1] + 1u;
+    hi = k == n ? n_cells + 1u : skeys[k] + 1u;
+  }
+  const uint32_t len = hi > lo ? hi - lo : 0u;
+  if (len <= kOwnCells)
+    for (uint32_t c = lo; c < hi; ++c) lb[c] = k;
+  // wave-uniform loop over the lanes with long ranges
+  uint64_t m = __ballot(len > kOwnCells);
+  const uint32_t lane = __lane_id();
+  while (m) {
+    const int l = __builtin_ctzll(m);
+    m &= m - 1;
+    const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)lo, l);
+    const uint32_t e = (uint32_t)__builtin_amdgcn_readlane((int)hi, l);
+    const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)k, l);
+    for (uint32_t c = a + lane; c < e; c += 64u) lb[c] = v;
+  }
 }
 
 void neighbors_build(Ctx& c) {
   const uint32_t n = (uint32_t)c.n;
   if (!c.keys_valid) keys_from_positions(c);
   if (n == 0) {
-    k_cell_lb_ends<<<blocks_for((uint64_t)c.grid.n_cells + 1), kBlock, 0, c.stream>>>(
-        0, c.grid.n_cells, nullptr, c.lb.as<uint32_t>());
+    k_cell_lb<<<1, kBlock, 0, c.stream>>>(0, c.grid.n_cells, nullptr, c.lb.as<uint32_t>());
     PR_HIP(hipGetLastError());
     c.sorted = true; c.built = true;
     return;
@@ -371,10 +381,8 @@ void neighbors_build(Ctx& c) {
   c.cur_ids ^= 1;
   {
     LaunchScope ls(&c, "cell_lb", 8.0 * n + 4.0 * c.grid.n_cells);
-    k_cell_lb<<<blocks_for(n), kBlock, 0, c.stream>>>(n, skeys, c.lb.as<uint32_t>());
-    PR_HIP(hipGetLastError());
-    k_cell_lb_ends<<<blocks_for((uint64_t)c.grid.n_cells + 1), kBlock, 0, c.stream>>>(
-        n, c.grid.n_cells, skeys, c.lb.as<uint32_t>());
+    k_cell_lb<<<blocks_for((uint64_t)n + 1), kBlock, 0, c.stream>>>(n, c.grid.n_cells, skeys,
+                                                                    c.lb.as<uint32_t>());
     PR_HIP(hipGetLastError());
   }
   c.sorted = true;
