@@ -196,6 +196,17 @@ static void allocate(Ctx& c) {
   const uint64_t tiles = sort_tiles(n) + 1;
   c.sort_hist.alloc(256 * tiles * 4);
   c.sort_tot.alloc(256 * 4);
+  c.inc_mask.alloc(((nn + 63) / 64) * 8);
+  c.inc_tiles.alloc((2 * tiles + 1) * 4);
+  c.inc_count.alloc(4);
+  c.inc_mov.alloc(nn * 8);
+  c.inc_tmp.alloc(nn * 8);
+  PR_HIP(hipHostMalloc((void**)&c.inc_host, sizeof(uint32_t), hipHostMallocDefault));
+  *c.inc_host = 0;
+  if (const char* sp = std::getenv("PRESTIGE_SORT")) {
+    if (std::string(sp) == "radix") c.sort_policy = Ctx::kSortRadix;
+    else if (std::string(sp) == "incremental") c.sort_policy = Ctx::kSortIncremental;
+  }
   c.lb.alloc(((size_t)c.grid.n_cells + 1) * 4);
   c.flags.alloc(FLAG_COUNT * 4);
   PR_HIP(hipMemsetAsync(c.flags.p, 0, FLAG_COUNT * 4, c.stream));
@@ -333,6 +344,7 @@ static void upload(Ctx& c, const std::string& name, const double* host, uint64_t
   if (name == "x" || name == "y" || name == "z") {
     c.keys_valid = false;
     c.sorted = false;
+    c.inc_ready = false;
     if (c.desc.model == PRESTIGE_MODEL_DEM) {
       // new positions invalidate the tangential contact history
       const size_t hr = c.desc.precision == PRESTIGE_MIXED ? 4 : 8;

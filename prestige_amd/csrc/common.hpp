@@ -164,6 +164,12 @@ struct Tape {
 };
 
 struct Ctx {
+  Ctx() = default;
+  Ctx(const Ctx&) = delete;
+  Ctx& operator=(const Ctx&) = delete;
+  ~Ctx() {
+    if (inc_host) (void)hipHostFree(inc_host);
+  }
   prestige_desc desc{};
   int device = 0;
   hipStream_t stream = nullptr;
@@ -190,6 +196,14 @@ struct Ctx {
   DevBuf keys[2], vals[2];       // radix-sort ping-pong
   int perm_buf = 0;              // which vals[] holds the last permutation
   DevBuf sort_hist, sort_tot;    // per-tile digit counts, per-digit totals
+  // incremental sort: moved-slot mask (bit per slot), per-tile moved counts
+  // and offsets, moved count, moved entries (key << 32 | slot) and scratch
+  DevBuf inc_mask, inc_tiles, inc_count, inc_mov, inc_tmp;
+  uint32_t* inc_host = nullptr;  // pinned: moved count of the last incremental sort
+  bool inc_ready = false;        // keys[cur_key] are the next keys of the last build's slots
+  uint32_t inc_skipped = 0;      // radix builds since the moved count was last measured
+  enum SortPolicy { kSortAuto = 0, kSortRadix = 1, kSortIncremental = 2 };
+  int sort_policy = kSortAuto;   // PRESTIGE_SORT=radix|incremental overrides the choice
   DevBuf lb;                     // n_cells + 1
   DevBuf flags;                  // FLAG_COUNT uint32
   // gather outputs

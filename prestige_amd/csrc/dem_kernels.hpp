@@ -263,6 +263,7 @@ void dem_step_impl(Ctx& c) {
   c.cur_hist ^= 1;
   c.sorted = false;
   c.keys_valid = true;
+  c.inc_ready = true;   // keys[cur_key ^ 1] and lb still describe the slots the step kept
 }
 
 }  // namespace prestige

@@ -41,6 +41,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(uint32_t n, const T4* __re
 }
 
 void keys_from_positions(Ctx& c) {
+  c.inc_ready = false;
   if (c.n == 0) { c.keys_valid = true; return; }
   uint32_t n = (uint32_t)c.n;
   LaunchScope ls(&c, "cell_keys", 32.0 * n);
@@ -257,6 +258,296 @@ static void radix_sort(Ctx& c) {
 }
 
 // ---------------------------------------------------------------------------
+// incremental stable sort.  Between two builds only the slots whose particle
+// changed cell ("moved", key_new != key_old) break the order: slots are in
+// key_old order, so the slots that kept their key are still sorted and keep
+// their relative order, and the stable sort of key_new merges them with the
+// few moved ones sorted by (key_new, slot).  With P(t) = moved slots before
+// slot t and M the moved entries sorted by (key, slot):
+//   kept slot i (key c):  position i - P(i) + #{M entries < (c, i)}
+//   moved entry b = (c, i): position b + t - P(t), t = lb_old[c] if i <
+//                           lb_old[c], else lb_old[c + 1]
+// (the kept slots before (c, i) are the slots before t that kept their key;
+// i never lies inside cell c's old range, whose slots all had key c).  Five
+// light launches replace the radix sort's nine; the permutation and keys are
+// exactly the stable sort's, as the radix sort would give them.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kIncLds = 16384;      // moved entries one block sorts in LDS (128 KB)
+constexpr int kIncSortThreads = 1024;
+
+// moved bit per slot (mask word w covers slots 64 w .. 64 w + 63) and the
+// number of moved slots of each kTile-slot tile
+__global__ __launch_bounds__(kSortThreads) void k_inc_flags(uint32_t n, const uint32_t* __restrict__ knew,
+                                                            const uint32_t* __restrict__ kold,
+                                                            uint64_t* __restrict__ mask,
+                                                            uint32_t* __restrict__ tile_cnt) {
+  __shared__ uint32_t s_cnt;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * (uint32_t)kTile;
+  uint32_t cnt = 0;
+  for (int it = 0; it < kSortItems; ++it) {
+    const uint32_t i = base + (uint32_t)it * kSortThreads + threadIdx.x;
+    const uint64_t b = __ballot(i < n && knew[i] != kold[i]);
+    // lane 0's slot is the word's first (slots of a wavefront are 64-aligned)
+    if (__lane_id() == 0 && i < n) {
+      mask[i >> 6] = b;
+      cnt += (uint32_t)__popcll(b);
+    }
+  }
+  if (__lane_id() == 0) atomicAdd(&s_cnt, cnt);
+  __syncthreads();
+  if (threadIdx.x == 0) tile_cnt[blockIdx.x] = s_cnt;
+}
+
+// one block: tile_off[t] = moved slots before tile t (t = 0 .. n_tiles; the
+// last entry is the total, also written to *count)
+__global__ __launch_bounds__(kSortThreads) void k_inc_scan(const uint32_t* __restrict__ tile_cnt,
+                                                           uint32_t n_tiles, uint32_t* __restrict__ tile_off,
+                                                           uint32_t* __restrict__ count) {
+  __shared__ uint32_t s4[4];
+  uint32_t carry = 0;
+  for (uint32_t b = 0; b < n_tiles; b += kSortThreads) {
+    const uint32_t idx = b + threadIdx.x;
+    const uint32_t v = idx < n_tiles ? tile_cnt[idx] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_excl_scan(v, s4, &total);
+    if (idx < n_tiles) tile_off[idx] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) {
+    tile_off[n_tiles] = carry;
+    count[0] = carry;
+  }
+}
+
+// Moved slots of a tile in slot order, as (key << 32 | slot), from
+// tile_off[tile]; visit(i, rank_before_i, moved) sees every slot of the tile
+// with the number of moved slots before it.  Every thread of the block calls.
+template <class Visit>
+__device__ inline void inc_tile_walk(uint32_t n, const uint64_t* __restrict__ mask, uint32_t run,
+                                     Visit&& visit) {
+  __shared__ uint32_t s_w[kSortThreads / 64];
+  const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
+  const uint32_t base = blockIdx.x * (uint32_t)kTile;
+  for (int it = 0; it < kSortItems; ++it) {
+    const uint32_t i0 = base + (uint32_t)it * kSortThreads + 64u * w;   // the wavefront's first slot
+    const uint32_t i = i0 + lane;
+    const uint64_t word = i0 < n ? mask[i0 >> 6] : 0ull;
+    if (lane == 0) s_w[w] = (uint32_t)__popcll(word);
+    __syncthreads();
+    uint32_t before = run, total = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kSortThreads / 64; ++k) {
+      before += k < w ? s_w[k] : 0u;
+      total += s_w[k];
+    }
+    if (i < n) visit(i, before + (uint32_t)__popcll(word & lanes_below()), ((word >> lane) & 1ull) != 0ull);
+    run += total;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_inc_gather(uint32_t n, const uint32_t* __restrict__ knew,
+                                                             const uint64_t* __restrict__ mask,
+                                                             const uint32_t* __restrict__ tile_off,
+                                                             uint64_t* __restrict__ mov) {
+  inc_tile_walk(n, mask, tile_off[blockIdx.x], [&](uint32_t i, uint32_t before, bool moved) {
+    if (moved) mov[before] = ((uint64_t)knew[i] << 32) | i;
+  });
+}
+
+// One block sorts the moved entries by (key, slot): bitonic in LDS when they
+// fit, else a stable LSD radix sort on the key over global memory (entries
+// arrive in slot order, so key order alone keeps (key, slot) order).
+__global__ __launch_bounds__(kIncSortThreads) void k_inc_sort(const uint32_t* __restrict__ count, int key_bits,
+                                                              uint64_t* __restrict__ mov,
+                                                              uint64_t* __restrict__ tmp) {
+  __shared__ uint64_t s[kIncLds];
+  const uint32_t m = count[0];
+  const uint32_t t = threadIdx.x;
+  if (m <= 1u) return;
+  if (m <= kIncLds) {
+    uint32_t P = 2;
+    while (P < m) P <<= 1;
+    for (uint32_t k = t; k < P; k += kIncSortThreads) s[k] = k < m ? mov[k] : ~0ull;
+    __syncthreads();
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+        for (uint32_t a = t; a < P; a += kIncSortThreads) {
+          const uint32_t b = a ^ j;
+          if (b > a) {
+            const uint64_t x = s[a], y = s[b];
+            const bool up = (a & k) == 0;
+            if ((x > y) == up) {
+              s[a] = y;
+              s[b] = x;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (uint32_t k = t; k < m; k += kIncSortThreads) mov[k] = s[k];
+    return;
+  }
+  // radix fallback: digit histogram, then a stable scatter in chunks of
+  // kIncSortThreads entries (wavefront rank by ballot match + counts of the
+  // earlier wavefronts of the chunk)
+  constexpr uint32_t kW = kIncSortThreads / 64;
+  uint32_t* hist = reinterpret_cast<uint32_t*>(s);            // [kBuckets]
+  uint32_t* wcnt = hist + kBuckets;                           // [kW][kBuckets]
+  const uint32_t w = t >> 6;
+  uint64_t* src = mov;
+  uint64_t* dst = tmp;
+  const int passes = (key_bits + kRadixBits - 1) / kRadixBits;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = 32 + p * kRadixBits;
+    for (uint32_t k = t; k < kBuckets; k += kIncSortThreads) hist[k] = 0;
+    __syncthreads();
+    for (uint32_t c0 = 0; c0 < m; c0 += kIncSortThreads) {
+      const uint32_t k = c0 + t;
+      const bool valid = k < m;
+      const uint32_t d = valid ? (uint32_t)(src[k] >> shift) & (kBuckets - 1) : 0u;
+      const uint64_t peers = match_digit(d, valid);
+      if (valid && (peers & lanes_below()) == 0ull) atomicAdd(&hist[d], (uint32_t)__popcll(peers));
+    }
+    __syncthreads();
+    if (t < 64) {   // exclusive scan of the 256 counts by one wavefront, 4 per lane
+      uint32_t v[4], sum = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        v[q] = hist[4 * t + q];
+        sum += v[q];
+      }
+      const uint32_t incl = wave_incl_scan(sum);
+      uint32_t run = incl - sum;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        hist[4 * t + q] = run;
+        run += v[q];
+      }
+    }
+    __syncthreads();
+    for (uint32_t c0 = 0; c0 < m; c0 += kIncSortThreads) {
+      const uint32_t k = c0 + t;
+      const bool valid = k < m;
+      const uint64_t e = valid ? src[k] : 0ull;
+      const uint32_t d = (uint32_t)(e >> shift) & (kBuckets - 1);
+      for (uint32_t q = t; q < kW * kBuckets; q += kIncSortThreads) wcnt[q] = 0;
+      __syncthreads();
+      const uint64_t peers = match_digit(d, valid);
+      const uint32_t rank = (uint32_t)__popcll(peers & lanes_below());
+      if (valid && rank == 0) wcnt[w * kBuckets + d] = (uint32_t)__popcll(peers);
+      __syncthreads();
+      if (t < kBuckets) {   // per digit: offsets of the wavefronts of this chunk
+        uint32_t r = hist[t];
+        for (uint32_t q = 0; q < kW; ++q) {
+          const uint32_t c = wcnt[q * kBuckets + t];
+          wcnt[q * kBuckets + t] = r;
+          r += c;
+        }
+        hist[t] = r;
+      }
+      __syncthreads();
+      if (valid) dst[wcnt[w * kBuckets + d] + rank] = e;
+      __syncthreads();
+    }
+    uint64_t* x = src;
+    src = dst;
+    dst = x;
+  }
+  if (src != mov)
+    for (uint32_t k = t; k < m; k += kIncSortThreads) mov[k] = src[k];
+}
+
+// Moved slots before slot t (t <= n).
+__device__ inline uint32_t inc_moved_before(uint32_t t, const uint64_t* __restrict__ mask,
+                                            const uint32_t* __restrict__ tile_off) {
+  const uint32_t tile = t / (uint32_t)kTile;
+  uint32_t c = tile_off[tile];
+  const uint32_t w1 = t >> 6;
+  for (uint32_t w = tile * ((uint32_t)kTile / 64u); w < w1; ++w) c += (uint32_t)__popcll(mask[w]);
+  if (t & 63u) c += (uint32_t)__popcll(mask[w1] & ((1ull << (t & 63u)) - 1ull));
+  return c;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_inc_place(
+    uint32_t n, uint32_t n_tiles, const uint32_t* __restrict__ knew, const uint64_t* __restrict__ mask,
+    const uint32_t* __restrict__ tile_off, const uint64_t* __restrict__ mov, const uint32_t* __restrict__ count,
+    const uint32_t* __restrict__ lb_old, uint32_t* __restrict__ kout, uint32_t* __restrict__ perm) {
+  const uint32_t m = count[0];
+  if (blockIdx.x < n_tiles) {
+    inc_tile_walk(n, mask, tile_off[blockIdx.x], [&](uint32_t i, uint32_t before, bool moved) {
+      if (moved) return;
+      const uint32_t key = knew[i];
+      const uint64_t e = ((uint64_t)key << 32) | i;
+      uint32_t lo = 0, hi = m;   // moved entries below (key, i)
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (mov[mid] < e) lo = mid + 1;
+        else hi = mid;
+      }
+      const uint32_t pos = i - before + lo;
+      kout[pos] = key;
+      perm[pos] = i;
+    });
+  }
+  for (uint32_t b = blockIdx.x * (uint32_t)kSortThreads + threadIdx.x; b < m; b += gridDim.x * (uint32_t)kSortThreads) {
+    const uint64_t e = mov[b];
+    const uint32_t key = (uint32_t)(e >> 32), i = (uint32_t)e;
+    const uint32_t t = i < lb_old[key] ? lb_old[key] : lb_old[key + 1];
+    const uint32_t pos = b + t - inc_moved_before(t, mask, tile_off);
+    kout[pos] = key;
+    perm[pos] = i;
+  }
+}
+
+// Stable sort of keys[cur_key] from the previous build's order (keys[cur_key
+// ^ 1] = its sorted keys, lb its cell table); same outputs as radix_sort.
+static void incremental_sort(Ctx& c) {
+  const uint32_t n = (uint32_t)c.n;
+  const uint32_t n_tiles = (uint32_t)sort_tiles(n);
+  const uint32_t* knew = c.keys[c.cur_key].as<uint32_t>();
+  uint32_t* kold = c.keys[c.cur_key ^ 1].as<uint32_t>();
+  uint64_t* mask = c.inc_mask.as<uint64_t>();
+  uint32_t* cnt = c.inc_count.as<uint32_t>();
+  uint32_t* tiles = c.inc_tiles.as<uint32_t>();   // [0, n_tiles) counts, [n_tiles, 2 n_tiles + 1) offsets
+  {
+    LaunchScope ls(&c, "sort_inc_flags", 8.0 * n);
+    k_inc_flags<<<n_tiles, kSortThreads, 0, c.stream>>>(n, knew, kold, mask, tiles);
+    PR_HIP(hipGetLastError());
+  }
+  {
+    LaunchScope ls(&c, "sort_inc_scan", 8.0 * n_tiles);
+    k_inc_scan<<<1, kSortThreads, 0, c.stream>>>(tiles, n_tiles, tiles + n_tiles, cnt);
+    PR_HIP(hipGetLastError());
+  }
+  // moved count for the host's choice at the next build (read without waiting)
+  PR_HIP(hipMemcpyAsync(c.inc_host, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+  {
+    LaunchScope ls(&c, "sort_inc_gather", 4.0 * n);
+    k_inc_gather<<<n_tiles, kSortThreads, 0, c.stream>>>(n, knew, mask, tiles + n_tiles, c.inc_mov.as<uint64_t>());
+    PR_HIP(hipGetLastError());
+  }
+  {
+    LaunchScope ls(&c, "sort_inc_sort", 0.0);
+    k_inc_sort<<<1, kIncSortThreads, 0, c.stream>>>(cnt, key_bits(c.grid.n_cells), c.inc_mov.as<uint64_t>(),
+                                                    c.inc_tmp.as<uint64_t>());
+    PR_HIP(hipGetLastError());
+  }
+  {
+    LaunchScope ls(&c, "sort_inc_place", 12.0 * n);
+    k_inc_place<<<n_tiles, kSortThreads, 0, c.stream>>>(n, n_tiles, knew, mask, tiles + n_tiles,
+                                                        c.inc_mov.as<uint64_t>(), cnt, c.lb.as<uint32_t>(), kold,
+                                                        c.vals[0].as<uint32_t>());
+    PR_HIP(hipGetLastError());
+  }
+  c.cur_key ^= 1;
+  c.perm_buf = 0;
+}
+
+// ---------------------------------------------------------------------------
 // reorder into key order.  PROBE also writes the FP32 probe (offset from the
 // FP64 cell origin times off_scale, cell-coordinate residues mod 4 in .w);
 // MIXED also writes FP32 copies of the velocity (mass in .w) and, with OMEGA,
@@ -351,7 +642,21 @@ void neighbors_build(Ctx& c) {
     c.sorted = true; c.built = true;
     return;
   }
-  radix_sort(c);
+  // incremental sort when the state is still in the previous build's slot
+  // order with fresh keys (right after a step) and the last moved count seen
+  // was small; every few radix builds the moved count is measured again
+  bool inc = false;
+  if (c.inc_ready && c.sort_policy != Ctx::kSortRadix) {
+    const uint32_t seen = *c.inc_host;
+    inc = c.sort_policy == Ctx::kSortIncremental || seen <= kIncLds / 2u || ++c.inc_skipped >= 16u;
+  }
+  if (inc) {
+    c.inc_skipped = 0;
+    incremental_sort(c);
+  } else {
+    radix_sort(c);
+  }
+  c.inc_ready = false;
   const uint32_t* perm = c.vals[c.perm_buf].as<uint32_t>();
   const uint32_t* skeys = c.keys[c.cur_key].as<uint32_t>();
   const int src = c.cur, dst = c.cur ^ 1;

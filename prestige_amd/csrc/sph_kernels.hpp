@@ -975,6 +975,7 @@ void sph_step_impl(Ctx& c) {
     c.cur_key = kdst;
     c.sorted = false;
     c.keys_valid = true;
+    c.inc_ready = true;   // keys[cur_key ^ 1] and lb still describe the slots the step kept
   }
 }
 

@@ -57,6 +57,42 @@ def test_rebuild_after_motion_bitexact():
             _compare_build(ctx, st)
 
 
+@pytest.mark.parametrize("policy,speed,n", [
+    ("incremental", 10.0, 20000),    # a few thousand moved slots: sorted in LDS
+    ("incremental", 40.0, 150000),   # > 16k moved slots: one-block radix path
+    ("radix", 40.0, 150000),
+])
+def test_sort_policies_over_steps(monkeypatch, policy, speed, n):
+    """Each build starts from the previous step's order.  The
+    incremental sort (slots that kept their cell stay in order, moved ones are
+    sorted and merged) and the full radix sort must both give the oracle's
+    stable sort, however many slots moved."""
+    monkeypatch.setenv("PRESTIGE_SORT", policy)
+    d, arrs = make_scene(pa.SCENE_DEM_POUR, n_target=n, seed=7)
+    rng = np.random.default_rng(1)
+    for k in ("vx", "vy", "vz"):
+        arrs[k] = rng.uniform(-speed, speed, d.n_particles)
+    d.dt = 1e-4
+    st = oracle_state(d, arrs)
+    moved = []
+    with pa.Context(d) as ctx:
+        upload_all(ctx, arrs)
+        prev = None
+        for _ in range(4):
+            assert st.step(1) == 0
+            ctx.step(1)
+            # rebuild from the step's order with the keys the step computed
+            assert st.build() == 0
+            ctx.neighbors_build()
+            g = _compare_build(ctx, st)
+            if prev is not None:
+                moved.append(int(np.sum(g["keys"][np.argsort(g["perm"])] != prev)))
+            prev = g["keys"].copy()
+    print(f"{policy}: slots that changed cell per step {moved}")
+    if speed > 20.0:
+        assert min(moved) > 16384
+
+
 def test_eq1_allpairs_bitexact():
     rng = np.random.default_rng(0)
     d, arrs = make_scene(pa.SCENE_DEM_LATTICE, n_target=3000, seed=1)
