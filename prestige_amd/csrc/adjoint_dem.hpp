@@ -401,6 +401,11 @@ void adj_seed_gather(Ctx& c, const uint32_t* ids, const double* seed_x, const do
 void adj_grad_scatter(Ctx& c, const uint32_t* ids, const double4* vb, double* grad_v0_host);
 void ensure_adj_buffers(Ctx& c, uint32_t grid_blocks);
 
+// Keep what the reverse sweep needs from the step that just ran.  Taken by
+// swap (the next step overwrites them before reading): the sorted pre-step
+// state, the permutation and the contact history the step read.  Copied:
+// ids (the next reorder reads them) and the history the step wrote (the next
+// step's input).
 template <class H>
 void record_dem_step(Ctx& c, StepRecord& r) {
   const size_t n = c.n ? c.n : 1, mc = c.desc.max_contacts;
@@ -409,15 +414,15 @@ void record_dem_step(Ctx& c, StepRecord& r) {
     d.alloc(b);
     PR_HIP(hipMemcpyAsync(d.p, s.p, b, hipMemcpyDeviceToDevice, c.stream));
   };
-  cp(r.state_pos, c.pos[prev], n * sizeof(double4));
-  cp(r.state_vel, c.vel[prev], n * sizeof(double4));
-  cp(r.state_om, c.omega[prev], n * sizeof(double4));
+  take_buffer(c.pos[prev], r.state_pos);
+  take_buffer(c.vel[prev], r.state_vel);
+  take_buffer(c.omega[prev], r.state_om);
+  take_buffer(c.vals[c.perm_buf], r.perm);
+  take_buffer(c.hcnt[hin], r.hcnt);
+  take_buffer(c.hid[hin], r.hid);
+  take_buffer(c.hxi[hin], r.hxi);
+  take_buffer(c.wxi[hin], r.wxi);
   cp(r.ids, c.ids[c.cur_ids], n * 4);
-  cp(r.perm, c.vals[c.perm_buf], n * 4);
-  cp(r.hcnt, c.hcnt[hin], n * 4);
-  cp(r.hid, c.hid[hin], n * mc * 4);
-  cp(r.hxi, c.hxi[hin], n * mc * 3 * sizeof(H));
-  cp(r.wxi, c.wxi[hin], n * 18 * sizeof(H));
   cp(r.hcnt_out, c.hcnt[hout], n * 4);
   cp(r.hid_out, c.hid[hout], n * mc * 4);
 }

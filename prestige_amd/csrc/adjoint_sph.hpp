@@ -447,13 +447,6 @@ void adj_seed_gather(Ctx& c, const uint32_t* ids, const double* seed_x, const do
 void adj_grad_scatter(Ctx& c, const uint32_t* ids, const double4* vb, double* grad_v0_host);
 void ensure_adj_buffers(Ctx& c, uint32_t grid_blocks);
 
-// Hand a buffer the step just wrote to the record without copying: the next
-// step only writes into the context side of the swap before reading it.
-inline void take_buffer(DevBuf& ctx_buf, DevBuf& rec_buf) {
-  if (rec_buf.bytes != ctx_buf.bytes) rec_buf.alloc(ctx_buf.bytes);
-  std::swap(ctx_buf.p, rec_buf.p);
-}
-
 // Keep what the reverse sweep needs from the step that just ran.  Taken by
 // swap: the pair geometry (sorted pre-step state or FP32 probes), the
 // permutation and the row-run masks, all of which the next step

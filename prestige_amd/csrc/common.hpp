@@ -9,6 +9,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/prestige_hip.h"
@@ -115,6 +116,13 @@ struct Timing {
 struct Ctx;
 
 // Records `name` around one launch when timing is on.
+// Hand a buffer a step just wrote to an adjoint record without copying: the
+// next step only writes into the context side of the swap before reading it.
+inline void take_buffer(DevBuf& ctx_buf, DevBuf& rec_buf) {
+  if (rec_buf.bytes != ctx_buf.bytes) rec_buf.alloc(ctx_buf.bytes);
+  std::swap(ctx_buf.p, rec_buf.p);
+}
+
 struct LaunchScope {
   Ctx* ctx;
   size_t stat;
