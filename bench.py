@@ -50,16 +50,13 @@ def _context(pa, scene_id, scene_args, precision):
     return d, arrs, ctx
 
 
-KERNEL_STEPS = 200   # steps whose kernels are timed with HIP events, after the timed region
-
-
 def run_form(pa, scene_args, precision, steps, warmup):
     """Throughput over `steps` timed steps with nothing recorded between the
     kernels (a HIP event pair around every launch stretches the stream by
-    about 0.15 ms per step), then each kernel's average duration from HIP
-    events on the library's stream over the next min(steps, KERNEL_STEPS)
-    steps of the same run."""
-    d, _, ctx = _context(pa, pa.SCENE_DAM_BREAK, scene_args, precision)
+    about 0.15 ms per step).  Then the same steps again from the initial
+    state, with HIP events around every kernel on the library's stream: each
+    kernel's average duration over the same stretch of the simulation."""
+    d, arrs, ctx = _context(pa, pa.SCENE_DAM_BREAK, scene_args, precision)
     with ctx:
         ctx.step(warmup)
         ctx.sync()
@@ -67,9 +64,12 @@ def run_form(pa, scene_args, precision, steps, warmup):
         ctx.step(steps)
         ctx.sync()
         t1 = time.perf_counter()
-        ctx.timing_enable(True)   # per-kernel HIP events on the library's stream
+        for k in STATE:
+            ctx.upload(k, arrs[k])
+        ctx.step(warmup)
         ctx.sync()
-        ctx.step(min(steps, KERNEL_STEPS))
+        ctx.timing_enable(True)   # per-kernel HIP events on the library's stream
+        ctx.step(steps)
         ctx.sync()
         kt = ctx.timing()
         ctx.timing_enable(False)
