@@ -273,6 +273,10 @@ static void radix_sort(Ctx& c) {
 // exactly the stable sort's, as the radix sort would give them.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kIncLds = 16384;      // moved entries one block sorts in LDS (128 KB)
+#ifndef PRESTIGE_BITONIC_MAX
+#define PRESTIGE_BITONIC_MAX 2048
+#endif
+constexpr uint32_t kIncBitonicMax = PRESTIGE_BITONIC_MAX;   // larger moved sets take the radix path
 constexpr int kIncSortThreads = 1024;
 
 // moved bit per slot (mask word w covers slots 64 w .. 64 w + 63) and the
@@ -367,7 +371,7 @@ __global__ __launch_bounds__(kIncSortThreads) void k_inc_sort(const uint32_t* __
   const uint32_t m = count[0];
   const uint32_t t = threadIdx.x;
   if (m <= 1u) return;
-  if (m <= kIncLds) {
+  if (m <= kIncBitonicMax && m <= kIncLds) {
     uint32_t P = 2;
     while (P < m) P <<= 1;
     for (uint32_t k = t; k < P; k += kIncSortThreads) s[k] = k < m ? mov[k] : ~0ull;

@@ -58,8 +58,9 @@ def test_rebuild_after_motion_bitexact():
 
 
 @pytest.mark.parametrize("policy,speed,n", [
-    ("incremental", 10.0, 20000),    # a few thousand moved slots: sorted in LDS
-    ("incremental", 40.0, 150000),   # > 16k moved slots: one-block radix path
+    ("incremental", 10.0, 20000),    # about a thousand moved slots: bitonic sort in LDS
+    ("incremental", 20.0, 100000),   # several thousand: one-block radix sort
+    ("incremental", 40.0, 150000),   # more than the LDS holds
     ("radix", 40.0, 150000),
 ])
 def test_sort_policies_over_steps(monkeypatch, policy, speed, n):
