@@ -198,11 +198,13 @@ static void allocate(Ctx& c) {
   c.sort_tot.alloc(256 * 4);
   c.inc_mask.alloc(((nn + 63) / 64) * 8);
   c.inc_tiles.alloc((2 * tiles + 1) * 4);
+  PR_HIP(hipMemsetAsync(c.inc_tiles.p, 0, (2 * tiles + 1) * 4, c.stream));
   c.inc_count.alloc(4);
   c.inc_mov.alloc(nn * 8);
   c.inc_tmp.alloc(nn * 8);
-  PR_HIP(hipHostMalloc((void**)&c.inc_host, sizeof(uint32_t), hipHostMallocDefault));
+  PR_HIP(hipHostMalloc((void**)&c.inc_host, sizeof(uint32_t), hipHostMallocMapped));
   *c.inc_host = 0;
+  PR_HIP(hipHostGetDevicePointer((void**)&c.inc_host_dev, c.inc_host, 0));
   if (const char* sp = std::getenv("PRESTIGE_SORT")) {
     if (std::string(sp) == "radix") c.sort_policy = Ctx::kSortRadix;
     else if (std::string(sp) == "incremental") c.sort_policy = Ctx::kSortIncremental;

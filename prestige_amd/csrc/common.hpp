@@ -116,6 +116,30 @@ struct Timing {
 struct Ctx;
 
 // Records `name` around one launch when timing is on.
+// Moved-slot mask of the next incremental sort, written by the step kernels:
+// word w has bit l set when slot 64 w + l gets a different cell key from the
+// step, and tile_cnt[t] counts the moved slots of sort tile t (kSortTile
+// slots; cleared by the sort that consumes it).  Fixed particles never move,
+// so the ballot of the live lanes is the whole word: the first live lane of
+// the wavefront stores it (live_mask is the wavefront's live lanes, taken
+// while all its lanes with a slot below n were active), and a wavefront
+// without live lanes stores 0 from its fixed path.
+constexpr uint32_t kSortTile = 2048;
+struct MovedOut {
+  uint64_t* mask;
+  uint32_t* tile_cnt;
+};
+__device__ inline void moved_word_live(const MovedOut& mo, uint32_t i, bool moved, uint64_t live_mask) {
+  const uint64_t b = __ballot(moved);
+  if (mo.mask && __lane_id() == (uint32_t)__builtin_ctzll(live_mask)) {
+    mo.mask[i >> 6] = b;
+    if (b) atomicAdd(&mo.tile_cnt[i / kSortTile], (uint32_t)__popcll(b));
+  }
+}
+__device__ inline void moved_word_fixed(const MovedOut& mo, uint32_t i, uint64_t live_mask) {
+  if (mo.mask && live_mask == 0ull && __lane_id() == 0u) mo.mask[i >> 6] = 0ull;
+}
+
 // Hand a buffer a step just wrote to an adjoint record without copying: the
 // next step only writes into the context side of the swap before reading it.
 inline void take_buffer(DevBuf& ctx_buf, DevBuf& rec_buf) {
@@ -207,7 +231,8 @@ struct Ctx {
   // incremental sort: moved-slot mask (bit per slot), per-tile moved counts
   // and offsets, moved count, moved entries (key << 32 | slot) and scratch
   DevBuf inc_mask, inc_tiles, inc_count, inc_mov, inc_tmp;
-  uint32_t* inc_host = nullptr;  // pinned: moved count of the last incremental sort
+  uint32_t* inc_host = nullptr;  // pinned, mapped: moved count of the last incremental sort
+  uint32_t* inc_host_dev = nullptr;  // the same word as the device writes it
   bool inc_ready = false;        // keys[cur_key] are the next keys of the last build's slots
   uint32_t inc_skipped = 0;      // radix builds since the moved count was last measured
   enum SortPolicy { kSortAuto = 0, kSortRadix = 1, kSortIncremental = 2 };

@@ -57,7 +57,7 @@ __global__ __launch_bounds__(kBlock) void k_dem_force(
     const uint32_t* __restrict__ lb, Grid g, DemConst kc, const double4* __restrict__ spos,
     const double4* __restrict__ svel, const double4* __restrict__ som, double4* __restrict__ opos,
     double4* __restrict__ ovel, double4* __restrict__ oom, uint32_t* __restrict__ okeys,
-    double4* __restrict__ oforce, DemHist hist, uint32_t* flags) {
+    double4* __restrict__ oforce, DemHist hist, MovedOut moved, uint32_t* flags) {
   using real = typename DemForm<F>::real;
   using G4 = typename DemForm<F>::G4;
   const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -69,11 +69,13 @@ __global__ __launch_bounds__(kBlock) void k_dem_force(
   real* xi_out = (real*)hist.xi_out;
   real* wxi_out = (real*)hist.wxi_out;
   const uint32_t key_i = skeys[i];
+  const uint64_t live_mask = __ballot((uint64_t)me < n_moving);
   if ((uint64_t)me >= n_moving) {
     opos[i] = spos[i]; ovel[i] = svel[i]; oom[i] = som[i]; okeys[i] = key_i;
     if (oforce) oforce[i] = make_double4(0, 0, 0, 0);
     hist.cnt_out[me] = 0;
     for (int c = 0; c < 18; ++c) wxi_out[(size_t)c * n + me] = real(0.0);
+    moved_word_fixed(moved, i, live_mask);
     return;
   }
   const CellRows cr = decode_cell(key_i, g);
@@ -227,6 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_dem_force(
   else
     atomicOr(&flags[FLAG_DOMAIN], 1u);
   okeys[i] = key;
+  moved_word_live(moved, i, key != key_i, live_mask);
 }
 
 template <int F>
@@ -256,7 +259,7 @@ void dem_step_impl(Ctx& c) {
       c.lb.as<uint32_t>(), c.grid, c.dem, c.pos[c.cur].as<double4>(), c.vel[c.cur].as<double4>(),
       c.omega[c.cur].as<double4>(), c.pos[dst].as<double4>(), c.vel[dst].as<double4>(),
       c.omega[dst].as<double4>(), c.keys[kdst].as<uint32_t>(), c.keep_acc ? c.acc.as<double4>() : nullptr,
-      hist, c.flags.as<uint32_t>());
+      hist, MovedOut{c.inc_mask.as<uint64_t>(), c.inc_tiles.as<uint32_t>()}, c.flags.as<uint32_t>());
   PR_HIP(hipGetLastError());
   c.cur = dst;
   c.cur_key = kdst;

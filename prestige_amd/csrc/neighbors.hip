@@ -66,6 +66,7 @@ constexpr int kSortThreads = 256;
 #endif
 constexpr int kSortItems = PRESTIGE_SORT_ITEMS;   // keys per thread and tile
 constexpr int kTile = kSortThreads * kSortItems;
+static_assert(kTile == (int)kSortTile, "the step kernels count moved slots per sort tile");
 static_assert(kSortThreads == kBuckets, "one thread per bucket in the scan phases");
 
 // Lanes of this wavefront whose digit equals mine (and whose item is valid):
@@ -88,7 +89,8 @@ __device__ inline uint64_t lanes_below() {
 __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t* __restrict__ keys,
                                                             uint32_t n, int shift,
                                                             uint32_t n_tiles,
-                                                            uint32_t* __restrict__ hist) {
+                                                            uint32_t* __restrict__ hist,
+                                                            uint32_t* __restrict__ tile_cnt) {
   __shared__ uint32_t h[kBuckets];
   const uint32_t t = threadIdx.x;
   h[t] = 0;
@@ -103,6 +105,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t* __re
   }
   __syncthreads();
   hist[t * n_tiles + blockIdx.x] = h[t];
+  // the moved-slot counts the last step kernel left for an incremental sort
+  // this build does not use start again from zero
+  if (t == 0 && tile_cnt) tile_cnt[blockIdx.x] = 0;
 }
 
 // Inclusive scan within a wavefront of 64 lanes.
@@ -233,7 +238,8 @@ static void radix_sort(Ctx& c) {
     {
       LaunchScope ls(&c, "sort_hist", 4.0 * n);
       k_sort_hist<<<n_tiles, kSortThreads, 0, c.stream>>>(c.keys[src].as<uint32_t>(), n, shift,
-                                                          n_tiles, c.sort_hist.as<uint32_t>());
+                                                          n_tiles, c.sort_hist.as<uint32_t>(),
+                                                          p == 0 ? c.inc_tiles.as<uint32_t>() : nullptr);
       PR_HIP(hipGetLastError());
     }
     {
@@ -268,9 +274,10 @@ static void radix_sort(Ctx& c) {
 //   moved entry b = (c, i): position b + t - P(t), t = lb_old[c] if i <
 //                           lb_old[c], else lb_old[c + 1]
 // (the kept slots before (c, i) are the slots before t that kept their key;
-// i never lies inside cell c's old range, whose slots all had key c).  Five
-// light launches replace the radix sort's nine; the permutation and keys are
-// exactly the stable sort's, as the radix sort would give them.
+// i never lies inside cell c's old range, whose slots all had key c).  The
+// step kernel that computed the keys also wrote the moved-slot mask; four
+// light launches replace the radix sort's nine, and the permutation and keys
+// are exactly the stable sort's, as the radix sort would give them.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kIncLds = 16384;      // moved entries one block sorts in LDS (128 KB)
 #ifndef PRESTIGE_BITONIC_MAX
@@ -279,41 +286,24 @@ constexpr uint32_t kIncLds = 16384;      // moved entries one block sorts in LDS
 constexpr uint32_t kIncBitonicMax = PRESTIGE_BITONIC_MAX;   // larger moved sets take the radix path
 constexpr int kIncSortThreads = 1024;
 
-// moved bit per slot (mask word w covers slots 64 w .. 64 w + 63) and the
-// number of moved slots of each kTile-slot tile
-__global__ __launch_bounds__(kSortThreads) void k_inc_flags(uint32_t n, const uint32_t* __restrict__ knew,
-                                                            const uint32_t* __restrict__ kold,
-                                                            uint64_t* __restrict__ mask,
-                                                            uint32_t* __restrict__ tile_cnt) {
-  __shared__ uint32_t s_cnt;
-  if (threadIdx.x == 0) s_cnt = 0;
-  __syncthreads();
-  const uint32_t base = blockIdx.x * (uint32_t)kTile;
-  uint32_t cnt = 0;
-  for (int it = 0; it < kSortItems; ++it) {
-    const uint32_t i = base + (uint32_t)it * kSortThreads + threadIdx.x;
-    const uint64_t b = __ballot(i < n && knew[i] != kold[i]);
-    // lane 0's slot is the word's first (slots of a wavefront are 64-aligned)
-    if (__lane_id() == 0 && i < n) {
-      mask[i >> 6] = b;
-      cnt += (uint32_t)__popcll(b);
-    }
-  }
-  if (__lane_id() == 0) atomicAdd(&s_cnt, cnt);
-  __syncthreads();
-  if (threadIdx.x == 0) tile_cnt[blockIdx.x] = s_cnt;
-}
-
 // one block: tile_off[t] = moved slots before tile t (t = 0 .. n_tiles; the
-// last entry is the total, also written to *count)
-__global__ __launch_bounds__(kSortThreads) void k_inc_scan(const uint32_t* __restrict__ tile_cnt,
-                                                           uint32_t n_tiles, uint32_t* __restrict__ tile_off,
-                                                           uint32_t* __restrict__ count) {
+// last entry is the total, also written to *count and to the host-mapped word
+// *host_count, which the host reads without waiting when it picks the next
+// build's sort).  The per-tile counts come from the step kernel that wrote
+// the keys and are cleared here for the next step.
+__global__ __launch_bounds__(kSortThreads) void k_inc_scan(uint32_t* __restrict__ tile_cnt, uint32_t n_tiles,
+                                                           uint32_t* __restrict__ tile_off,
+                                                           uint32_t* __restrict__ count,
+                                                           uint32_t* __restrict__ host_count) {
   __shared__ uint32_t s4[4];
   uint32_t carry = 0;
   for (uint32_t b = 0; b < n_tiles; b += kSortThreads) {
     const uint32_t idx = b + threadIdx.x;
-    const uint32_t v = idx < n_tiles ? tile_cnt[idx] : 0u;
+    uint32_t v = 0;
+    if (idx < n_tiles) {
+      v = tile_cnt[idx];
+      tile_cnt[idx] = 0;
+    }
     uint32_t total;
     const uint32_t ex = block_excl_scan(v, s4, &total);
     if (idx < n_tiles) tile_off[idx] = carry + ex;
@@ -322,6 +312,8 @@ __global__ __launch_bounds__(kSortThreads) void k_inc_scan(const uint32_t* __res
   if (threadIdx.x == 0) {
     tile_off[n_tiles] = carry;
     count[0] = carry;
+    *host_count = carry;
+    __threadfence_system();
   }
 }
 
@@ -513,25 +505,19 @@ static void incremental_sort(Ctx& c) {
   const uint32_t n = (uint32_t)c.n;
   const uint32_t n_tiles = (uint32_t)sort_tiles(n);
   const uint32_t* knew = c.keys[c.cur_key].as<uint32_t>();
-  uint32_t* kold = c.keys[c.cur_key ^ 1].as<uint32_t>();
-  uint64_t* mask = c.inc_mask.as<uint64_t>();
+  uint32_t* kold = c.keys[c.cur_key ^ 1].as<uint32_t>();   // overwritten with the sorted keys
+  const uint64_t* mask = c.inc_mask.as<uint64_t>();         // written by the step kernel
   uint32_t* cnt = c.inc_count.as<uint32_t>();
-  uint32_t* tiles = c.inc_tiles.as<uint32_t>();   // [0, n_tiles) counts, [n_tiles, 2 n_tiles + 1) offsets
+  uint32_t* tile_cnt = c.inc_tiles.as<uint32_t>();      // n_tiles, from the step kernel
+  uint32_t* tiles = tile_cnt + n_tiles;                  // tile offsets, n_tiles + 1
   {
-    LaunchScope ls(&c, "sort_inc_flags", 8.0 * n);
-    k_inc_flags<<<n_tiles, kSortThreads, 0, c.stream>>>(n, knew, kold, mask, tiles);
+    LaunchScope ls(&c, "sort_inc_scan", 12.0 * n_tiles);
+    k_inc_scan<<<1, kSortThreads, 0, c.stream>>>(tile_cnt, n_tiles, tiles, cnt, c.inc_host_dev);
     PR_HIP(hipGetLastError());
   }
-  {
-    LaunchScope ls(&c, "sort_inc_scan", 8.0 * n_tiles);
-    k_inc_scan<<<1, kSortThreads, 0, c.stream>>>(tiles, n_tiles, tiles + n_tiles, cnt);
-    PR_HIP(hipGetLastError());
-  }
-  // moved count for the host's choice at the next build (read without waiting)
-  PR_HIP(hipMemcpyAsync(c.inc_host, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
   {
     LaunchScope ls(&c, "sort_inc_gather", 4.0 * n);
-    k_inc_gather<<<n_tiles, kSortThreads, 0, c.stream>>>(n, knew, mask, tiles + n_tiles, c.inc_mov.as<uint64_t>());
+    k_inc_gather<<<n_tiles, kSortThreads, 0, c.stream>>>(n, knew, mask, tiles, c.inc_mov.as<uint64_t>());
     PR_HIP(hipGetLastError());
   }
   {
@@ -542,7 +528,7 @@ static void incremental_sort(Ctx& c) {
   }
   {
     LaunchScope ls(&c, "sort_inc_place", 12.0 * n);
-    k_inc_place<<<n_tiles, kSortThreads, 0, c.stream>>>(n, n_tiles, knew, mask, tiles + n_tiles,
+    k_inc_place<<<n_tiles, kSortThreads, 0, c.stream>>>(n, n_tiles, knew, mask, tiles,
                                                         c.inc_mov.as<uint64_t>(), cnt, c.lb.as<uint32_t>(), kold,
                                                         c.vals[0].as<uint32_t>());
     PR_HIP(hipGetLastError());
@@ -651,7 +637,7 @@ void neighbors_build(Ctx& c) {
   // was small; every few radix builds the moved count is measured again
   bool inc = false;
   if (c.inc_ready && c.sort_policy != Ctx::kSortRadix) {
-    const uint32_t seen = *c.inc_host;
+    const uint32_t seen = *(volatile const uint32_t*)c.inc_host;
     inc = c.sort_policy == Ctx::kSortIncremental || seen <= kIncLds / 2u || ++c.inc_skipped >= 16u;
   }
   if (inc) {

@@ -756,7 +756,7 @@ __global__ __launch_bounds__(kForceBlock) void k_sph_force(
     SphUnits u, const typename SphForm<F>::S4* __restrict__ spos,
     const typename SphForm<F>::S4* __restrict__ svel, typename SphForm<F>::S4* __restrict__ opos,
     typename SphForm<F>::S4* __restrict__ ovel, uint32_t* __restrict__ okeys,
-    typename SphForm<F>::S4* __restrict__ oacc, uint32_t* flags) {
+    typename SphForm<F>::S4* __restrict__ oacc, MovedOut moved, uint32_t* flags) {
   using real = typename SphForm<F>::real;
   using G4 = typename SphForm<F>::G4;
   using R2 = typename SphForm<F>::R2;
@@ -839,11 +839,13 @@ __global__ __launch_bounds__(kForceBlock) void k_sph_force(
       },
       [] {});
   if (i >= n) return;
+  const uint64_t live_mask = __ballot(live);
   if (!live) {
     opos[i] = spos[i];
     ovel[i] = svel[i];
     okeys[i] = skeys[i];
     if (oacc) oacc[i] = S4{0, 0, 0, 0};
+    moved_word_fixed(moved, i, live_mask);
     return;
   }
   const S4 xs = spos[i];
@@ -883,6 +885,7 @@ __global__ __launch_bounds__(kForceBlock) void k_sph_force(
   else
     atomicOr(&flags[FLAG_DOMAIN], 1u);
   okeys[i] = key;
+  moved_word_live(moved, i, key != skeys[i], live_mask);
 }
 
 template <int F>
@@ -969,7 +972,7 @@ void sph_step_impl(Ctx& c) {
         n, c.n_moving, gpos, gvel, c.rp.as<R2>(), c.ids[c.cur_ids].as<uint32_t>(), skeys,
         c.nmeta.as<uint2>(), c.nword.as<uint32_t>(), c.grid, c.sph, u, c.pos[c.cur].as<S4>(), c.vel[c.cur].as<S4>(),
         c.pos[dst].as<S4>(), c.vel[dst].as<S4>(), c.keys[kdst].as<uint32_t>(),
-        c.keep_acc ? c.acc.as<S4>() : nullptr, c.flags.as<uint32_t>());
+        c.keep_acc ? c.acc.as<S4>() : nullptr, MovedOut{c.inc_mask.as<uint64_t>(), c.inc_tiles.as<uint32_t>()}, c.flags.as<uint32_t>());
     PR_HIP(hipGetLastError());
     c.cur = dst;
     c.cur_key = kdst;
