@@ -94,6 +94,29 @@ def test_sort_policies_over_steps(monkeypatch, policy, speed, n):
         assert min(moved) > 16384
 
 
+@pytest.mark.parametrize("n", [1, 7, 63, 64, 65, 200])
+def test_small_and_ragged_counts_over_steps(n):
+    """Counts below, at and just past one wavefront: the step kernels' moved
+    mask (one word per wavefront, partial last word), the incremental sort
+    and the state stay identical to the oracle."""
+    d, arrs = make_scene(pa.SCENE_DEM_LATTICE, n_target=n, seed=4)
+    rng = np.random.default_rng(n)
+    for k in ("vx", "vy", "vz"):
+        arrs[k] = rng.uniform(-2.0, 2.0, d.n_particles)
+    d.dt = 1e-4
+    st = oracle_state(d, arrs)
+    with pa.Context(d) as ctx:
+        upload_all(ctx, arrs)
+        for _ in range(3):
+            assert st.step(10) == 0
+            ctx.step(10)
+            assert st.build() == 0
+            ctx.neighbors_build()
+            _compare_build(ctx, st)
+        for k in ("x", "y", "z", "vx", "vy", "vz"):
+            assert np.array_equal(ctx.download(k), st.get(k)), k
+
+
 def test_eq1_allpairs_bitexact():
     rng = np.random.default_rng(0)
     d, arrs = make_scene(pa.SCENE_DEM_LATTICE, n_target=3000, seed=1)
