@@ -1,5 +1,6 @@
-// neighbors.hip — cell keys, stable LSD radix sort, reorder into key order,
-// cell lower-bound table, id-order scatter/gather, and the eq1 probe.
+// neighbors.hip — cell keys, stable LSD radix sort, incremental stable sort
+// between steps, reorder into key order, cell lower-bound table, id-order
+// scatter/gather, and the eq1 probe.
 //
 // The sorted layout is what every gather kernel iterates: for a particle in
 // cell (cx, cy, cz) the three cells cx-1..cx+1 of each of the nine (dy, dz)

@@ -8,22 +8,24 @@
 //
 // Two passes per step, split where the fused bodies would read rho[j] and
 // p[j] that the density body writes; both run one wavefront per block and
-// visit the nine row runs of a particle's cell in lockstep:
+// visit the nine row runs of a particle's cell in lockstep, each run followed
+// by the opposite row (sph_run):
 //   density  (a) scan the run's candidates in FP32 (offsets from a point of
-//            the wavefront, staged in LDS once per wavefront and run),
-//            marking every candidate inside the cutoff plus a small relative
-//            margin in 32-bit words, which are stored with the run's
-//            (first slot, count) for the force pass; (b) walk the set bits in
-//            the form's own precision, keep the exact r < 2h pairs and sum the
-//            density.  Then the EOS.
-//   force    stage the gather data of the wavefront's hits of the run in LDS
-//            and walk the stored masks: momentum + viscosity, then
-//            symplectic Euler and the next step's cell key.
+//            the wavefront with |q|^2, staged in LDS once per wavefront and
+//            run), marking every candidate inside the cutoff plus a margin
+//            in 32-bit words, which are stored with the run's (first slot,
+//            count) for the force pass; (b) walk the set bits in the form's
+//            own precision, keep the exact r < 2h pairs and sum the density.
+//            Then the EOS.
+//   force    stage the gather data of the wavefront's hits of a pair of
+//            opposite runs in LDS and walk the stored masks: momentum +
+//            viscosity, then symplectic Euler, the next step's cell key and
+//            the moved-slot mask of the next incremental sort.
 // Scanning in FP32 halves the bytes each candidate costs, and the expensive
-// pair bodies only run over the set bits.  Runs in order and bits in
-// ascending order are the scan order, so FP64 sums are formed in exactly the
-// order of the host oracle; the stored masks are also the frozen neighbour
-// sets the adjoint reuses.
+// pair bodies only run over the set bits.  Runs in the order of sph_run and
+// bits in ascending order are the visiting order of the host oracle, so FP64
+// sums are formed in exactly its order; the stored masks are also the frozen
+// neighbour sets the adjoint reuses.
 //
 // Forms:
 //   kF64   state and pair arithmetic in FP64, positions absolute.  The
