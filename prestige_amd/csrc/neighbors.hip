@@ -265,6 +265,100 @@ static void radix_sort(Ctx& c) {
 }
 
 // ---------------------------------------------------------------------------
+// reorder into key order.  PROBE also writes the FP32 probe (offset from the
+// FP64 cell origin times off_scale, cell-coordinate residues mod 4 in .w);
+// MIXED also writes FP32 copies of the velocity (mass in .w) and, with OMEGA,
+// of the angular velocity (radius in .w).
+// ---------------------------------------------------------------------------
+
+// Gather-side copies of one particle moved from slot s to slot k (cell key
+// of slot k: key): state, id, and for the mixed form the FP32 probe (offset
+// from the FP64 origin of the cell, cell code in .w) and velocity copies.
+template <class T4>
+struct ReorderIO {
+  const T4 *pos_in, *vel_in, *om_in;
+  const uint32_t* id_in;
+  T4 *pos_out, *vel_out, *om_out;
+  uint32_t* id_out;
+  Grid g;
+  float4 *off32, *vel32, *om32;
+};
+
+template <class T4, bool OMEGA, bool PROBE, bool MIXED>
+__device__ inline void move_slot(const ReorderIO<T4>& io, uint32_t s, uint32_t k, uint32_t key) {
+  const T4 p = io.pos_in[s];
+  const T4 v = io.vel_in[s];
+  io.pos_out[k] = p;
+  io.vel_out[k] = v;
+  io.id_out[k] = io.id_in[s];
+  if constexpr (OMEGA) {
+    const T4 w = io.om_in[s];
+    io.om_out[k] = w;
+    if constexpr (MIXED) io.om32[k] = make_float4((float)w.x, (float)w.y, (float)w.z, (float)p.w);
+  }
+  if constexpr (PROBE) {
+    const Grid& g = io.g;
+    const uint32_t cx = key % (uint32_t)g.dims[0];
+    const uint32_t rest = key / (uint32_t)g.dims[0];
+    const uint32_t cy = rest % (uint32_t)g.dims[1];
+    const uint32_t cz = rest / (uint32_t)g.dims[1];
+    const float ox = (float)(((double)p.x - (g.lo[0] + (double)cx * g.cs)) * g.off_scale);
+    const float oy = (float)(((double)p.y - (g.lo[1] + (double)cy * g.cs)) * g.off_scale);
+    const float oz = (float)(((double)p.z - (g.lo[2] + (double)cz * g.cs)) * g.off_scale);
+    io.off32[k] = make_float4(ox, oy, oz, __uint_as_float(cell_code(cx, cy, cz)));
+  }
+  if constexpr (MIXED) io.vel32[k] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
+}
+
+template <class T4, bool OMEGA, bool PROBE, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_reorder(uint32_t n, const uint32_t* __restrict__ perm,
+                                                    const uint32_t* __restrict__ skeys, ReorderIO<T4> io) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) move_slot<T4, OMEGA, PROBE, MIXED>(io, perm[k], k, skeys[k]);
+}
+
+// The copies of one build from buffer src to dst, as the context's model and
+// precision need them.
+template <class T4, bool OMEGA, bool MIXED>
+ReorderIO<T4> reorder_io(Ctx& c, int src, int dst) {
+  return ReorderIO<T4>{c.pos[src].as<T4>(), c.vel[src].as<T4>(), OMEGA ? c.omega[src].as<T4>() : nullptr,
+                       c.ids[c.cur_ids].as<uint32_t>(), c.pos[dst].as<T4>(), c.vel[dst].as<T4>(),
+                       OMEGA ? c.omega[dst].as<T4>() : nullptr, c.ids[c.cur_ids ^ 1].as<uint32_t>(), c.grid,
+                       MIXED ? c.off32.as<float4>() : nullptr, MIXED ? c.vel32.as<float4>() : nullptr,
+                       (MIXED && OMEGA) ? c.om32.as<float4>() : nullptr};
+}
+
+// Calls fn.template operator()<T4, OMEGA, PROBE, MIXED>(io) for the context's
+// form (the mixed forms build the FP32 probe; DEM moves omega).
+template <class Fn>
+void with_reorder_form(Ctx& c, int src, int dst, Fn&& fn) {
+  const bool dem = c.desc.model == PRESTIGE_MODEL_DEM;
+  const bool sph = c.desc.model == PRESTIGE_MODEL_WCSPH;
+  const bool mixed = c.desc.precision == PRESTIGE_MIXED;
+  const bool f32 = c.desc.precision == PRESTIGE_F32;
+  if (dem) {
+    if (mixed) fn.template operator()<double4, true, true, true>(reorder_io<double4, true, true>(c, src, dst));
+    else fn.template operator()<double4, true, false, false>(reorder_io<double4, true, false>(c, src, dst));
+  } else if (f32) {
+    fn.template operator()<float4, false, false, false>(reorder_io<float4, false, false>(c, src, dst));
+  } else if (sph && mixed) {
+    fn.template operator()<double4, false, true, true>(reorder_io<double4, false, true>(c, src, dst));
+  } else {
+    fn.template operator()<double4, false, false, false>(reorder_io<double4, false, false>(c, src, dst));
+  }
+}
+
+// Bytes one build moves per particle in the reorder.
+double reorder_bytes(const Ctx& c) {
+  const bool dem = c.desc.model == PRESTIGE_MODEL_DEM;
+  const bool mixed = c.desc.precision == PRESTIGE_MIXED;
+  const bool f32 = c.desc.precision == PRESTIGE_F32;
+  const double elem = f32 ? 12.0 : 24.0;   // three coordinates per vector
+  return 4.0 + 2.0 * (2.0 * elem + 4.0) + (dem ? 2.0 * elem + 16.0 : 0.0) + (mixed ? 4.0 + 16.0 + 16.0 : 0.0) +
+         (mixed && dem ? 16.0 : 0.0);
+}
+
+// ---------------------------------------------------------------------------
 // incremental stable sort.  Between two builds only the slots whose particle
 // changed cell ("moved", key_new != key_old) break the order: slots are in
 // key_old order, so the slots that kept their key are still sorted and keep
@@ -318,34 +412,49 @@ __global__ __launch_bounds__(kSortThreads) void k_inc_scan(uint32_t* __restrict_
   }
 }
 
+// Gather and placement walk a sort tile with kIncThreads threads, kIncItems
+// slots each: more wavefronts in flight to cover the placement's dependent
+// loads (binary search, then the state copy).
+constexpr int kIncThreads = 1024;
+constexpr int kIncItems = kTile / kIncThreads;
+static_assert(kIncItems * kIncThreads == kTile, "whole tiles");
+
 // Moved slots of a tile in slot order, as (key << 32 | slot), from
 // tile_off[tile]; visit(i, rank_before_i, moved) sees every slot of the tile
-// with the number of moved slots before it.  Every thread of the block calls.
+// with the number of moved slots before it.  All mask words of the tile are
+// read up front, so one barrier serves the whole tile.  Every thread of the
+// block calls.
 template <class Visit>
 __device__ inline void inc_tile_walk(uint32_t n, const uint64_t* __restrict__ mask, uint32_t run,
                                      Visit&& visit) {
-  __shared__ uint32_t s_w[kSortThreads / 64];
+  constexpr uint32_t kWaves = kIncThreads / 64;
+  __shared__ uint32_t s_w[kIncItems * kWaves];
   const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
   const uint32_t base = blockIdx.x * (uint32_t)kTile;
-  for (int it = 0; it < kSortItems; ++it) {
-    const uint32_t i0 = base + (uint32_t)it * kSortThreads + 64u * w;   // the wavefront's first slot
-    const uint32_t i = i0 + lane;
-    const uint64_t word = i0 < n ? mask[i0 >> 6] : 0ull;
-    if (lane == 0) s_w[w] = (uint32_t)__popcll(word);
-    __syncthreads();
-    uint32_t before = run, total = 0;
+  uint64_t word[kIncItems];
 #pragma unroll
-    for (uint32_t k = 0; k < kSortThreads / 64; ++k) {
-      before += k < w ? s_w[k] : 0u;
-      total += s_w[k];
+  for (int it = 0; it < kIncItems; ++it) {
+    const uint32_t i0 = base + (uint32_t)it * kIncThreads + 64u * w;   // the wavefront's first slot
+    word[it] = i0 < n ? mask[i0 >> 6] : 0ull;
+    if (lane == 0) s_w[it * kWaves + w] = (uint32_t)__popcll(word[it]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < kIncItems; ++it) {
+    uint32_t before = run, total = 0;
+    for (uint32_t k = 0; k < kWaves; ++k) {
+      const uint32_t cnt = s_w[it * kWaves + k];
+      before += k < w ? cnt : 0u;
+      total += cnt;
     }
-    if (i < n) visit(i, before + (uint32_t)__popcll(word & lanes_below()), ((word >> lane) & 1ull) != 0ull);
+    const uint32_t i = base + (uint32_t)it * kIncThreads + 64u * w + lane;
+    if (i < n)
+      visit(i, before + (uint32_t)__popcll(word[it] & lanes_below()), ((word[it] >> lane) & 1ull) != 0ull);
     run += total;
-    __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_inc_gather(uint32_t n, const uint32_t* __restrict__ knew,
+__global__ __launch_bounds__(kIncThreads) void k_inc_gather(uint32_t n, const uint32_t* __restrict__ knew,
                                                              const uint64_t* __restrict__ mask,
                                                              const uint32_t* __restrict__ tile_off,
                                                              uint64_t* __restrict__ mov) {
@@ -469,10 +578,12 @@ __device__ inline uint32_t inc_moved_before(uint32_t t, const uint64_t* __restri
   return c;
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_inc_place(
+template <class T4, bool OMEGA, bool PROBE, bool MIXED>
+__global__ __launch_bounds__(kIncThreads) void k_inc_place(
     uint32_t n, uint32_t n_tiles, const uint32_t* __restrict__ knew, const uint64_t* __restrict__ mask,
     const uint32_t* __restrict__ tile_off, const uint64_t* __restrict__ mov, const uint32_t* __restrict__ count,
-    const uint32_t* __restrict__ lb_old, uint32_t* __restrict__ kout, uint32_t* __restrict__ perm) {
+    const uint32_t* __restrict__ lb_old, uint32_t* __restrict__ kout, uint32_t* __restrict__ perm,
+    ReorderIO<T4> io) {
   const uint32_t m = count[0];
   if (blockIdx.x < n_tiles) {
     inc_tile_walk(n, mask, tile_off[blockIdx.x], [&](uint32_t i, uint32_t before, bool moved) {
@@ -488,15 +599,17 @@ __global__ __launch_bounds__(kSortThreads) void k_inc_place(
       const uint32_t pos = i - before + lo;
       kout[pos] = key;
       perm[pos] = i;
+      move_slot<T4, OMEGA, PROBE, MIXED>(io, i, pos, key);
     });
   }
-  for (uint32_t b = blockIdx.x * (uint32_t)kSortThreads + threadIdx.x; b < m; b += gridDim.x * (uint32_t)kSortThreads) {
+  for (uint32_t b = blockIdx.x * (uint32_t)kIncThreads + threadIdx.x; b < m; b += gridDim.x * (uint32_t)kIncThreads) {
     const uint64_t e = mov[b];
     const uint32_t key = (uint32_t)(e >> 32), i = (uint32_t)e;
     const uint32_t t = i < lb_old[key] ? lb_old[key] : lb_old[key + 1];
     const uint32_t pos = b + t - inc_moved_before(t, mask, tile_off);
     kout[pos] = key;
     perm[pos] = i;
+    move_slot<T4, OMEGA, PROBE, MIXED>(io, i, pos, key);
   }
 }
 
@@ -518,7 +631,7 @@ static void incremental_sort(Ctx& c) {
   }
   {
     LaunchScope ls(&c, "sort_inc_gather", 4.0 * n);
-    k_inc_gather<<<n_tiles, kSortThreads, 0, c.stream>>>(n, knew, mask, tiles, c.inc_mov.as<uint64_t>());
+    k_inc_gather<<<n_tiles, kIncThreads, 0, c.stream>>>(n, knew, mask, tiles, c.inc_mov.as<uint64_t>());
     PR_HIP(hipGetLastError());
   }
   {
@@ -528,67 +641,20 @@ static void incremental_sort(Ctx& c) {
     PR_HIP(hipGetLastError());
   }
   {
-    LaunchScope ls(&c, "sort_inc_place", 12.0 * n);
-    k_inc_place<<<n_tiles, kSortThreads, 0, c.stream>>>(n, n_tiles, knew, mask, tiles,
-                                                        c.inc_mov.as<uint64_t>(), cnt, c.lb.as<uint32_t>(), kold,
-                                                        c.vals[0].as<uint32_t>());
+    // placement and the reorder in one pass: each slot's state goes straight
+    // to its sorted position
+    LaunchScope ls(&c, "sort_inc_place", (12.0 + reorder_bytes(c)) * n);
+    const uint64_t* mov = c.inc_mov.as<uint64_t>();
+    const uint32_t* lb_old = c.lb.as<uint32_t>();
+    uint32_t* perm = c.vals[0].as<uint32_t>();
+    with_reorder_form(c, c.cur, c.cur ^ 1, [&]<class T4, bool OMEGA, bool PROBE, bool MIXED>(ReorderIO<T4> io) {
+      k_inc_place<T4, OMEGA, PROBE, MIXED><<<n_tiles, kIncThreads, 0, c.stream>>>(
+          n, n_tiles, knew, mask, tiles, mov, cnt, lb_old, kold, perm, io);
+    });
     PR_HIP(hipGetLastError());
   }
   c.cur_key ^= 1;
   c.perm_buf = 0;
-}
-
-// ---------------------------------------------------------------------------
-// reorder into key order.  PROBE also writes the FP32 probe (offset from the
-// FP64 cell origin times off_scale, cell-coordinate residues mod 4 in .w);
-// MIXED also writes FP32 copies of the velocity (mass in .w) and, with OMEGA,
-// of the angular velocity (radius in .w).
-// ---------------------------------------------------------------------------
-
-template <class T4, bool OMEGA, bool PROBE, bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_reorder(
-    uint32_t n, const uint32_t* __restrict__ perm, const T4* __restrict__ pos_in,
-    const T4* __restrict__ vel_in, const T4* __restrict__ om_in,
-    const uint32_t* __restrict__ id_in, T4* __restrict__ pos_out, T4* __restrict__ vel_out,
-    T4* __restrict__ om_out, uint32_t* __restrict__ id_out, const uint32_t* __restrict__ skeys,
-    Grid g, float4* __restrict__ off32, float4* __restrict__ vel32, float4* __restrict__ om32) {
-  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  uint32_t s = perm[k];
-  T4 p = pos_in[s];
-  T4 v = vel_in[s];
-  pos_out[k] = p;
-  vel_out[k] = v;
-  id_out[k] = id_in[s];
-  if constexpr (OMEGA) {
-    T4 w = om_in[s];
-    om_out[k] = w;
-    if constexpr (MIXED) om32[k] = make_float4((float)w.x, (float)w.y, (float)w.z, (float)p.w);
-  }
-  if constexpr (PROBE) {
-    uint32_t key = skeys[k];
-    uint32_t cx = key % (uint32_t)g.dims[0];
-    uint32_t rest = key / (uint32_t)g.dims[0];
-    uint32_t cy = rest % (uint32_t)g.dims[1];
-    uint32_t cz = rest / (uint32_t)g.dims[1];
-    float ox = (float)(((double)p.x - (g.lo[0] + (double)cx * g.cs)) * g.off_scale);
-    float oy = (float)(((double)p.y - (g.lo[1] + (double)cy * g.cs)) * g.off_scale);
-    float oz = (float)(((double)p.z - (g.lo[2] + (double)cz * g.cs)) * g.off_scale);
-    off32[k] = make_float4(ox, oy, oz, __uint_as_float(cell_code(cx, cy, cz)));
-  }
-  if constexpr (MIXED) vel32[k] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
-}
-
-template <class T4, bool OMEGA, bool PROBE, bool MIXED>
-static void launch_reorder(Ctx& c, const uint32_t* perm, const uint32_t* skeys, int src, int dst) {
-  const uint32_t n = (uint32_t)c.n;
-  k_reorder<T4, OMEGA, PROBE, MIXED><<<blocks_for(n), kBlock, 0, c.stream>>>(
-      n, perm, c.pos[src].as<T4>(), c.vel[src].as<T4>(), OMEGA ? c.omega[src].as<T4>() : nullptr,
-      c.ids[c.cur_ids].as<uint32_t>(), c.pos[dst].as<T4>(), c.vel[dst].as<T4>(),
-      OMEGA ? c.omega[dst].as<T4>() : nullptr, c.ids[c.cur_ids ^ 1].as<uint32_t>(), skeys, c.grid,
-      PROBE ? c.off32.as<float4>() : nullptr, MIXED ? c.vel32.as<float4>() : nullptr,
-      (MIXED && OMEGA) ? c.om32.as<float4>() : nullptr);
-  PR_HIP(hipGetLastError());
 }
 
 // lb[c] = first slot with key >= c, for c = 0 .. n_cells.  Thread k = 0 .. n
@@ -648,30 +714,15 @@ void neighbors_build(Ctx& c) {
     radix_sort(c);
   }
   c.inc_ready = false;
-  const uint32_t* perm = c.vals[c.perm_buf].as<uint32_t>();
   const uint32_t* skeys = c.keys[c.cur_key].as<uint32_t>();
   const int src = c.cur, dst = c.cur ^ 1;
-  const bool dem = c.desc.model == PRESTIGE_MODEL_DEM;
-  const bool sph = c.desc.model == PRESTIGE_MODEL_WCSPH;
-  const bool mixed = c.desc.precision == PRESTIGE_MIXED;
-  const bool f32 = c.desc.precision == PRESTIGE_F32;
-  {
-    const double elem = f32 ? 12.0 : 24.0;   // three coordinates per vector
-    const bool probe = mixed;
-    double bytes = 4.0 + 2.0 * (2.0 * elem + 4.0) + (dem ? 2.0 * elem + 16.0 : 0.0) + 4.0 * (probe ? 1.0 : 0.0) +
-                   (probe ? 16.0 : 0.0) + (mixed ? 16.0 : 0.0) + (mixed && dem ? 16.0 : 0.0);
-    LaunchScope ls(&c, "reorder", bytes * n);
-    if (sph) {
-      if (f32) launch_reorder<float4, false, false, false>(c, perm, skeys, src, dst);
-      else if (mixed) launch_reorder<double4, false, true, true>(c, perm, skeys, src, dst);
-      else launch_reorder<double4, false, false, false>(c, perm, skeys, src, dst);
-    } else if (dem) {
-      if (mixed) launch_reorder<double4, true, true, true>(c, perm, skeys, src, dst);
-      else launch_reorder<double4, true, false, false>(c, perm, skeys, src, dst);
-    } else {
-      if (f32) launch_reorder<float4, false, false, false>(c, perm, skeys, src, dst);
-      else launch_reorder<double4, false, false, false>(c, perm, skeys, src, dst);
-    }
+  if (!inc) {   // the incremental sort moved the state itself
+    const uint32_t* perm = c.vals[c.perm_buf].as<uint32_t>();
+    LaunchScope ls(&c, "reorder", reorder_bytes(c) * n);
+    with_reorder_form(c, src, dst, [&]<class T4, bool OMEGA, bool PROBE, bool MIXED>(ReorderIO<T4> io) {
+      k_reorder<T4, OMEGA, PROBE, MIXED><<<blocks_for(n), kBlock, 0, c.stream>>>(n, perm, skeys, io);
+    });
+    PR_HIP(hipGetLastError());
   }
   c.cur = dst;
   c.cur_ids ^= 1;
