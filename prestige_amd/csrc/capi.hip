@@ -197,6 +197,7 @@ static void allocate(Ctx& c) {
   c.sort_hist.alloc(256 * tiles * 4);
   c.sort_tot.alloc(256 * 4);
   c.inc_mask.alloc(((nn + 63) / 64) * 8);
+  c.inc_woff.alloc(((nn + 63) / 64) * 4);
   c.inc_tiles.alloc((2 * tiles + 1) * 4);
   PR_HIP(hipMemsetAsync(c.inc_tiles.p, 0, (2 * tiles + 1) * 4, c.stream));
   c.inc_count.alloc(4);

@@ -370,9 +370,11 @@ double reorder_bytes(const Ctx& c) {
 //                           lb_old[c], else lb_old[c + 1]
 // (the kept slots before (c, i) are the slots before t that kept their key;
 // i never lies inside cell c's old range, whose slots all had key c).  The
-// step kernel that computed the keys also wrote the moved-slot mask; four
-// light launches replace the radix sort's nine, and the permutation and keys
-// are exactly the stable sort's, as the radix sort would give them.
+// step kernel that computed the keys also wrote the moved-slot mask and the
+// per-tile moved counts; three launches (gather, sort of M, placement, which
+// also moves the particle state) replace the radix sort's nine and the
+// reorder, and the permutation and keys are exactly the stable sort's, as
+// the radix sort would give them.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kIncLds = 16384;      // moved entries one block sorts in LDS (128 KB)
 #ifndef PRESTIGE_BITONIC_MAX
@@ -381,54 +383,29 @@ constexpr uint32_t kIncLds = 16384;      // moved entries one block sorts in LDS
 constexpr uint32_t kIncBitonicMax = PRESTIGE_BITONIC_MAX;   // larger moved sets take the radix path
 constexpr int kIncSortThreads = 1024;
 
-// one block: tile_off[t] = moved slots before tile t (t = 0 .. n_tiles; the
-// last entry is the total, also written to *count and to the host-mapped word
-// *host_count, which the host reads without waiting when it picks the next
-// build's sort).  The per-tile counts come from the step kernel that wrote
-// the keys and are cleared here for the next step.
-__global__ __launch_bounds__(kSortThreads) void k_inc_scan(uint32_t* __restrict__ tile_cnt, uint32_t n_tiles,
-                                                           uint32_t* __restrict__ tile_off,
-                                                           uint32_t* __restrict__ count,
-                                                           uint32_t* __restrict__ host_count) {
-  __shared__ uint32_t s4[4];
-  uint32_t carry = 0;
-  for (uint32_t b = 0; b < n_tiles; b += kSortThreads) {
-    const uint32_t idx = b + threadIdx.x;
-    uint32_t v = 0;
-    if (idx < n_tiles) {
-      v = tile_cnt[idx];
-      tile_cnt[idx] = 0;
-    }
-    uint32_t total;
-    const uint32_t ex = block_excl_scan(v, s4, &total);
-    if (idx < n_tiles) tile_off[idx] = carry + ex;
-    carry += total;
-  }
-  if (threadIdx.x == 0) {
-    tile_off[n_tiles] = carry;
-    count[0] = carry;
-    *host_count = carry;
-    __threadfence_system();
-  }
-}
-
 // Gather and placement walk a sort tile with kIncThreads threads, kIncItems
 // slots each: more wavefronts in flight to cover the placement's dependent
-// loads (binary search, then the state copy).
+// loads (search, then the state copy).
 constexpr int kIncThreads = 1024;
 constexpr int kIncItems = kTile / kIncThreads;
+constexpr uint32_t kIncWaves = kIncThreads / 64;
 static_assert(kIncItems * kIncThreads == kTile, "whole tiles");
 
-// Moved slots of a tile in slot order, as (key << 32 | slot), from
-// tile_off[tile]; visit(i, rank_before_i, moved) sees every slot of the tile
-// with the number of moved slots before it.  All mask words of the tile are
-// read up front, so one barrier serves the whole tile.  Every thread of the
-// block calls.
+__device__ inline uint32_t wave_sum(uint32_t x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+// Moved slots of a tile in slot order, as (key << 32 | slot), from run (the
+// moved slots before the tile); visit(i, rank_before_i, moved, valid) sees
+// every slot of the tile with the number of moved slots before it, called by
+// every lane of every wavefront (valid: i < n).  All mask words of the tile
+// are read up front, so one barrier serves the whole tile.
 template <class Visit>
 __device__ inline void inc_tile_walk(uint32_t n, const uint64_t* __restrict__ mask, uint32_t run,
                                      Visit&& visit) {
-  constexpr uint32_t kWaves = kIncThreads / 64;
-  __shared__ uint32_t s_w[kIncItems * kWaves];
+  __shared__ uint32_t s_w[kIncItems * kIncWaves];
   const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
   const uint32_t base = blockIdx.x * (uint32_t)kTile;
   uint64_t word[kIncItems];
@@ -436,30 +413,66 @@ __device__ inline void inc_tile_walk(uint32_t n, const uint64_t* __restrict__ ma
   for (int it = 0; it < kIncItems; ++it) {
     const uint32_t i0 = base + (uint32_t)it * kIncThreads + 64u * w;   // the wavefront's first slot
     word[it] = i0 < n ? mask[i0 >> 6] : 0ull;
-    if (lane == 0) s_w[it * kWaves + w] = (uint32_t)__popcll(word[it]);
+    if (lane == 0) s_w[it * kIncWaves + w] = (uint32_t)__popcll(word[it]);
   }
   __syncthreads();
 #pragma unroll
   for (int it = 0; it < kIncItems; ++it) {
     uint32_t before = run, total = 0;
-    for (uint32_t k = 0; k < kWaves; ++k) {
-      const uint32_t cnt = s_w[it * kWaves + k];
+    for (uint32_t k = 0; k < kIncWaves; ++k) {
+      const uint32_t cnt = s_w[it * kIncWaves + k];
       before += k < w ? cnt : 0u;
       total += cnt;
     }
     const uint32_t i = base + (uint32_t)it * kIncThreads + 64u * w + lane;
-    if (i < n)
-      visit(i, before + (uint32_t)__popcll(word[it] & lanes_below()), ((word[it] >> lane) & 1ull) != 0ull);
+    visit(i, before + (uint32_t)__popcll(word[it] & lanes_below()), ((word[it] >> lane) & 1ull) != 0ull, i < n);
     run += total;
   }
 }
 
-__global__ __launch_bounds__(kIncThreads) void k_inc_gather(uint32_t n, const uint32_t* __restrict__ knew,
-                                                             const uint64_t* __restrict__ mask,
-                                                             const uint32_t* __restrict__ tile_off,
-                                                             uint64_t* __restrict__ mov) {
-  inc_tile_walk(n, mask, tile_off[blockIdx.x], [&](uint32_t i, uint32_t before, bool moved) {
-    if (moved) mov[before] = ((uint64_t)knew[i] << 32) | i;
+// Block b of n_tiles: sums the per-tile moved counts (from the step kernel
+// that wrote the keys) of the tiles before b, writes that to tile_off[b] and
+// the count before each mask word to word_off, and lists its moved slots at
+// their rank.  Block 0 writes the total to tile_off[n_tiles], *count and the
+// host-mapped word *host_count, which the host reads without waiting when it
+// picks the next build's sort.  The placement clears the tile counts.
+__global__ __launch_bounds__(kIncThreads) void k_inc_gather(
+    uint32_t n, uint32_t n_tiles, const uint32_t* __restrict__ tile_cnt, const uint32_t* __restrict__ knew,
+    const uint64_t* __restrict__ mask, uint32_t* __restrict__ tile_off, uint32_t* __restrict__ word_off,
+    uint32_t* __restrict__ count, uint32_t* __restrict__ host_count, uint64_t* __restrict__ mov) {
+  __shared__ uint32_t s_red[2 * kIncWaves];
+  uint32_t pre = 0, tot = 0;
+  for (uint32_t t = threadIdx.x; t < n_tiles; t += kIncThreads) {
+    const uint32_t v = tile_cnt[t];
+    tot += v;
+    pre += t < blockIdx.x ? v : 0u;
+  }
+  pre = wave_sum(pre);
+  tot = wave_sum(tot);
+  const uint32_t w = threadIdx.x >> 6;
+  if (__lane_id() == 0) {
+    s_red[w] = pre;
+    s_red[kIncWaves + w] = tot;
+  }
+  __syncthreads();
+  pre = 0;
+  tot = 0;
+  for (uint32_t k = 0; k < kIncWaves; ++k) {
+    pre += s_red[k];
+    tot += s_red[kIncWaves + k];
+  }
+  if (threadIdx.x == 0) {
+    tile_off[blockIdx.x] = pre;
+    if (blockIdx.x == 0) {
+      tile_off[n_tiles] = tot;
+      count[0] = tot;
+      *host_count = tot;
+      __threadfence_system();
+    }
+  }
+  inc_tile_walk(n, mask, pre, [&](uint32_t i, uint32_t before, bool moved, bool valid) {
+    if (valid && (i & 63u) == 0u) word_off[i >> 6] = before;
+    if (valid && moved) mov[before] = ((uint64_t)knew[i] << 32) | i;
   });
 }
 
@@ -567,46 +580,85 @@ __global__ __launch_bounds__(kIncSortThreads) void k_inc_sort(const uint32_t* __
     for (uint32_t k = t; k < m; k += kIncSortThreads) mov[k] = src[k];
 }
 
-// Moved slots before slot t (t <= n).
-__device__ inline uint32_t inc_moved_before(uint32_t t, const uint64_t* __restrict__ mask,
-                                            const uint32_t* __restrict__ tile_off) {
-  const uint32_t tile = t / (uint32_t)kTile;
-  uint32_t c = tile_off[tile];
-  const uint32_t w1 = t >> 6;
-  for (uint32_t w = tile * ((uint32_t)kTile / 64u); w < w1; ++w) c += (uint32_t)__popcll(mask[w]);
-  if (t & 63u) c += (uint32_t)__popcll(mask[w1] & ((1ull << (t & 63u)) - 1ull));
-  return c;
+// Moved slots before slot t (t <= n; m = all moved slots).
+__device__ inline uint32_t inc_moved_before(uint32_t t, uint32_t n, uint32_t m, const uint64_t* __restrict__ mask,
+                                            const uint32_t* __restrict__ word_off) {
+  if (t >= n) return m;
+  const uint32_t r = t & 63u;
+  return word_off[t >> 6] + (r ? (uint32_t)__popcll(mask[t >> 6] & ((1ull << r) - 1ull)) : 0u);
+}
+
+// Lower bounds in the sorted moved entries mov[0 .. m) of ea (returned in .x)
+// and eb >= ea (in .y), one per half-wavefront: each step probes 32 evenly
+// spaced entries of the remaining range and keeps the gap that holds the
+// bound, so m = 32k entries take three dependent loads.  Every lane of the
+// wavefront calls.
+__device__ inline uint2 wave_lower_bounds(const uint64_t* __restrict__ mov, uint32_t m, uint64_t ea, uint64_t eb) {
+  const uint32_t lane = __lane_id();
+  const bool upper = lane >= 32u;
+  const uint64_t e = upper ? eb : ea;
+  const uint32_t j = lane & 31u;
+  uint32_t lo = 0, hi = m;   // the bound lies in [lo, hi]
+  while (__ballot(lo < hi)) {
+    const uint32_t chunk = (hi - lo + 31u) >> 5;   // 0 when this half is done
+    const uint32_t idx = lo + j * chunk;
+    const bool below = chunk != 0u && idx < hi && mov[idx] < e;
+    const uint64_t bal = __ballot(below);
+    const uint32_t c = (uint32_t)__popcll(upper ? (bal >> 32) : (bal & 0xffffffffull));
+    if (chunk != 0u) {
+      if (c == 0u) {
+        hi = lo;   // mov[lo] >= e
+      } else {
+        const uint32_t nlo = lo + (c - 1u) * chunk + 1u;
+        hi = min(lo + c * chunk, hi);
+        lo = nlo;
+      }
+    }
+  }
+  return make_uint2((uint32_t)__builtin_amdgcn_readlane((int)lo, 0),
+                    (uint32_t)__builtin_amdgcn_readlane((int)lo, 32));
 }
 
 template <class T4, bool OMEGA, bool PROBE, bool MIXED>
 __global__ __launch_bounds__(kIncThreads) void k_inc_place(
     uint32_t n, uint32_t n_tiles, const uint32_t* __restrict__ knew, const uint64_t* __restrict__ mask,
-    const uint32_t* __restrict__ tile_off, const uint64_t* __restrict__ mov, const uint32_t* __restrict__ count,
-    const uint32_t* __restrict__ lb_old, uint32_t* __restrict__ kout, uint32_t* __restrict__ perm,
-    ReorderIO<T4> io) {
+    const uint32_t* __restrict__ tile_off, const uint32_t* __restrict__ word_off, const uint64_t* __restrict__ mov,
+    const uint32_t* __restrict__ count, const uint32_t* __restrict__ lb_old, uint32_t* __restrict__ tile_cnt,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ perm, ReorderIO<T4> io) {
   const uint32_t m = count[0];
-  if (blockIdx.x < n_tiles) {
-    inc_tile_walk(n, mask, tile_off[blockIdx.x], [&](uint32_t i, uint32_t before, bool moved) {
-      if (moved) return;
-      const uint32_t key = knew[i];
-      const uint64_t e = ((uint64_t)key << 32) | i;
-      uint32_t lo = 0, hi = m;   // moved entries below (key, i)
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (mov[mid] < e) lo = mid + 1;
-        else hi = mid;
-      }
-      const uint32_t pos = i - before + lo;
-      kout[pos] = key;
-      perm[pos] = i;
-      move_slot<T4, OMEGA, PROBE, MIXED>(io, i, pos, key);
-    });
-  }
-  for (uint32_t b = blockIdx.x * (uint32_t)kIncThreads + threadIdx.x; b < m; b += gridDim.x * (uint32_t)kIncThreads) {
+  if (threadIdx.x == 0) tile_cnt[blockIdx.x] = 0;   // for the next step's counts
+  inc_tile_walk(n, mask, tile_off[blockIdx.x], [&](uint32_t i, uint32_t before, bool moved, bool valid) {
+    const bool kept = valid && !moved;
+    const uint64_t km = __ballot(kept);
+    if (km == 0ull) return;
+    const uint32_t key = kept ? knew[i] : 0u;
+    const uint64_t e = ((uint64_t)key << 32) | i;
+    // kept slots are in (key, slot) order, so the wavefront's bounds lie
+    // between those of its first and last kept slot
+    const int first = __builtin_ctzll(km), last = 63 - __builtin_clzll(km);
+    const uint64_t ea = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)key, first) << 32) |
+                        (uint32_t)__builtin_amdgcn_readlane((int)i, first);
+    const uint64_t eb = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)key, last) << 32) |
+                        (uint32_t)__builtin_amdgcn_readlane((int)i, last);
+    const uint2 bnd = wave_lower_bounds(mov, m, ea, eb);
+    if (!kept) return;
+    uint32_t lo = bnd.x, hi = bnd.y;   // moved entries below (key, i)
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (mov[mid] < e) lo = mid + 1;
+      else hi = mid;
+    }
+    const uint32_t pos = i - before + lo;
+    kout[pos] = key;
+    perm[pos] = i;
+    move_slot<T4, OMEGA, PROBE, MIXED>(io, i, pos, key);
+  });
+  // moved entries, spread over the blocks
+  for (uint32_t b = threadIdx.x * n_tiles + blockIdx.x; b < m; b += kIncThreads * n_tiles) {
     const uint64_t e = mov[b];
     const uint32_t key = (uint32_t)(e >> 32), i = (uint32_t)e;
     const uint32_t t = i < lb_old[key] ? lb_old[key] : lb_old[key + 1];
-    const uint32_t pos = b + t - inc_moved_before(t, mask, tile_off);
+    const uint32_t pos = b + t - inc_moved_before(t, n, m, mask, word_off);
     kout[pos] = key;
     perm[pos] = i;
     move_slot<T4, OMEGA, PROBE, MIXED>(io, i, pos, key);
@@ -624,14 +676,11 @@ static void incremental_sort(Ctx& c) {
   uint32_t* cnt = c.inc_count.as<uint32_t>();
   uint32_t* tile_cnt = c.inc_tiles.as<uint32_t>();      // n_tiles, from the step kernel
   uint32_t* tiles = tile_cnt + n_tiles;                  // tile offsets, n_tiles + 1
+  uint32_t* woff = c.inc_woff.as<uint32_t>();            // moved slots before each mask word
   {
-    LaunchScope ls(&c, "sort_inc_scan", 12.0 * n_tiles);
-    k_inc_scan<<<1, kSortThreads, 0, c.stream>>>(tile_cnt, n_tiles, tiles, cnt, c.inc_host_dev);
-    PR_HIP(hipGetLastError());
-  }
-  {
-    LaunchScope ls(&c, "sort_inc_gather", 4.0 * n);
-    k_inc_gather<<<n_tiles, kIncThreads, 0, c.stream>>>(n, knew, mask, tiles, c.inc_mov.as<uint64_t>());
+    LaunchScope ls(&c, "sort_inc_gather", 4.0 * n_tiles * n_tiles + 8.5 * n);
+    k_inc_gather<<<n_tiles, kIncThreads, 0, c.stream>>>(n, n_tiles, tile_cnt, knew, mask, tiles, woff, cnt,
+                                                        c.inc_host_dev, c.inc_mov.as<uint64_t>());
     PR_HIP(hipGetLastError());
   }
   {
@@ -649,7 +698,7 @@ static void incremental_sort(Ctx& c) {
     uint32_t* perm = c.vals[0].as<uint32_t>();
     with_reorder_form(c, c.cur, c.cur ^ 1, [&]<class T4, bool OMEGA, bool PROBE, bool MIXED>(ReorderIO<T4> io) {
       k_inc_place<T4, OMEGA, PROBE, MIXED><<<n_tiles, kIncThreads, 0, c.stream>>>(
-          n, n_tiles, knew, mask, tiles, mov, cnt, lb_old, kold, perm, io);
+          n, n_tiles, knew, mask, tiles, woff, mov, cnt, lb_old, tile_cnt, kold, perm, io);
     });
     PR_HIP(hipGetLastError());
   }
