@@ -706,22 +706,31 @@ static void incremental_sort(Ctx& c) {
   c.perm_buf = 0;
 }
 
-// lb[c] = first slot with key >= c, for c = 0 .. n_cells.  Thread k = 0 .. n
-// owns the cells [lo, hi) whose lower bound is k: lo = key[k-1] + 1 (0 for
-// k = 0), hi = key[k] + 1 (n_cells + 1 for k = n), so every cell is written
-// exactly once and no clearing pass is needed.  A lane writes a range of up
-// to kOwnCells cells itself; longer ranges (the empty stretches of a row
-// beside the walls) are written by the whole wavefront, 64 cells per store.
+// lb[c] = first slot with key >= c, for c = 0 .. n_cells.  Thread k < n owns
+// the cells [lo, hi) whose lower bound is k: lo = key[k-1] + 1 (0 for k = 0),
+// hi = key[k] + 1; the cells after the last key (lower bound n: the empty
+// top of the domain) are spread over kTailBlocks extra blocks.  Every cell
+// is written exactly once and no clearing pass is needed.  A lane writes a
+// range of up to kOwnCells cells itself; longer ranges (the empty stretches
+// of a row beside the walls) are written by the whole wavefront, 64 cells per
+// store.
 constexpr uint32_t kOwnCells = 4;
+constexpr uint32_t kTailBlocks = 64;
 
 __global__ __launch_bounds__(kBlock) void k_cell_lb(uint32_t n, uint32_t n_cells,
                                                     const uint32_t* __restrict__ skeys,
-                                                    uint32_t* __restrict__ lb) {
+                                                    uint32_t* __restrict__ lb, uint32_t key_blocks) {
+  if (blockIdx.x >= key_blocks) {
+    const uint32_t first = n == 0 ? 0u : skeys[n - 1] + 1u;
+    const uint32_t stride = (gridDim.x - key_blocks) * kBlock;
+    for (uint32_t c = first + (blockIdx.x - key_blocks) * kBlock + threadIdx.x; c <= n_cells; c += stride) lb[c] = n;
+    return;
+  }
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t lo = 0, hi = 0;
-  if (k <= n) {
+  if (k < n) {
     lo = k == 0 ? 0u : skeys[k - 1] + 1u;
-    hi = k == n ? n_cells + 1u : skeys[k] + 1u;
+    hi = skeys[k] + 1u;
   }
   const uint32_t len = hi > lo ? hi - lo : 0u;
   if (len <= kOwnCells)
@@ -739,12 +748,19 @@ __global__ __launch_bounds__(kBlock) void k_cell_lb(uint32_t n, uint32_t n_cells
   }
 }
 
+static void launch_cell_lb(Ctx& c, const uint32_t* skeys) {
+  const uint32_t n = (uint32_t)c.n;
+  const uint32_t key_blocks = (uint32_t)blocks_for(n);
+  k_cell_lb<<<key_blocks + kTailBlocks, kBlock, 0, c.stream>>>(n, c.grid.n_cells, skeys, c.lb.as<uint32_t>(),
+                                                               key_blocks);
+  PR_HIP(hipGetLastError());
+}
+
 void neighbors_build(Ctx& c) {
   const uint32_t n = (uint32_t)c.n;
   if (!c.keys_valid) keys_from_positions(c);
   if (n == 0) {
-    k_cell_lb<<<1, kBlock, 0, c.stream>>>(0, c.grid.n_cells, nullptr, c.lb.as<uint32_t>());
-    PR_HIP(hipGetLastError());
+    launch_cell_lb(c, nullptr);
     c.sorted = true; c.built = true;
     return;
   }
@@ -777,9 +793,7 @@ void neighbors_build(Ctx& c) {
   c.cur_ids ^= 1;
   {
     LaunchScope ls(&c, "cell_lb", 8.0 * n + 4.0 * c.grid.n_cells);
-    k_cell_lb<<<blocks_for((uint64_t)n + 1), kBlock, 0, c.stream>>>(n, c.grid.n_cells, skeys,
-                                                                    c.lb.as<uint32_t>());
-    PR_HIP(hipGetLastError());
+    launch_cell_lb(c, skeys);
   }
   c.sorted = true;
   c.built = true;
