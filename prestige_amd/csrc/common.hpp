@@ -229,8 +229,8 @@ struct Ctx {
   int perm_buf = 0;              // which vals[] holds the last permutation
   DevBuf sort_hist, sort_tot;    // per-tile digit counts, per-digit totals
   // incremental sort: moved-slot mask (bit per slot), per-tile moved counts
-  // and offsets, moved count, moved entries (key << 32 | slot), scratch, and
-  // the moved slots before each mask word
+  // and offsets, moved count, moved entries (key << 32 | slot), the same
+  // sorted, and the moved slots before each mask word
   DevBuf inc_mask, inc_tiles, inc_count, inc_mov, inc_tmp, inc_woff;
   uint32_t* inc_host = nullptr;  // pinned, mapped: moved count of the last incremental sort
   uint32_t* inc_host_dev = nullptr;  // the same word as the device writes it

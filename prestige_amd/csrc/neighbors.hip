@@ -328,8 +328,15 @@ ReorderIO<T4> reorder_io(Ctx& c, int src, int dst) {
                        (MIXED && OMEGA) ? c.om32.as<float4>() : nullptr};
 }
 
-// Calls fn.template operator()<T4, OMEGA, PROBE, MIXED>(io) for the context's
-// form (the mixed forms build the FP32 probe; DEM moves omega).
+// The copy variant as a type: fn(ReorderForm<...>{}, io) picks the kernel.
+template <class T4, bool OMEGA, bool PROBE, bool MIXED>
+struct ReorderForm {
+  using T = T4;
+  static constexpr bool omega = OMEGA, probe = PROBE, mixed = MIXED;
+};
+
+// Calls fn(form, io) for the context's form (the mixed forms build the FP32
+// probe; DEM moves omega).
 template <class Fn>
 void with_reorder_form(Ctx& c, int src, int dst, Fn&& fn) {
   const bool dem = c.desc.model == PRESTIGE_MODEL_DEM;
@@ -337,14 +344,14 @@ void with_reorder_form(Ctx& c, int src, int dst, Fn&& fn) {
   const bool mixed = c.desc.precision == PRESTIGE_MIXED;
   const bool f32 = c.desc.precision == PRESTIGE_F32;
   if (dem) {
-    if (mixed) fn.template operator()<double4, true, true, true>(reorder_io<double4, true, true>(c, src, dst));
-    else fn.template operator()<double4, true, false, false>(reorder_io<double4, true, false>(c, src, dst));
+    if (mixed) fn(ReorderForm<double4, true, true, true>{}, reorder_io<double4, true, true>(c, src, dst));
+    else fn(ReorderForm<double4, true, false, false>{}, reorder_io<double4, true, false>(c, src, dst));
   } else if (f32) {
-    fn.template operator()<float4, false, false, false>(reorder_io<float4, false, false>(c, src, dst));
+    fn(ReorderForm<float4, false, false, false>{}, reorder_io<float4, false, false>(c, src, dst));
   } else if (sph && mixed) {
-    fn.template operator()<double4, false, true, true>(reorder_io<double4, false, true>(c, src, dst));
+    fn(ReorderForm<double4, false, true, true>{}, reorder_io<double4, false, true>(c, src, dst));
   } else {
-    fn.template operator()<double4, false, false, false>(reorder_io<double4, false, false>(c, src, dst));
+    fn(ReorderForm<double4, false, false, false>{}, reorder_io<double4, false, false>(c, src, dst));
   }
 }
 
@@ -376,11 +383,10 @@ double reorder_bytes(const Ctx& c) {
 // reorder, and the permutation and keys are exactly the stable sort's, as
 // the radix sort would give them.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kIncLds = 16384;      // moved entries one block sorts in LDS (128 KB)
-#ifndef PRESTIGE_BITONIC_MAX
-#define PRESTIGE_BITONIC_MAX 2048
-#endif
-constexpr uint32_t kIncBitonicMax = PRESTIGE_BITONIC_MAX;   // larger moved sets take the radix path
+// Up to kRankMax moved entries are ordered by counting their ranks, one
+// block per 64 entries; more take a one-block radix sort (the host then
+// prefers the full radix sort for the next builds).
+constexpr uint32_t kRankMax = 8192;
 constexpr int kIncSortThreads = 1024;
 
 // Gather and placement walk a sort tile with kIncThreads threads, kIncItems
@@ -476,49 +482,61 @@ __global__ __launch_bounds__(kIncThreads) void k_inc_gather(
   });
 }
 
-// One block sorts the moved entries by (key, slot): bitonic in LDS when they
-// fit, else a stable LSD radix sort on the key over global memory (entries
-// arrive in slot order, so key order alone keeps (key, slot) order).
+// Writes the moved entries sorted by (key, slot) to out.  Up to kRankMax
+// entries, block b places the 64-entry chunk b of them at its rank
+// #{y : (key_y, y) < (key_x, x)}, counted by its 16 wavefronts over
+// interleaved 64-entry chunks of y and summed in LDS.  The entries are in
+// slot order, so (key_y, y) < (key_x, x) is key_y < key_x + [y < x]: one
+// 32-bit compare per pair, the y keys read by scalar loads.  Above kRankMax
+// entries block 0 runs a stable LSD radix sort on the key over mov and out
+// (entries arrive in slot order, so key order alone keeps (key, slot) order).
 __global__ __launch_bounds__(kIncSortThreads) void k_inc_sort(const uint32_t* __restrict__ count, int key_bits,
                                                               uint64_t* __restrict__ mov,
-                                                              uint64_t* __restrict__ tmp) {
-  __shared__ uint64_t s[kIncLds];
+                                                              uint64_t* __restrict__ out) {
+  constexpr uint32_t kW = kIncSortThreads / 64;
   const uint32_t m = count[0];
   const uint32_t t = threadIdx.x;
-  if (m <= 1u) return;
-  if (m <= kIncBitonicMax && m <= kIncLds) {
-    uint32_t P = 2;
-    while (P < m) P <<= 1;
-    for (uint32_t k = t; k < P; k += kIncSortThreads) s[k] = k < m ? mov[k] : ~0ull;
-    __syncthreads();
-    for (uint32_t k = 2; k <= P; k <<= 1) {
-      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-        for (uint32_t a = t; a < P; a += kIncSortThreads) {
-          const uint32_t b = a ^ j;
-          if (b > a) {
-            const uint64_t x = s[a], y = s[b];
-            const bool up = (a & k) == 0;
-            if ((x > y) == up) {
-              s[a] = y;
-              s[b] = x;
-            }
-          }
-        }
-        __syncthreads();
+  if (m <= kRankMax) {
+    const uint32_t nch = (m + 63u) >> 6;
+    const uint32_t xc = blockIdx.x;
+    if (xc >= nch) return;
+    __shared__ uint32_t s_cnt[kW][64];
+    const uint32_t lane = __lane_id();
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(t >> 6));
+    const uint32_t* keys = reinterpret_cast<const uint32_t*>(mov) + 1;   // high halves, stride 2
+    const uint32_t x = xc * 64u + lane;
+    const uint32_t kx = x < m ? keys[2u * x] : 0u;
+    uint32_t cnt = 0;
+    for (uint32_t c = w; c < nch; c += kW) {
+      const uint32_t y0 = c * 64u;
+      if (c != xc && y0 + 64u <= m) {
+        const uint32_t thr = c < xc ? kx + 1u : kx;   // all y before x, or all after
+#pragma unroll 16
+        for (uint32_t y = y0; y < y0 + 64u; ++y) cnt += keys[2u * y] < thr ? 1u : 0u;
+      } else {
+        const uint32_t y1 = min(y0 + 64u, m);
+        for (uint32_t y = y0; y < y1; ++y) cnt += keys[2u * y] < (y < x ? kx + 1u : kx) ? 1u : 0u;
       }
     }
-    for (uint32_t k = t; k < m; k += kIncSortThreads) mov[k] = s[k];
+    s_cnt[w][lane] = cnt;
+    __syncthreads();
+    if (w == 0 && x < m) {
+      uint32_t r = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < kW; ++k) r += s_cnt[k][lane];
+      if (r < m) out[r] = mov[x];   // always
+    }
     return;
   }
-  // radix fallback: digit histogram, then a stable scatter in chunks of
-  // kIncSortThreads entries (wavefront rank by ballot match + counts of the
-  // earlier wavefronts of the chunk)
-  constexpr uint32_t kW = kIncSortThreads / 64;
-  uint32_t* hist = reinterpret_cast<uint32_t*>(s);            // [kBuckets]
-  uint32_t* wcnt = hist + kBuckets;                           // [kW][kBuckets]
+  if (blockIdx.x != 0) return;
+  // digit histogram, then a stable scatter in chunks of kIncSortThreads
+  // entries (wavefront rank by ballot match + counts of the earlier
+  // wavefronts of the chunk)
+  __shared__ uint32_t hist[kBuckets];
+  __shared__ uint32_t wcnt[kW * kBuckets];
   const uint32_t w = t >> 6;
   uint64_t* src = mov;
-  uint64_t* dst = tmp;
+  uint64_t* dst = out;
   const int passes = (key_bits + kRadixBits - 1) / kRadixBits;
   for (int p = 0; p < passes; ++p) {
     const int shift = 32 + p * kRadixBits;
@@ -556,28 +574,28 @@ __global__ __launch_bounds__(kIncSortThreads) void k_inc_sort(const uint32_t* __
       for (uint32_t q = t; q < kW * kBuckets; q += kIncSortThreads) wcnt[q] = 0;
       __syncthreads();
       const uint64_t peers = match_digit(d, valid);
-      const uint32_t rank = (uint32_t)__popcll(peers & lanes_below());
-      if (valid && rank == 0) wcnt[w * kBuckets + d] = (uint32_t)__popcll(peers);
+      const uint32_t r = (uint32_t)__popcll(peers & lanes_below());
+      if (valid && r == 0) wcnt[w * kBuckets + d] = (uint32_t)__popcll(peers);
       __syncthreads();
       if (t < kBuckets) {   // per digit: offsets of the wavefronts of this chunk
-        uint32_t r = hist[t];
+        uint32_t run = hist[t];
         for (uint32_t q = 0; q < kW; ++q) {
           const uint32_t c = wcnt[q * kBuckets + t];
-          wcnt[q * kBuckets + t] = r;
-          r += c;
+          wcnt[q * kBuckets + t] = run;
+          run += c;
         }
-        hist[t] = r;
+        hist[t] = run;
       }
       __syncthreads();
-      if (valid) dst[wcnt[w * kBuckets + d] + rank] = e;
+      if (valid) dst[wcnt[w * kBuckets + d] + r] = e;
       __syncthreads();
     }
     uint64_t* x = src;
     src = dst;
     dst = x;
   }
-  if (src != mov)
-    for (uint32_t k = t; k < m; k += kIncSortThreads) mov[k] = src[k];
+  if (src != out)
+    for (uint32_t k = t; k < m; k += kIncSortThreads) out[k] = src[k];
 }
 
 // Moved slots before slot t (t <= n; m = all moved slots).
@@ -649,6 +667,7 @@ __global__ __launch_bounds__(kIncThreads) void k_inc_place(
       else hi = mid;
     }
     const uint32_t pos = i - before + lo;
+    if (pos >= n) return;   // never, when the moved counts are consistent
     kout[pos] = key;
     perm[pos] = i;
     move_slot<T4, OMEGA, PROBE, MIXED>(io, i, pos, key);
@@ -659,6 +678,7 @@ __global__ __launch_bounds__(kIncThreads) void k_inc_place(
     const uint32_t key = (uint32_t)(e >> 32), i = (uint32_t)e;
     const uint32_t t = i < lb_old[key] ? lb_old[key] : lb_old[key + 1];
     const uint32_t pos = b + t - inc_moved_before(t, n, m, mask, word_off);
+    if (pos >= n) continue;   // never, when the moved counts are consistent
     kout[pos] = key;
     perm[pos] = i;
     move_slot<T4, OMEGA, PROBE, MIXED>(io, i, pos, key);
@@ -685,19 +705,21 @@ static void incremental_sort(Ctx& c) {
   }
   {
     LaunchScope ls(&c, "sort_inc_sort", 0.0);
-    k_inc_sort<<<1, kIncSortThreads, 0, c.stream>>>(cnt, key_bits(c.grid.n_cells), c.inc_mov.as<uint64_t>(),
-                                                    c.inc_tmp.as<uint64_t>());
+    k_inc_sort<<<kRankMax / 64u, kIncSortThreads, 0, c.stream>>>(cnt, key_bits(c.grid.n_cells),
+                                                                 c.inc_mov.as<uint64_t>(),
+                                                                 c.inc_tmp.as<uint64_t>());
     PR_HIP(hipGetLastError());
   }
   {
     // placement and the reorder in one pass: each slot's state goes straight
     // to its sorted position
     LaunchScope ls(&c, "sort_inc_place", (12.0 + reorder_bytes(c)) * n);
-    const uint64_t* mov = c.inc_mov.as<uint64_t>();
+    const uint64_t* mov = c.inc_tmp.as<uint64_t>();   // sorted
     const uint32_t* lb_old = c.lb.as<uint32_t>();
     uint32_t* perm = c.vals[0].as<uint32_t>();
-    with_reorder_form(c, c.cur, c.cur ^ 1, [&]<class T4, bool OMEGA, bool PROBE, bool MIXED>(ReorderIO<T4> io) {
-      k_inc_place<T4, OMEGA, PROBE, MIXED><<<n_tiles, kIncThreads, 0, c.stream>>>(
+    with_reorder_form(c, c.cur, c.cur ^ 1, [&](auto form, auto io) {
+      using F = decltype(form);
+      k_inc_place<typename F::T, F::omega, F::probe, F::mixed><<<n_tiles, kIncThreads, 0, c.stream>>>(
           n, n_tiles, knew, mask, tiles, woff, mov, cnt, lb_old, tile_cnt, kold, perm, io);
     });
     PR_HIP(hipGetLastError());
@@ -770,7 +792,7 @@ void neighbors_build(Ctx& c) {
   bool inc = false;
   if (c.inc_ready && c.sort_policy != Ctx::kSortRadix) {
     const uint32_t seen = *(volatile const uint32_t*)c.inc_host;
-    inc = c.sort_policy == Ctx::kSortIncremental || seen <= kIncLds / 2u || ++c.inc_skipped >= 16u;
+    inc = c.sort_policy == Ctx::kSortIncremental || seen <= kRankMax || ++c.inc_skipped >= 16u;
   }
   if (inc) {
     c.inc_skipped = 0;
@@ -784,8 +806,9 @@ void neighbors_build(Ctx& c) {
   if (!inc) {   // the incremental sort moved the state itself
     const uint32_t* perm = c.vals[c.perm_buf].as<uint32_t>();
     LaunchScope ls(&c, "reorder", reorder_bytes(c) * n);
-    with_reorder_form(c, src, dst, [&]<class T4, bool OMEGA, bool PROBE, bool MIXED>(ReorderIO<T4> io) {
-      k_reorder<T4, OMEGA, PROBE, MIXED><<<blocks_for(n), kBlock, 0, c.stream>>>(n, perm, skeys, io);
+    with_reorder_form(c, src, dst, [&](auto form, auto io) {
+      using F = decltype(form);
+      k_reorder<typename F::T, F::omega, F::probe, F::mixed><<<blocks_for(n), kBlock, 0, c.stream>>>(n, perm, skeys, io);
     });
     PR_HIP(hipGetLastError());
   }
