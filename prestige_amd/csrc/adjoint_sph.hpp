@@ -130,12 +130,19 @@ __device__ inline void block_sum2(double a, double b, double2* __restrict__ part
 // (dalpha, dc0).  One wavefront per block; each row run's neighbour data are
 // staged in LDS as in the forward force pass.
 constexpr int kAdjBlock = 64;
+// Wavefronts per SIMD the register budgets are held to (each kernel's
+// occupancy before the tile loop).
+template <int F>
+constexpr int kAdjForceWaves = F == kF64 ? 2 : 4;
+template <int F>
+constexpr int kAdjDensityWaves = F == kF64 ? 5 : 7;
 
 template <int F>
-__global__ __launch_bounds__(kAdjBlock) void k_adj_force(
+__global__ __launch_bounds__(kAdjBlock, kAdjForceWaves<F>) void k_adj_force(
     uint32_t n, uint64_t n_moving, SphStepRecord rec, Grid g, SphConst kc, SphUnits u, double alpha, double c0,
     double rho0, const typename AdjForm<F>::C4* __restrict__ cvec, double4* __restrict__ xbf,
-    double4* __restrict__ vb, typename AdjForm<F>::ar* __restrict__ rhob, double2* __restrict__ partial) {
+    double4* __restrict__ vb, typename AdjForm<F>::ar* __restrict__ rhob, double2* __restrict__ partial,
+    TileQueue tq) {
   using real = typename SphForm<F>::real;
   using G4 = typename SphForm<F>::G4;
   using R2 = typename SphForm<F>::R2;
@@ -148,138 +155,140 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_force(
   __shared__ R2 s_rp[kForceCap];
   __shared__ C4 s_c[kForceCap];
   const uint32_t lane = threadIdx.x;
-  const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kAdjBlock + lane;
-  const bool live = i < n;
-  double pa = 0.0, pc = 0.0;
-  G4 pi{}, vi{};
-  R2 rpi{};
-  C4 ci{};
-  if (live) {
-    pi = gpos[i];
-    vi = gvel[i];
-    rpi = rp[i];
-    ci = cvec[i];
-  }
-  {
-    const real H2 = (real)u.H2;
-    const ar inv_h = (ar)u.inv_h, eps2 = (ar)u.eps2, hv = (ar)u.hv, ac0 = (ar)kc.alpha_c0;
-    const ar rhoi = (ar)rpi.x, pri = (ar)rpi.y;
-    const ar cix = (ar)ci.x, ciy = (ar)ci.y, ciz = (ar)ci.z;
-    // sums: per row run in the pair precision (a few terms each), then FP64
-    double xx = 0, xy = 0, xz = 0, vx = 0, vy = 0, vz = 0, pb = 0, rb = 0, sa = 0;
-    ar qxx = 0, qxy = 0, qxz = 0, qvx = 0, qvy = 0, qvz = 0, qpb = 0, qrb = 0, qsa = 0;
-    auto end_run = [&] {
-      xx += (double)qxx; xy += (double)qxy; xz += (double)qxz;
-      vx += (double)qvx; vy += (double)qvy; vz += (double)qvz;
-      pb += (double)qpb; rb += (double)qrb; sa += (double)qsa;
-      qxx = qxy = qxz = qvx = qvy = qvz = qpb = qrb = qsa = ar(0.0);
-    };
-    // pair rows over a batch of neighbours fetched by get(j, p, v, rp, c)
-    auto body = [&](const uint32_t* jj, auto&& get) {
-      G4 pj[kUnr], vj[kUnr];
-      R2 rpj[kUnr];
-      C4 cj[kUnr];
-#pragma unroll
-      for (int v = 0; v < kUnr; ++v) get(jj[v] == kPad ? jj[0] : jj[v], pj[v], vj[v], rpj[v], cj[v]);
-#pragma unroll
-      for (int v = 0; v < kUnr; ++v) {
-        if (jj[v] == kPad) continue;
-        real dxr, dyr, dzr;
-        form_pair<F>(pi, pj[v], u.cs_u, dxr, dyr, dzr);
-        real r2r = dxr * dxr + dyr * dyr + dzr * dzr;
-        if (!(r2r < H2 && r2r > real(0.0))) continue;
-        const ar dx = (ar)dxr, dy = (ar)dyr, dz = (ar)dzr, r2 = (ar)r2r;
-        ar rr, irr;
-        adj_len(r2, rr, irr);
-        const ar q = rr * inv_h;
-        const ar f1 = spline_df(q), f2 = spline_d2f(q);
-        const ar gr = f1 * irr;
-        const ar gp = (f2 * inv_h - gr) * irr;   // d(f'(q)/r)/dr
-        const ar ex = (ar)vi.x - (ar)vj[v].x, ey = (ar)vi.y - (ar)vj[v].y, ez = (ar)vi.z - (ar)vj[v].z;
-        const ar w = ex * dx + ey * dy + ez * dz;
-        const ar D = r2 + eps2, iD = adj_rcp(D);
-        const ar rbar = ar(0.5) * (rhoi + (ar)rpj[v].x), irb = adj_rcp(rbar);
-        ar mu = ar(0.0), Pi = ar(0.0);
-        if (w < ar(0.0)) {
-          mu = hv * w * iD;
-          Pi = -ac0 * mu * irb;
-        }
-        const ar Q = pri + (ar)rpj[v].y + Pi;
-        const ar S = Q * gr;
-        const ar Cx = cix - (ar)cj[v].x, Cy = ciy - (ar)cj[v].y, Cz = ciz - (ar)cj[v].z;
-        const ar U = Cx * dx + Cy * dy + Cz * dz;
-        const ar Ug = U * gr;
-        const ar t = U * Q * gp * irr;
-        qxx += S * Cx + t * dx;
-        qxy += S * Cy + t * dy;
-        qxz += S * Cz + t * dz;
-        qpb += Ug;
-        if (w < ar(0.0)) {
-          const ar mub = -Ug * ac0 * irb;
-          const ar hD = hv * iD, k2 = ar(2.0) * hv * w * iD * iD;
-          qxx += mub * (hD * ex - k2 * dx);
-          qxy += mub * (hD * ey - k2 * dy);
-          qxz += mub * (hD * ez - k2 * dz);
-          qvx += mub * hD * dx;
-          qvy += mub * hD * dy;
-          qvz += mub * hD * dz;
-          qrb += Ug * ac0 * mu * ar(0.5) * irb * irb;
-          // d/dalpha = c0 sum, d/dc0 = alpha sum (Pi is linear in alpha c0)
-          qsa -= (cix * dx + ciy * dy + ciz * dz) * gr * mu * irb;
-        }
-      }
-    };
-    walk_runs_staged<kUnr, kForceCap>(
-        rec.meta, rec.words, n, i, live, lane,
-        [&](uint32_t k, uint32_t j) {
-          s_p[k] = gpos[j];
-          s_v[k] = gvel[j];
-          s_rp[k] = rp[j];
-          s_c[k] = cvec[j];
-        },
-        [&](const uint32_t* jj) {
-          body(jj, [&](uint32_t k, G4& p, G4& v, R2& r, C4& cc) {
-            p = s_p[k];
-            v = s_v[k];
-            r = s_rp[k];
-            cc = s_c[k];
-          });
-        },
-        [&](const uint32_t* jj) {
-          body(jj, [&](uint32_t j, G4& p, G4& v, R2& r, C4& cc) {
-            p = gpos[j];
-            v = gvel[j];
-            r = rp[j];
-            cc = cvec[j];
-          });
-        },
-        end_run);
+  for_each_tile(tq, [&](uint32_t tile) {
+    const uint32_t i = tile * kAdjBlock + lane;
+    const bool live = i < n;
+    double pa = 0.0, pc = 0.0;
+    G4 pi{}, vi{};
+    R2 rpi{};
+    C4 ci{};
     if (live) {
-      pa = sa * c0;
-      pc = sa * alpha;
-      // EOS: P = p / rho^2, p = B((rho/rho0)^7 - 1) unless the forward clamped
-      // it to 0 (then P = 0 and the EOS has no derivative)
-      const double rho = (double)rpi.x;
-      const double ratio = rho * kc.inv_rho0;
-      const double r6 = ratio * ratio * ratio * ratio * ratio * ratio;
-      const double p = (double)rpi.y * rho * rho;
-      if ((double)rpi.y > 0.0) {
-        const double dpdrho = 7.0 * kc.B * r6 * kc.inv_rho0;
-        const double dPdrho = dpdrho / (rho * rho) - 2.0 * p / (rho * rho * rho);
-        rb += pb * dPdrho;
-        const double dPdc0 = (r6 * ratio - 1.0) * (2.0 * rho0 * c0 / 7.0) / (rho * rho);
-        pc += pb * dPdc0;
-      }
-      rhob[i] = (ar)rb;
-      const double us = u.unit_scale;   // pair-geometry units -> metres for position adjoints
-      xbf[i] = make_double4(us * xx, us * xy, us * xz, 0.0);
-      if ((uint64_t)rec.ids[i] < n_moving) {
-        double4 v0 = vb[i];
-        vb[i] = make_double4(v0.x + vx, v0.y + vy, v0.z + vz, 0.0);
+      pi = gpos[i];
+      vi = gvel[i];
+      rpi = rp[i];
+      ci = cvec[i];
+    }
+    {
+      const real H2 = (real)u.H2;
+      const ar inv_h = (ar)u.inv_h, eps2 = (ar)u.eps2, hv = (ar)u.hv, ac0 = (ar)kc.alpha_c0;
+      const ar rhoi = (ar)rpi.x, pri = (ar)rpi.y;
+      const ar cix = (ar)ci.x, ciy = (ar)ci.y, ciz = (ar)ci.z;
+      // sums: per row run in the pair precision (a few terms each), then FP64
+      double xx = 0, xy = 0, xz = 0, vx = 0, vy = 0, vz = 0, pb = 0, rb = 0, sa = 0;
+      ar qxx = 0, qxy = 0, qxz = 0, qvx = 0, qvy = 0, qvz = 0, qpb = 0, qrb = 0, qsa = 0;
+      auto end_run = [&] {
+        xx += (double)qxx; xy += (double)qxy; xz += (double)qxz;
+        vx += (double)qvx; vy += (double)qvy; vz += (double)qvz;
+        pb += (double)qpb; rb += (double)qrb; sa += (double)qsa;
+        qxx = qxy = qxz = qvx = qvy = qvz = qpb = qrb = qsa = ar(0.0);
+      };
+      // pair rows over a batch of neighbours fetched by get(j, p, v, rp, c)
+      auto body = [&](const uint32_t* jj, auto&& get) {
+        G4 pj[kUnr], vj[kUnr];
+        R2 rpj[kUnr];
+        C4 cj[kUnr];
+  #pragma unroll
+        for (int v = 0; v < kUnr; ++v) get(jj[v] == kPad ? jj[0] : jj[v], pj[v], vj[v], rpj[v], cj[v]);
+  #pragma unroll
+        for (int v = 0; v < kUnr; ++v) {
+          if (jj[v] == kPad) continue;
+          real dxr, dyr, dzr;
+          form_pair<F>(pi, pj[v], u.cs_u, dxr, dyr, dzr);
+          real r2r = dxr * dxr + dyr * dyr + dzr * dzr;
+          if (!(r2r < H2 && r2r > real(0.0))) continue;
+          const ar dx = (ar)dxr, dy = (ar)dyr, dz = (ar)dzr, r2 = (ar)r2r;
+          ar rr, irr;
+          adj_len(r2, rr, irr);
+          const ar q = rr * inv_h;
+          const ar f1 = spline_df(q), f2 = spline_d2f(q);
+          const ar gr = f1 * irr;
+          const ar gp = (f2 * inv_h - gr) * irr;   // d(f'(q)/r)/dr
+          const ar ex = (ar)vi.x - (ar)vj[v].x, ey = (ar)vi.y - (ar)vj[v].y, ez = (ar)vi.z - (ar)vj[v].z;
+          const ar w = ex * dx + ey * dy + ez * dz;
+          const ar D = r2 + eps2, iD = adj_rcp(D);
+          const ar rbar = ar(0.5) * (rhoi + (ar)rpj[v].x), irb = adj_rcp(rbar);
+          ar mu = ar(0.0), Pi = ar(0.0);
+          if (w < ar(0.0)) {
+            mu = hv * w * iD;
+            Pi = -ac0 * mu * irb;
+          }
+          const ar Q = pri + (ar)rpj[v].y + Pi;
+          const ar S = Q * gr;
+          const ar Cx = cix - (ar)cj[v].x, Cy = ciy - (ar)cj[v].y, Cz = ciz - (ar)cj[v].z;
+          const ar U = Cx * dx + Cy * dy + Cz * dz;
+          const ar Ug = U * gr;
+          const ar t = U * Q * gp * irr;
+          qxx += S * Cx + t * dx;
+          qxy += S * Cy + t * dy;
+          qxz += S * Cz + t * dz;
+          qpb += Ug;
+          if (w < ar(0.0)) {
+            const ar mub = -Ug * ac0 * irb;
+            const ar hD = hv * iD, k2 = ar(2.0) * hv * w * iD * iD;
+            qxx += mub * (hD * ex - k2 * dx);
+            qxy += mub * (hD * ey - k2 * dy);
+            qxz += mub * (hD * ez - k2 * dz);
+            qvx += mub * hD * dx;
+            qvy += mub * hD * dy;
+            qvz += mub * hD * dz;
+            qrb += Ug * ac0 * mu * ar(0.5) * irb * irb;
+            // d/dalpha = c0 sum, d/dc0 = alpha sum (Pi is linear in alpha c0)
+            qsa -= (cix * dx + ciy * dy + ciz * dz) * gr * mu * irb;
+          }
+        }
+      };
+      walk_runs_staged<kUnr, kForceCap>(
+          rec.meta, rec.words, n, i, live, lane,
+          [&](uint32_t k, uint32_t j) {
+            s_p[k] = gpos[j];
+            s_v[k] = gvel[j];
+            s_rp[k] = rp[j];
+            s_c[k] = cvec[j];
+          },
+          [&](const uint32_t* jj) {
+            body(jj, [&](uint32_t k, G4& p, G4& v, R2& r, C4& cc) {
+              p = s_p[k];
+              v = s_v[k];
+              r = s_rp[k];
+              cc = s_c[k];
+            });
+          },
+          [&](const uint32_t* jj) {
+            body(jj, [&](uint32_t j, G4& p, G4& v, R2& r, C4& cc) {
+              p = gpos[j];
+              v = gvel[j];
+              r = rp[j];
+              cc = cvec[j];
+            });
+          },
+          end_run);
+      if (live) {
+        pa = sa * c0;
+        pc = sa * alpha;
+        // EOS: P = p / rho^2, p = B((rho/rho0)^7 - 1) unless the forward clamped
+        // it to 0 (then P = 0 and the EOS has no derivative)
+        const double rho = (double)rpi.x;
+        const double ratio = rho * kc.inv_rho0;
+        const double r6 = ratio * ratio * ratio * ratio * ratio * ratio;
+        const double p = (double)rpi.y * rho * rho;
+        if ((double)rpi.y > 0.0) {
+          const double dpdrho = 7.0 * kc.B * r6 * kc.inv_rho0;
+          const double dPdrho = dpdrho / (rho * rho) - 2.0 * p / (rho * rho * rho);
+          rb += pb * dPdrho;
+          const double dPdc0 = (r6 * ratio - 1.0) * (2.0 * rho0 * c0 / 7.0) / (rho * rho);
+          pc += pb * dPdc0;
+        }
+        rhob[i] = (ar)rb;
+        const double us = u.unit_scale;   // pair-geometry units -> metres for position adjoints
+        xbf[i] = make_double4(us * xx, us * xy, us * xz, 0.0);
+        if ((uint64_t)rec.ids[i] < n_moving) {
+          double4 v0 = vb[i];
+          vb[i] = make_double4(v0.x + vx, v0.y + vy, v0.z + vz, 0.0);
+        }
       }
     }
-  }
-  block_sum2<kAdjBlock / 64>(pa, pc, partial, blockIdx.x);
+    block_sum2<kAdjBlock / 64>(pa, pc, partial, tile);   // one partial per tile: a fixed sum order
+  });
 }
 
 // Density adjoint + assembly: xbar_k = xbar'_k + force part + density part
@@ -287,11 +296,11 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_force(
 // step's slot order through the frozen permutation.  One wavefront per
 // block, neighbour data staged per row run.
 template <int F>
-__global__ __launch_bounds__(kAdjBlock) void k_adj_density(
+__global__ __launch_bounds__(kAdjBlock, kAdjDensityWaves<F>) void k_adj_density(
     uint32_t n, uint64_t n_moving, SphStepRecord rec, Grid g, SphConst kc, SphUnits u,
     const typename AdjForm<F>::ar* __restrict__ rhob, const double4* __restrict__ xb_in,
     const double4* __restrict__ xbf, const double4* __restrict__ vb_in, double4* __restrict__ xb_out,
-    double4* __restrict__ vb_out) {
+    double4* __restrict__ vb_out, TileQueue tq) {
   using real = typename SphForm<F>::real;
   using G4 = typename SphForm<F>::G4;
   using ar = typename AdjForm<F>::ar;
@@ -299,73 +308,75 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_density(
   __shared__ G4 s_p[kForceCap];
   __shared__ ar s_rb[kForceCap];
   const uint32_t lane = threadIdx.x;
-  const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kAdjBlock + lane;
-  const bool live = i < n && (uint64_t)rec.ids[i] < n_moving;
-  G4 pi{};
-  ar rbi = ar(0.0);
-  if (live) {
-    pi = gpos[i];
-    rbi = rhob[i];
-  }
-  const real H2 = (real)u.H2;
-  const ar inv_h = (ar)u.inv_h;
-  double sx = 0, sy = 0, sz = 0;
-  ar qx = 0, qy = 0, qz = 0;  // per row run, then FP64
-  auto body = [&](const uint32_t* jj, auto&& get) {
-    G4 pj[kUnr];
-    ar rbj[kUnr];
-#pragma unroll
-    for (int v = 0; v < kUnr; ++v) get(jj[v] == kPad ? jj[0] : jj[v], pj[v], rbj[v]);
-#pragma unroll
-    for (int v = 0; v < kUnr; ++v) {
-      if (jj[v] == kPad) continue;
-      real dxr, dyr, dzr;
-      form_pair<F>(pi, pj[v], u.cs_u, dxr, dyr, dzr);
-      real r2r = dxr * dxr + dyr * dyr + dzr * dzr;
-      if (!(r2r < H2 && r2r > real(0.0))) continue;
-      ar rr, irr;
-      adj_len((ar)r2r, rr, irr);
-      const ar s = (rbi + rbj[v]) * spline_df(rr * inv_h) * irr;
-      qx += s * (ar)dxr;
-      qy += s * (ar)dyr;
-      qz += s * (ar)dzr;
+  for_each_tile(tq, [&](uint32_t tile) {
+    const uint32_t i = tile * kAdjBlock + lane;
+    const bool live = i < n && (uint64_t)rec.ids[i] < n_moving;
+    G4 pi{};
+    ar rbi = ar(0.0);
+    if (live) {
+      pi = gpos[i];
+      rbi = rhob[i];
     }
-  };
-  walk_runs_staged<kUnr, kForceCap>(
-      rec.meta, rec.words, n, i, live, lane,
-      [&](uint32_t k, uint32_t j) {
-        s_p[k] = gpos[j];
-        s_rb[k] = rhob[j];
-      },
-      [&](const uint32_t* jj) {
-        body(jj, [&](uint32_t k, G4& p, ar& rb) {
-          p = s_p[k];
-          rb = s_rb[k];
+    const real H2 = (real)u.H2;
+    const ar inv_h = (ar)u.inv_h;
+    double sx = 0, sy = 0, sz = 0;
+    ar qx = 0, qy = 0, qz = 0;  // per row run, then FP64
+    auto body = [&](const uint32_t* jj, auto&& get) {
+      G4 pj[kUnr];
+      ar rbj[kUnr];
+  #pragma unroll
+      for (int v = 0; v < kUnr; ++v) get(jj[v] == kPad ? jj[0] : jj[v], pj[v], rbj[v]);
+  #pragma unroll
+      for (int v = 0; v < kUnr; ++v) {
+        if (jj[v] == kPad) continue;
+        real dxr, dyr, dzr;
+        form_pair<F>(pi, pj[v], u.cs_u, dxr, dyr, dzr);
+        real r2r = dxr * dxr + dyr * dyr + dzr * dzr;
+        if (!(r2r < H2 && r2r > real(0.0))) continue;
+        ar rr, irr;
+        adj_len((ar)r2r, rr, irr);
+        const ar s = (rbi + rbj[v]) * spline_df(rr * inv_h) * irr;
+        qx += s * (ar)dxr;
+        qy += s * (ar)dyr;
+        qz += s * (ar)dzr;
+      }
+    };
+    walk_runs_staged<kUnr, kForceCap>(
+        rec.meta, rec.words, n, i, live, lane,
+        [&](uint32_t k, uint32_t j) {
+          s_p[k] = gpos[j];
+          s_rb[k] = rhob[j];
+        },
+        [&](const uint32_t* jj) {
+          body(jj, [&](uint32_t k, G4& p, ar& rb) {
+            p = s_p[k];
+            rb = s_rb[k];
+          });
+        },
+        [&](const uint32_t* jj) {
+          body(jj, [&](uint32_t j, G4& p, ar& rb) {
+            p = gpos[j];
+            rb = rhob[j];
+          });
+        },
+        [&] {
+          sx += (double)qx;
+          sy += (double)qy;
+          sz += (double)qz;
+          qx = qy = qz = ar(0.0);
         });
-      },
-      [&](const uint32_t* jj) {
-        body(jj, [&](uint32_t j, G4& p, ar& rb) {
-          p = gpos[j];
-          rb = rhob[j];
-        });
-      },
-      [&] {
-        sx += (double)qx;
-        sy += (double)qy;
-        sz += (double)qz;
-        qx = qy = qz = ar(0.0);
-      });
-  if (i >= n) return;
-  const uint32_t dst = rec.perm[i];
-  if (!live) {
-    xb_out[dst] = make_double4(0, 0, 0, 0);
-    vb_out[dst] = make_double4(0, 0, 0, 0);
-    return;
-  }
-  const double k = kc.msig * u.inv_h * u.unit_scale;
-  const double4 a = xb_in[i], b = xbf[i];
-  xb_out[dst] = make_double4(a.x + b.x + k * sx, a.y + b.y + k * sy, a.z + b.z + k * sz, 0.0);
-  vb_out[dst] = vb_in[i];
+    if (i >= n) return;
+    const uint32_t dst = rec.perm[i];
+    if (!live) {
+      xb_out[dst] = make_double4(0, 0, 0, 0);
+      vb_out[dst] = make_double4(0, 0, 0, 0);
+      return;
+    }
+    const double k = kc.msig * u.inv_h * u.unit_scale;
+    const double4 a = xb_in[i], b = xbf[i];
+    xb_out[dst] = make_double4(a.x + b.x + k * sx, a.y + b.y + k * sy, a.z + b.z + k * sz, 0.0);
+    vb_out[dst] = vb_in[i];
+  });
 }
 
 // Sum the block partials of one reverse step into acc[0..1] (one block).
@@ -406,7 +417,8 @@ template <int F>
 void sph_reverse_step(Ctx& c, const SphStepRecord& rec, AdjBufs& A) {
   const uint32_t n = (uint32_t)c.n;
   const SphUnits u = sph_units<F>(c);
-  const uint32_t grid = xcd_grid(blocks_for(n, kAdjBlock));
+  const uint32_t n_tiles = blocks_for(n, kAdjBlock);
+  const TileQueue tq{c.tile_q.as<uint32_t>(), n_tiles, (n_tiles + 7u) / 8u};
   {
     LaunchScope ls(&c, "adj_prep", 2.0 * 32.0 * n + 32.0 * n);
     k_adj_prep<F><<<blocks_for(n), kBlock, 0, c.stream>>>(n, c.n_moving, rec.ids, A.xb, A.vb,
@@ -416,20 +428,22 @@ void sph_reverse_step(Ctx& c, const SphStepRecord& rec, AdjBufs& A) {
   {
     const double ge = (double)sizeof(typename SphForm<F>::real);
     LaunchScope ls(&c, F == kF64 ? "adj_force_f64" : "adj_force_mixed", (8.0 * ge + 24.0 + 24.0 + 8.0 + 48.0) * n);
-    k_adj_force<F><<<grid, kAdjBlock, 0, c.stream>>>(n, c.n_moving, rec, c.grid, c.sph, u, c.desc.alpha, c.desc.c0,
-                                                 c.desc.rho0, (const typename AdjForm<F>::C4*)A.cvec, A.xbf, A.vb,
-                                                 (typename AdjForm<F>::ar*)A.rhob, A.partial);
+    static const int per_cu = blocks_per_cu(k_adj_force<F>, kAdjBlock);
+    k_adj_force<F><<<queue_grid(per_cu, c.n_cus, n_tiles), kAdjBlock, 0, c.stream>>>(
+        n, c.n_moving, rec, c.grid, c.sph, u, c.desc.alpha, c.desc.c0, c.desc.rho0,
+        (const typename AdjForm<F>::C4*)A.cvec, A.xbf, A.vb, (typename AdjForm<F>::ar*)A.rhob, A.partial, tq);
     PR_HIP(hipGetLastError());
-    k_adj_reduce<<<1, 256, 0, c.stream>>>(grid, A.partial, A.acc);
+    k_adj_reduce<<<1, 256, 0, c.stream>>>(n_tiles, A.partial, A.acc);
     PR_HIP(hipGetLastError());
   }
   {
     const double ge = (double)sizeof(typename SphForm<F>::real);
     LaunchScope ls(&c, F == kF64 ? "adj_density_f64" : "adj_density_mixed",
                    (3.0 * ge + 8.0 + 24.0 * 3 + 24.0 * 2 + 8.0) * n);
-    k_adj_density<F><<<grid, kAdjBlock, 0, c.stream>>>(n, c.n_moving, rec, c.grid, c.sph, u,
-                                                   (const typename AdjForm<F>::ar*)A.rhob, A.xb, A.xbf, A.vb,
-                                                   A.xb2, A.vb2);
+    static const int per_cu = blocks_per_cu(k_adj_density<F>, kAdjBlock);
+    k_adj_density<F><<<queue_grid(per_cu, c.n_cus, n_tiles), kAdjBlock, 0, c.stream>>>(
+        n, c.n_moving, rec, c.grid, c.sph, u, (const typename AdjForm<F>::ar*)A.rhob, A.xb, A.xbf, A.vb, A.xb2,
+        A.vb2, tq);
     PR_HIP(hipGetLastError());
   }
   std::swap(A.xb, A.xb2);
@@ -504,8 +518,7 @@ template <int F>
 void sph_adjoint(Ctx& c, const double* seed_x, const double* seed_v, double* grad_params, double* grad_v0) {
   Tape& T = c.tape;
   const uint32_t n = (uint32_t)c.n;
-  const uint32_t grid = xcd_grid(blocks_for(n, kAdjBlock));
-  ensure_adj_buffers(c, grid);
+  ensure_adj_buffers(c, blocks_for(n, kAdjBlock));   // one force partial per tile
   AdjBufs A{T.xb.as<double4>(), T.vb.as<double4>(), T.xb2.as<double4>(), T.vb2.as<double4>(),
             T.cvec.p, T.xbf.as<double4>(), T.rhob.p, T.partial.as<double2>(),
             T.acc.as<double>()};

@@ -212,6 +212,9 @@ static void allocate(Ctx& c) {
   }
   c.lb.alloc(((size_t)c.grid.n_cells + 1) * 4);
   c.flags.alloc(FLAG_COUNT * 4);
+  c.tile_q.alloc(kQueueWords * 4);
+  PR_HIP(hipMemsetAsync(c.tile_q.p, 0, kQueueWords * 4, c.stream));
+  PR_HIP(hipDeviceGetAttribute(&c.n_cus, hipDeviceAttributeMultiprocessorCount, c.device));
   PR_HIP(hipMemsetAsync(c.flags.p, 0, FLAG_COUNT * 4, c.stream));
   const size_t es = f32 || (mixed && c.desc.model == PRESTIGE_MODEL_WCSPH) ? 4 : 8;
   c.rp.alloc(nn * 2 * es);

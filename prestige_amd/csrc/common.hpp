@@ -239,6 +239,8 @@ struct Ctx {
   enum SortPolicy { kSortAuto = 0, kSortRadix = 1, kSortIncremental = 2 };
   int sort_policy = kSortAuto;   // PRESTIGE_SORT=radix|incremental overrides the choice
   DevBuf lb;                     // n_cells + 1
+  DevBuf tile_q;                 // tile-queue counters of the WCSPH gathers (kQueueWords)
+  int n_cus = 0;                 // compute units of the device
   DevBuf flags;                  // FLAG_COUNT uint32
   // gather outputs
   DevBuf rp;                     // (rho, p/rho^2) per slot: double2 or float2
@@ -291,6 +293,63 @@ inline uint32_t xcd_grid(uint32_t nb) { return ((nb + 7u) / 8u) * 8u; }
 __device__ inline uint32_t xcd_tile(uint32_t b, uint32_t grid) {
   uint32_t per = grid / 8u;
   return (b % 8u) * per + b / 8u;
+}
+
+// Dynamic form of the same split for the WCSPH gathers, whose tiles cost very
+// different amounts (the top and bottom of a tank are wall particles only):
+// XCD q owns tiles [q per, (q+1) per) and its wavefronts take them in order
+// from a counter; a wavefront whose range is used up takes tiles from the
+// other ranges in turn, so no XCD idles while another still has work.  A tile
+// is claimed only when the wavefront is free to start it (a claim held during
+// the current tile would keep it from idle wavefronts at the end).  The grid
+// is at most the resident wavefronts, each looping over tiles; the last
+// wavefront to finish clears the counters for the next launch (they start at
+// zero when allocated).  Launches sharing the counters must be stream-ordered.
+constexpr uint32_t kQueueStride = 32;   // one 128-byte line per counter
+constexpr uint32_t kQueueWords = 9 * kQueueStride;
+struct TileQueue {
+  uint32_t* ctr;      // range heads at ctr[q * kQueueStride], finished wavefronts at ctr[8 * kQueueStride]
+  uint32_t n_tiles;   // tiles of one wavefront's slots
+  uint32_t per;       // tiles per XCD range
+};
+
+// Blocks of `block` threads of `kernel` one compute unit holds at once.
+template <class K>
+int blocks_per_cu(K kernel, int block) {
+  int n = 0;
+  PR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, block, 0));
+  return n;
+}
+
+// Grid of a tile-queue launch: the wavefronts the device holds at once
+// (blocks_per_cu from the occupancy of the kernel), a multiple of 8, never
+// more than the tiles need.
+inline uint32_t queue_grid(int blocks_per_cu, int n_cus, uint32_t n_tiles) {
+  const uint32_t resident = (uint32_t)(blocks_per_cu > 0 ? blocks_per_cu : 1) * (uint32_t)(n_cus > 0 ? n_cus : 1);
+  const uint32_t cap = resident >= 8u ? resident / 8u * 8u : 8u;
+  const uint32_t need = xcd_grid(n_tiles);
+  return need < cap ? need : cap;
+}
+
+// body(tile) for every tile, each exactly once over the grid; one wavefront
+// per block, every lane calls.
+template <class Body>
+__device__ inline void for_each_tile(const TileQueue& tq, Body&& body) {
+  uint32_t q = blockIdx.x & 7u;
+  for (uint32_t tries = 0; tries < 8u;) {
+    uint32_t t = 0;
+    if (__lane_id() == 0u) t = atomicAdd(&tq.ctr[q * kQueueStride], 1u);
+    t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+    if (t >= tq.per) {   // this range is used up: try the next one
+      q = (q + 1u) & 7u;
+      ++tries;
+      continue;
+    }
+    const uint32_t tile = q * tq.per + t;
+    if (tile < tq.n_tiles) body(tile);
+  }
+  if (__lane_id() == 0u && atomicAdd(&tq.ctr[8 * kQueueStride], 1u) == gridDim.x - 1u)
+    for (uint32_t k = 0; k <= 8u; ++k) __hip_atomic_store(&tq.ctr[k * kQueueStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // The FP32 probe's .w carries the particle's cell coordinates modulo 4 (2

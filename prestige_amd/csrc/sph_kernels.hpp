@@ -520,171 +520,176 @@ __device__ inline void walk_runs_staged(const uint2* __restrict__ meta, const ui
 // (a wavefront spread over distant rows) is scanned from global memory.
 // ---------------------------------------------------------------------------
 constexpr int kDensBlock = 64;  // one wavefront: block barriers are wave barriers
+// Wavefronts per SIMD the register budget is held to (the occupancy each
+// form reached before the tile loop, which lets the compiler keep more live).
+constexpr int kDensWaves = 8;
 #ifndef PRESTIGE_STAGE_CAP
 #define PRESTIGE_STAGE_CAP 256
 #endif
 constexpr uint32_t kStageCap = PRESTIGE_STAGE_CAP;
 
 template <int F>
-__global__ __launch_bounds__(kDensBlock) void k_sph_density(
+__global__ __launch_bounds__(kDensBlock, kDensWaves) void k_sph_density(
     uint32_t n, const typename SphForm<F>::S4* __restrict__ spos, const typename SphForm<F>::G4* pos,
     const uint32_t* __restrict__ skeys, const uint32_t* __restrict__ lb, Grid g, SphConst kc, SphUnits u,
     uint2* __restrict__ meta, uint32_t* __restrict__ words, typename SphForm<F>::R2* __restrict__ rp,
     typename SphForm<F>::G4* wpos, typename SphForm<F>::G4* wvel,
-    typename SphForm<F>::real* __restrict__ pres, uint32_t* flags) {
+    typename SphForm<F>::real* __restrict__ pres, uint32_t* flags, TileQueue tq) {
   using real = typename SphForm<F>::real;
   using G4 = typename SphForm<F>::G4;
   using S4 = typename SphForm<F>::S4;
   __shared__ float4 s_q[kStageCap + kScanUnr];
   const uint32_t lane = threadIdx.x;
-  const uint32_t wbase = xcd_tile(blockIdx.x, gridDim.x) * kDensBlock;
-  const uint32_t i = wbase + lane;
-  const bool live = i < n;
-  // reference point of the wavefront (n >= 1 whenever the kernel runs)
-  const S4 p0 = spos[wbase < n ? wbase : n - 1];
-  const double xr = (double)p0.x, yr = (double)p0.y, zr = (double)p0.z, ih = u.scan_inv_h;
-  auto offset = [&](const S4& p) {
-    return make_float4((float)(((double)p.x - xr) * ih), (float)(((double)p.y - yr) * ih),
-                       (float)(((double)p.z - zr) * ih), 0.0f);
-  };
-  ScanPoint sp{0.0f, 0.0f, 0.0f, u.H2_probe};
-  CellRows cr{};
-  G4 pi{};
-  if (live) {
-    const float4 o = offset(spos[i]);
-    sp.x = o.x;
-    sp.y = o.y;
-    sp.z = o.z;
-    cr = decode_cell(skeys[i], g);
-    pi = pos[i];
-  }
-  // dot-form scan of staged runs: a = -2 s, and |s|^2 for the threshold
-  const float ax = -2.0f * sp.x, ay = -2.0f * sp.y, az = -2.0f * sp.z;
-  const float ss = __builtin_fmaf(sp.x, sp.x, __builtin_fmaf(sp.y, sp.y, sp.z * sp.z));
-  const float ss_max = wave_max_pos(ss);
-  const real H2 = (real)u.H2, inv_h = (real)u.inv_h;
-  Sum<real, F == kMixed> acc;
-  bool overflow = false;
-  const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
-  // runs in the WCSPH order, each scanned and walked before the next (a
-  // group's masks held together would cost registers, and this pass's walk
-  // reads global memory, which wants occupancy; the two-level loop also
-  // compiles to fewer registers than one loop of nine)
-  for (uint32_t kz = 0; kz < 3u; ++kz)
-  for (uint32_t ky = 0; ky < 3u; ++ky) {
-    const uint32_t r = sph_run(3u * kz + ky);
-    const int zz = cr.cz + (int)(r / 3u) - 1, yy = cr.cy + (int)(r % 3u) - 1;
-    const bool row_ok = live && zz >= 0 && zz < nz && yy >= 0 && yy < ny;
-    uint32_t j0 = 0, len = 0;
-    if (row_ok) {
-      const uint32_t row = ((uint32_t)zz * (uint32_t)ny + (uint32_t)yy) * (uint32_t)nx;
-      j0 = lb[row + (uint32_t)cr.x0];
-      len = lb[row + (uint32_t)cr.x1 + 1] - j0;
-      if (len > 128u) {
-        overflow = true;
-        len = 128u;
-      }
+  for_each_tile(tq, [&](uint32_t tile) {
+    const uint32_t wbase = tile * kDensBlock;
+    const uint32_t i = wbase + lane;
+    const bool live = i < n;
+    // reference point of the wavefront (n >= 1 whenever the kernel runs)
+    const S4 p0 = spos[wbase < n ? wbase : n - 1];
+    const double xr = (double)p0.x, yr = (double)p0.y, zr = (double)p0.z, ih = u.scan_inv_h;
+    auto offset = [&](const S4& p) {
+      return make_float4((float)(((double)p.x - xr) * ih), (float)(((double)p.y - yr) * ih),
+                         (float)(((double)p.z - zr) * ih), 0.0f);
+    };
+    ScanPoint sp{0.0f, 0.0f, 0.0f, u.H2_probe};
+    CellRows cr{};
+    G4 pi{};
+    if (live) {
+      const float4 o = offset(spos[i]);
+      sp.x = o.x;
+      sp.y = o.y;
+      sp.z = o.z;
+      cr = decode_cell(skeys[i], g);
+      pi = pos[i];
     }
-    if (live) meta[meta_index(r, n, i)] = make_uint2(j0, len);
-    // union of the wavefront's candidate ranges of this run
-    const uint32_t base = wave_min(len ? j0 : 0xFFFFFFFFu);
-    const uint32_t top = wave_max(len ? j0 + len : 0u);
-    if (top <= base) continue;
-    const bool staged = top - base <= kStageCap;
-    float t = 0.0f;
-    if (staged) {
-      float mq = 0.0f;
-      for (uint32_t kk = lane; kk < top - base; kk += kDensBlock) {
-        float4 q = offset(spos[base + kk]);
-        q.w = __builtin_fmaf(q.x, q.x, __builtin_fmaf(q.y, q.y, q.z * q.z));
-        s_q[kk] = q;
-        mq = q.w > mq ? q.w : mq;
-      }
-      // Rounding of the dot form with |s|, |q| <= M: |q|^2 (3 roundings),
-      // the three FMAs (partial sums up to 3 M^2), |s|^2 and the threshold,
-      // and the FP32 offsets themselves (2 u M per component on a pair vector
-      // of length < 2) stay below u (22 M^2 + 14 M + 5); the margin is
-      // 32 u (M^2 + M + 1), u = 2^-24, on top of the common probe margin.
-      const float m2 = fmaxf(ss_max, wave_max_pos(mq));
-      const float h2p = u.H2_probe + 0x1.0p-19f * (m2 + __builtin_sqrtf(m2) + 1.0f);
-      t = h2p - ss;
-      __syncthreads();
-    }
-    if (len) {
-      uint64_t m0, m1;
-      if (staged) {
-        scan_run([&](uint32_t k0, uint32_t cnt) {
-                   return scan_word_dot([&](uint32_t j) { return s_q[j - base]; }, k0, cnt, ax, ay, az, t);
-                 },
-                 j0, len, r, words, n, i, m0, m1);
-      } else {
-        scan_run([&](uint32_t k0, uint32_t cnt) {
-                   return scan_word([&](uint32_t j) { return offset(spos[j < n ? j : n - 1]); }, k0, cnt, sp);
-                 },
-                 j0, len, r, words, n, i, m0, m1);
-      }
-      walk_bits<kDensUnr>(m0, m1, [&](const uint32_t* jj) {
-        G4 pj[kDensUnr];
-#pragma unroll
-        for (int v = 0; v < kDensUnr; ++v) pj[v] = pos[j0 + (jj[v] == kPad ? jj[0] : jj[v])];
-#pragma unroll
-        for (int v = 0; v < kDensUnr; ++v) {
-          if (jj[v] == kPad) continue;
-          real dx, dy, dz;
-          form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
-          real r2 = dx * dx + dy * dy + dz * dz;
-          if (r2 < H2) {
-            // the self pair (and coincident slots) has r2 = 0, q = 0
-            real q = r2 > real(0.0) ? sqrt_pos(r2) * inv_h : real(0.0);
-            acc.add(spline_f(q));
-          }
+    // dot-form scan of staged runs: a = -2 s, and |s|^2 for the threshold
+    const float ax = -2.0f * sp.x, ay = -2.0f * sp.y, az = -2.0f * sp.z;
+    const float ss = __builtin_fmaf(sp.x, sp.x, __builtin_fmaf(sp.y, sp.y, sp.z * sp.z));
+    const float ss_max = wave_max_pos(ss);
+    const real H2 = (real)u.H2, inv_h = (real)u.inv_h;
+    Sum<real, F == kMixed> acc;
+    bool overflow = false;
+    const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
+    // runs in the WCSPH order, each scanned and walked before the next (a
+    // group's masks held together would cost registers, and this pass's walk
+    // reads global memory, which wants occupancy; the two-level loop also
+    // compiles to fewer registers than one loop of nine)
+    for (uint32_t kz = 0; kz < 3u; ++kz)
+    for (uint32_t ky = 0; ky < 3u; ++ky) {
+      const uint32_t r = sph_run(3u * kz + ky);
+      const int zz = cr.cz + (int)(r / 3u) - 1, yy = cr.cy + (int)(r % 3u) - 1;
+      const bool row_ok = live && zz >= 0 && zz < nz && yy >= 0 && yy < ny;
+      uint32_t j0 = 0, len = 0;
+      if (row_ok) {
+        const uint32_t row = ((uint32_t)zz * (uint32_t)ny + (uint32_t)yy) * (uint32_t)nx;
+        j0 = lb[row + (uint32_t)cr.x0];
+        len = lb[row + (uint32_t)cr.x1 + 1] - j0;
+        if (len > 128u) {
+          overflow = true;
+          len = 128u;
         }
-      });
+      }
+      if (live) meta[meta_index(r, n, i)] = make_uint2(j0, len);
+      // union of the wavefront's candidate ranges of this run
+      const uint32_t base = wave_min(len ? j0 : 0xFFFFFFFFu);
+      const uint32_t top = wave_max(len ? j0 + len : 0u);
+      if (top <= base) continue;
+      const bool staged = top - base <= kStageCap;
+      float t = 0.0f;
+      if (staged) {
+        float mq = 0.0f;
+        for (uint32_t kk = lane; kk < top - base; kk += kDensBlock) {
+          float4 q = offset(spos[base + kk]);
+          q.w = __builtin_fmaf(q.x, q.x, __builtin_fmaf(q.y, q.y, q.z * q.z));
+          s_q[kk] = q;
+          mq = q.w > mq ? q.w : mq;
+        }
+        // Rounding of the dot form with |s|, |q| <= M: |q|^2 (3 roundings),
+        // the three FMAs (partial sums up to 3 M^2), |s|^2 and the threshold,
+        // and the FP32 offsets themselves (2 u M per component on a pair vector
+        // of length < 2) stay below u (22 M^2 + 14 M + 5); the margin is
+        // 32 u (M^2 + M + 1), u = 2^-24, on top of the common probe margin.
+        const float m2 = fmaxf(ss_max, wave_max_pos(mq));
+        const float h2p = u.H2_probe + 0x1.0p-19f * (m2 + __builtin_sqrtf(m2) + 1.0f);
+        t = h2p - ss;
+        __syncthreads();
+      }
+      if (len) {
+        uint64_t m0, m1;
+        if (staged) {
+          scan_run([&](uint32_t k0, uint32_t cnt) {
+                     return scan_word_dot([&](uint32_t j) { return s_q[j - base]; }, k0, cnt, ax, ay, az, t);
+                   },
+                   j0, len, r, words, n, i, m0, m1);
+        } else {
+          scan_run([&](uint32_t k0, uint32_t cnt) {
+                     return scan_word([&](uint32_t j) { return offset(spos[j < n ? j : n - 1]); }, k0, cnt, sp);
+                   },
+                   j0, len, r, words, n, i, m0, m1);
+        }
+        walk_bits<kDensUnr>(m0, m1, [&](const uint32_t* jj) {
+          G4 pj[kDensUnr];
+  #pragma unroll
+          for (int v = 0; v < kDensUnr; ++v) pj[v] = pos[j0 + (jj[v] == kPad ? jj[0] : jj[v])];
+  #pragma unroll
+          for (int v = 0; v < kDensUnr; ++v) {
+            if (jj[v] == kPad) continue;
+            real dx, dy, dz;
+            form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
+            real r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < H2) {
+              // the self pair (and coincident slots) has r2 = 0, q = 0
+              real q = r2 > real(0.0) ? sqrt_pos(r2) * inv_h : real(0.0);
+              acc.add(spline_f(q));
+            }
+          }
+        });
+      }
+      if (staged) __syncthreads();  // the next run's staging overwrites s_q
     }
-    if (staged) __syncthreads();  // the next run's staging overwrites s_q
-  }
-  if (!live) return;
-  if (overflow) atomicOr(&flags[FLAG_CAPACITY], 1u);
-  if constexpr (F == kF64) {
-    double rho = kc.msig * acc.s;
-    double ratio = rho * kc.inv_rho0;
-    double r2_ = ratio * ratio;
-    double r4 = r2_ * r2_;
-    double r7 = r4 * r2_ * ratio;
-    double p = kc.B * (r7 - 1.0);
-    if (p < 0.0) p = 0.0;
-    const double pr = p / (rho * rho);
-    rp[i] = make_double2(rho, pr);
-    reinterpret_cast<double*>(wpos + i)[3] = rho;
-    reinterpret_cast<double*>(wvel + i)[3] = pr;
-    pres[i] = p;
-  } else if constexpr (F == kF32) {
-    float rho = (float)kc.msig * acc.s;
-    float ratio = rho * (float)kc.inv_rho0;
-    float r2_ = ratio * ratio;
-    float r4 = r2_ * r2_;
-    float r7 = r4 * r2_ * ratio;
-    float p = (float)kc.B * (r7 - 1.0f);
-    if (p < 0.0f) p = 0.0f;
-    const float pr = p / (rho * rho);
-    rp[i] = make_float2(rho, pr);
-    reinterpret_cast<float*>(wpos + i)[3] = rho;
-    reinterpret_cast<float*>(wvel + i)[3] = pr;
-    pres[i] = p;
-  } else {
-    // EOS in FP64 from the compensated sum: gamma = 7 amplifies the relative
-    // error of rho by 7 rho^7 / (rho^7 - rho0^7), so rho is never rounded to
-    // FP32 before p is formed.
-    double rho = kc.msig * acc.wide();
-    double ratio = rho * kc.inv_rho0;
-    double r2_ = ratio * ratio;
-    double r4 = r2_ * r2_;
-    double r7 = r4 * r2_ * ratio;
-    double p = kc.B * (r7 - 1.0);
-    if (p < 0.0) p = 0.0;
-    rp[i] = make_float2((float)rho, (float)(p / (rho * rho)));
-    pres[i] = (float)p;
-  }
+    if (!live) return;
+    if (overflow) atomicOr(&flags[FLAG_CAPACITY], 1u);
+    if constexpr (F == kF64) {
+      double rho = kc.msig * acc.s;
+      double ratio = rho * kc.inv_rho0;
+      double r2_ = ratio * ratio;
+      double r4 = r2_ * r2_;
+      double r7 = r4 * r2_ * ratio;
+      double p = kc.B * (r7 - 1.0);
+      if (p < 0.0) p = 0.0;
+      const double pr = p / (rho * rho);
+      rp[i] = make_double2(rho, pr);
+      reinterpret_cast<double*>(wpos + i)[3] = rho;
+      reinterpret_cast<double*>(wvel + i)[3] = pr;
+      pres[i] = p;
+    } else if constexpr (F == kF32) {
+      float rho = (float)kc.msig * acc.s;
+      float ratio = rho * (float)kc.inv_rho0;
+      float r2_ = ratio * ratio;
+      float r4 = r2_ * r2_;
+      float r7 = r4 * r2_ * ratio;
+      float p = (float)kc.B * (r7 - 1.0f);
+      if (p < 0.0f) p = 0.0f;
+      const float pr = p / (rho * rho);
+      rp[i] = make_float2(rho, pr);
+      reinterpret_cast<float*>(wpos + i)[3] = rho;
+      reinterpret_cast<float*>(wvel + i)[3] = pr;
+      pres[i] = p;
+    } else {
+      // EOS in FP64 from the compensated sum: gamma = 7 amplifies the relative
+      // error of rho by 7 rho^7 / (rho^7 - rho0^7), so rho is never rounded to
+      // FP32 before p is formed.
+      double rho = kc.msig * acc.wide();
+      double ratio = rho * kc.inv_rho0;
+      double r2_ = ratio * ratio;
+      double r4 = r2_ * r2_;
+      double r7 = r4 * r2_ * ratio;
+      double p = kc.B * (r7 - 1.0);
+      if (p < 0.0) p = 0.0;
+      rp[i] = make_float2((float)rho, (float)(p / (rho * rho)));
+      pres[i] = (float)p;
+    }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -704,6 +709,8 @@ __global__ __launch_bounds__(kDensBlock) void k_sph_density(
 // global memory.
 // ---------------------------------------------------------------------------
 constexpr int kForceBlock = 64;  // one wavefront: block barriers are wave barriers
+template <int F>
+constexpr int kForceWaves = F == kF64 ? 4 : (F == kF32 ? 6 : 5);
 #ifndef PRESTIGE_FORCE_CAP
 #define PRESTIGE_FORCE_CAP 160
 #endif
@@ -750,7 +757,7 @@ __device__ inline bool force_pair(const typename SphForm<F>::G4& pi, const typen
 }
 
 template <int F>
-__global__ __launch_bounds__(kForceBlock) void k_sph_force(
+__global__ __launch_bounds__(kForceBlock, kForceWaves<F>) void k_sph_force(
     uint32_t n, uint64_t n_moving, const typename SphForm<F>::G4* __restrict__ gpos,
     const typename SphForm<F>::G4* __restrict__ gvel, const typename SphForm<F>::R2* __restrict__ rp,
     const uint32_t* __restrict__ ids, const uint32_t* __restrict__ skeys,
@@ -758,7 +765,7 @@ __global__ __launch_bounds__(kForceBlock) void k_sph_force(
     SphUnits u, const typename SphForm<F>::S4* __restrict__ spos,
     const typename SphForm<F>::S4* __restrict__ svel, typename SphForm<F>::S4* __restrict__ opos,
     typename SphForm<F>::S4* __restrict__ ovel, uint32_t* __restrict__ okeys,
-    typename SphForm<F>::S4* __restrict__ oacc, MovedOut moved, uint32_t* flags) {
+    typename SphForm<F>::S4* __restrict__ oacc, MovedOut moved, uint32_t* flags, TileQueue tq) {
   using real = typename SphForm<F>::real;
   using G4 = typename SphForm<F>::G4;
   using R2 = typename SphForm<F>::R2;
@@ -767,127 +774,129 @@ __global__ __launch_bounds__(kForceBlock) void k_sph_force(
   __shared__ G4 s_p[kForceCap], s_v[kForceCap];
   __shared__ R2 s_rp[F == kMixed ? kForceCap : 1];
   const uint32_t lane = threadIdx.x;
-  const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kForceBlock + lane;
-  const bool live = i < n && (uint64_t)ids[i] < n_moving;
-  // (rho, p/rho^2) of a slot: the density pass also left them in the .w
-  // lanes of the FP64 / FP32 gather copies
-  G4 pi{}, vi{};
-  real rhoi = real(0.0), pri = real(0.0);
-  if (live) {
-    pi = gpos[i];
-    vi = gvel[i];
-    if constexpr (F == kMixed) {
-      const R2 rpi = rp[i];
-      rhoi = rpi.x;
-      pri = rpi.y;
-    } else {
-      rhoi = pi.w;
-      pri = vi.w;
-    }
-  }
-  const real alpha_c0 = (real)kc.alpha_c0, ach = alpha_c0 * (real)u.hv;
-  Sum<real, F == kMixed> sx, sy, sz;
-  // pair bodies over a batch of neighbours fetched by get(j, p, v, rho, pr)
-  auto body = [&](const uint32_t* jj, auto&& get) {
-    G4 pj[kUnr], vj[kUnr];
-    real rhoj[kUnr], prj[kUnr];
-#pragma unroll
-    for (int v = 0; v < kUnr; ++v) get(jj[v] == kPad ? jj[0] : jj[v], pj[v], vj[v], rhoj[v], prj[v]);
-#pragma unroll
-    for (int v = 0; v < kUnr; ++v) {
-      if (jj[v] == kPad) continue;
-      real rx, ry, rz;
-      if (force_pair<F>(pi, vi, rhoi, pri, pj[v], vj[v], rhoj[v], prj[v], u, ach, rx, ry, rz)) {
-        sx.add(rx);
-        sy.add(ry);
-        sz.add(rz);
+  for_each_tile(tq, [&](uint32_t tile) {
+    const uint32_t i = tile * kForceBlock + lane;
+    const bool live = i < n && (uint64_t)ids[i] < n_moving;
+    // (rho, p/rho^2) of a slot: the density pass also left them in the .w
+    // lanes of the FP64 / FP32 gather copies
+    G4 pi{}, vi{};
+    real rhoi = real(0.0), pri = real(0.0);
+    if (live) {
+      pi = gpos[i];
+      vi = gvel[i];
+      if constexpr (F == kMixed) {
+        const R2 rpi = rp[i];
+        rhoi = rpi.x;
+        pri = rpi.y;
+      } else {
+        rhoi = pi.w;
+        pri = vi.w;
       }
     }
-  };
-  walk_runs_staged<kUnr, kForceCap>(
-      meta, words, n, i, live, lane,
-      [&](uint32_t k, uint32_t j) {
-        s_p[k] = gpos[j];
-        s_v[k] = gvel[j];
-        if constexpr (F == kMixed) s_rp[k] = rp[j];
-      },
-      [&](const uint32_t* jj) {
-        body(jj, [&](uint32_t k, G4& p, G4& v, real& rho, real& pr) {
-          p = s_p[k];
-          v = s_v[k];
-          if constexpr (F == kMixed) {
-            const R2 t = s_rp[k];
-            rho = t.x;
-            pr = t.y;
-          } else {
-            rho = p.w;
-            pr = v.w;
-          }
-        });
-      },
-      [&](const uint32_t* jj) {
-        body(jj, [&](uint32_t j, G4& p, G4& v, real& rho, real& pr) {
-          p = gpos[j];
-          v = gvel[j];
-          if constexpr (F == kMixed) {
-            const R2 t = rp[j];
-            rho = t.x;
-            pr = t.y;
-          } else {
-            rho = p.w;
-            pr = v.w;
-          }
-        });
-      },
-      [] {});
-  if (i >= n) return;
-  const uint64_t live_mask = __ballot(live);
-  if (!live) {
-    opos[i] = spos[i];
-    ovel[i] = svel[i];
-    okeys[i] = skeys[i];
-    if (oacc) oacc[i] = S4{0, 0, 0, 0};
-    moved_word_fixed(moved, i, live_mask);
-    return;
-  }
-  const S4 xs = spos[i];
-  const S4 vs = svel[i];
-  sreal ax, ay, az;
-  if constexpr (F == kF32) {
-    const float msh = (float)kc.msh;
-    ax = (float)kc.g[0] - msh * sx.s;
-    ay = (float)kc.g[1] - msh * sy.s;
-    az = (float)kc.g[2] - msh * sz.s;
-  } else {
-    ax = kc.g[0] - kc.msh * sx.wide();
-    ay = kc.g[1] - kc.msh * sy.wide();
-    az = kc.g[2] - kc.msh * sz.wide();
-  }
-  const sreal dt = (sreal)kc.dt;
-  S4 v2 = vs, x2 = xs;
-  v2.x = vs.x + dt * ax;
-  v2.y = vs.y + dt * ay;
-  v2.z = vs.z + dt * az;
-  x2.x = xs.x + dt * v2.x;
-  x2.y = xs.y + dt * v2.y;
-  x2.z = xs.z + dt * v2.z;
-  opos[i] = x2;
-  ovel[i] = v2;
-  if (oacc) oacc[i] = S4{ax, ay, az, 0};
-  if (!isfinite(ax) || !isfinite(ay) || !isfinite(az)) atomicOr(&flags[FLAG_NONFINITE], 1u);
-  // next step's cell key, same FP64 floor as the key kernel
-  double fx = floor(((double)x2.x - g.lo[0]) * g.inv_cs);
-  double fy = floor(((double)x2.y - g.lo[1]) * g.inv_cs);
-  double fz = floor(((double)x2.z - g.lo[2]) * g.inv_cs);
-  bool ok = fx >= 0.0 && fy >= 0.0 && fz >= 0.0 && fx < (double)g.dims[0] &&
-            fy < (double)g.dims[1] && fz < (double)g.dims[2];
-  uint32_t key = 0;
-  if (ok)
-    key = ((uint32_t)fz * (uint32_t)g.dims[1] + (uint32_t)fy) * (uint32_t)g.dims[0] + (uint32_t)fx;
-  else
-    atomicOr(&flags[FLAG_DOMAIN], 1u);
-  okeys[i] = key;
-  moved_word_live(moved, i, key != skeys[i], live_mask);
+    const real alpha_c0 = (real)kc.alpha_c0, ach = alpha_c0 * (real)u.hv;
+    Sum<real, F == kMixed> sx, sy, sz;
+    // pair bodies over a batch of neighbours fetched by get(j, p, v, rho, pr)
+    auto body = [&](const uint32_t* jj, auto&& get) {
+      G4 pj[kUnr], vj[kUnr];
+      real rhoj[kUnr], prj[kUnr];
+  #pragma unroll
+      for (int v = 0; v < kUnr; ++v) get(jj[v] == kPad ? jj[0] : jj[v], pj[v], vj[v], rhoj[v], prj[v]);
+  #pragma unroll
+      for (int v = 0; v < kUnr; ++v) {
+        if (jj[v] == kPad) continue;
+        real rx, ry, rz;
+        if (force_pair<F>(pi, vi, rhoi, pri, pj[v], vj[v], rhoj[v], prj[v], u, ach, rx, ry, rz)) {
+          sx.add(rx);
+          sy.add(ry);
+          sz.add(rz);
+        }
+      }
+    };
+    walk_runs_staged<kUnr, kForceCap>(
+        meta, words, n, i, live, lane,
+        [&](uint32_t k, uint32_t j) {
+          s_p[k] = gpos[j];
+          s_v[k] = gvel[j];
+          if constexpr (F == kMixed) s_rp[k] = rp[j];
+        },
+        [&](const uint32_t* jj) {
+          body(jj, [&](uint32_t k, G4& p, G4& v, real& rho, real& pr) {
+            p = s_p[k];
+            v = s_v[k];
+            if constexpr (F == kMixed) {
+              const R2 t = s_rp[k];
+              rho = t.x;
+              pr = t.y;
+            } else {
+              rho = p.w;
+              pr = v.w;
+            }
+          });
+        },
+        [&](const uint32_t* jj) {
+          body(jj, [&](uint32_t j, G4& p, G4& v, real& rho, real& pr) {
+            p = gpos[j];
+            v = gvel[j];
+            if constexpr (F == kMixed) {
+              const R2 t = rp[j];
+              rho = t.x;
+              pr = t.y;
+            } else {
+              rho = p.w;
+              pr = v.w;
+            }
+          });
+        },
+        [] {});
+    if (i >= n) return;
+    const uint64_t live_mask = __ballot(live);
+    if (!live) {
+      opos[i] = spos[i];
+      ovel[i] = svel[i];
+      okeys[i] = skeys[i];
+      if (oacc) oacc[i] = S4{0, 0, 0, 0};
+      moved_word_fixed(moved, i, live_mask);
+      return;
+    }
+    const S4 xs = spos[i];
+    const S4 vs = svel[i];
+    sreal ax, ay, az;
+    if constexpr (F == kF32) {
+      const float msh = (float)kc.msh;
+      ax = (float)kc.g[0] - msh * sx.s;
+      ay = (float)kc.g[1] - msh * sy.s;
+      az = (float)kc.g[2] - msh * sz.s;
+    } else {
+      ax = kc.g[0] - kc.msh * sx.wide();
+      ay = kc.g[1] - kc.msh * sy.wide();
+      az = kc.g[2] - kc.msh * sz.wide();
+    }
+    const sreal dt = (sreal)kc.dt;
+    S4 v2 = vs, x2 = xs;
+    v2.x = vs.x + dt * ax;
+    v2.y = vs.y + dt * ay;
+    v2.z = vs.z + dt * az;
+    x2.x = xs.x + dt * v2.x;
+    x2.y = xs.y + dt * v2.y;
+    x2.z = xs.z + dt * v2.z;
+    opos[i] = x2;
+    ovel[i] = v2;
+    if (oacc) oacc[i] = S4{ax, ay, az, 0};
+    if (!isfinite(ax) || !isfinite(ay) || !isfinite(az)) atomicOr(&flags[FLAG_NONFINITE], 1u);
+    // next step's cell key, same FP64 floor as the key kernel
+    double fx = floor(((double)x2.x - g.lo[0]) * g.inv_cs);
+    double fy = floor(((double)x2.y - g.lo[1]) * g.inv_cs);
+    double fz = floor(((double)x2.z - g.lo[2]) * g.inv_cs);
+    bool ok = fx >= 0.0 && fy >= 0.0 && fz >= 0.0 && fx < (double)g.dims[0] &&
+              fy < (double)g.dims[1] && fz < (double)g.dims[2];
+    uint32_t key = 0;
+    if (ok)
+      key = ((uint32_t)fz * (uint32_t)g.dims[1] + (uint32_t)fy) * (uint32_t)g.dims[0] + (uint32_t)fx;
+    else
+      atomicOr(&flags[FLAG_DOMAIN], 1u);
+    okeys[i] = key;
+    moved_word_live(moved, i, key != skeys[i], live_mask);
+  });
 }
 
 template <int F>
@@ -945,6 +954,9 @@ void sph_step_impl(Ctx& c) {
   // .w lanes of the FP64 / FP32 gather copies receive (rho, p/rho^2)
   G4* wpos = (F == kMixed) ? nullptr : c.pos[c.cur].as<G4>();
   G4* wvel = (F == kMixed) ? nullptr : c.vel[c.cur].as<G4>();
+  static_assert(kDensBlock == kForceBlock, "both gathers walk the same tiles");
+  const uint32_t n_tiles = blocks_for(n, kDensBlock);
+  const TileQueue tq{c.tile_q.as<uint32_t>(), n_tiles, (n_tiles + 7u) / 8u};
   const double se = (F == kF32) ? 4.0 : 8.0;   // bytes per state scalar
   const double ge = (double)sizeof(real);     // bytes per scalar of the pair arithmetic
   {
@@ -953,11 +965,16 @@ void sph_step_impl(Ctx& c) {
     double per = (F == kMixed ? 12.0 : 3.0 * ge) + 3.0 * ge;
     LaunchScope ls(&c, F == kF64 ? "sph_density_f64" : (F == kF32 ? "sph_density_f32" : "sph_density_mixed"),
                    per * n);
-    const uint32_t dgrid = xcd_grid(blocks_for(n, kDensBlock));
+#ifdef PRESTIGE_DENS_PER_CU
+    static const int per_cu = PRESTIGE_DENS_PER_CU;
+#else
+    static const int per_cu = blocks_per_cu(k_sph_density<F>, kDensBlock);
+#endif
+    const uint32_t dgrid = queue_grid(per_cu, c.n_cus, tq.n_tiles);
     k_sph_density<F><<<dgrid, kDensBlock, 0, c.stream>>>(n, c.pos[c.cur].as<S4>(), gpos, skeys, c.lb.as<uint32_t>(),
                                                         c.grid, c.sph, u, c.nmeta.as<uint2>(),
                                                         c.nword.as<uint32_t>(), c.rp.as<R2>(), wpos, wvel,
-                                                        c.pres.as<real>(), c.flags.as<uint32_t>());
+                                                        c.pres.as<real>(), c.flags.as<uint32_t>(), tq);
     PR_HIP(hipGetLastError());
   }
   {
@@ -969,12 +986,18 @@ void sph_step_impl(Ctx& c) {
     LaunchScope ls(&c, F == kF64 ? "sph_force_f64" : (F == kF32 ? "sph_force_f32" : "sph_force_mixed"), per * nm);
     const int dst = c.cur ^ 1;
     const int kdst = c.cur_key ^ 1;
-    const uint32_t fgrid = xcd_grid(blocks_for(n, kForceBlock));
+#ifdef PRESTIGE_FORCE_PER_CU
+    static const int per_cu = PRESTIGE_FORCE_PER_CU;
+#else
+    static const int per_cu = blocks_per_cu(k_sph_force<F>, kForceBlock);
+#endif
+    const uint32_t fgrid = queue_grid(per_cu, c.n_cus, tq.n_tiles);
     k_sph_force<F><<<fgrid, kForceBlock, 0, c.stream>>>(
         n, c.n_moving, gpos, gvel, c.rp.as<R2>(), c.ids[c.cur_ids].as<uint32_t>(), skeys,
         c.nmeta.as<uint2>(), c.nword.as<uint32_t>(), c.grid, c.sph, u, c.pos[c.cur].as<S4>(), c.vel[c.cur].as<S4>(),
         c.pos[dst].as<S4>(), c.vel[dst].as<S4>(), c.keys[kdst].as<uint32_t>(),
-        c.keep_acc ? c.acc.as<S4>() : nullptr, MovedOut{c.inc_mask.as<uint64_t>(), c.inc_tiles.as<uint32_t>()}, c.flags.as<uint32_t>());
+        c.keep_acc ? c.acc.as<S4>() : nullptr, MovedOut{c.inc_mask.as<uint64_t>(), c.inc_tiles.as<uint32_t>()},
+        c.flags.as<uint32_t>(), tq);
     PR_HIP(hipGetLastError());
     c.cur = dst;
     c.cur_key = kdst;
