@@ -520,16 +520,25 @@ __device__ inline void walk_runs_staged(const uint2* __restrict__ meta, const ui
 // (a wavefront spread over distant rows) is scanned from global memory.
 // ---------------------------------------------------------------------------
 constexpr int kDensBlock = 64;  // one wavefront: block barriers are wave barriers
-// Wavefronts per SIMD the register budget is held to (the occupancy each
-// form reached before the tile loop, which lets the compiler keep more live).
-constexpr int kDensWaves = 8;
+// Wavefronts per SIMD the register budget is held to: the tile loop lets the
+// compiler keep more values live, and these are the budgets that measured
+// fastest (FP64 / mixed: the occupancy before the loop; FP32: one wavefront
+// fewer than that, which avoids spills).
+#ifndef PRESTIGE_F32_DENS_WAVES
+#define PRESTIGE_F32_DENS_WAVES 7
+#endif
+#ifndef PRESTIGE_F32_FORCE_WAVES
+#define PRESTIGE_F32_FORCE_WAVES 5
+#endif
+template <int F>
+constexpr int kDensWaves = F == kF32 ? PRESTIGE_F32_DENS_WAVES : 8;
 #ifndef PRESTIGE_STAGE_CAP
 #define PRESTIGE_STAGE_CAP 256
 #endif
 constexpr uint32_t kStageCap = PRESTIGE_STAGE_CAP;
 
 template <int F>
-__global__ __launch_bounds__(kDensBlock, kDensWaves) void k_sph_density(
+__global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
     uint32_t n, const typename SphForm<F>::S4* __restrict__ spos, const typename SphForm<F>::G4* pos,
     const uint32_t* __restrict__ skeys, const uint32_t* __restrict__ lb, Grid g, SphConst kc, SphUnits u,
     uint2* __restrict__ meta, uint32_t* __restrict__ words, typename SphForm<F>::R2* __restrict__ rp,
@@ -710,7 +719,7 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves) void k_sph_density(
 // ---------------------------------------------------------------------------
 constexpr int kForceBlock = 64;  // one wavefront: block barriers are wave barriers
 template <int F>
-constexpr int kForceWaves = F == kF64 ? 4 : (F == kF32 ? 6 : 5);
+constexpr int kForceWaves = F == kF64 ? 4 : (F == kF32 ? PRESTIGE_F32_FORCE_WAVES : 5);
 #ifndef PRESTIGE_FORCE_CAP
 #define PRESTIGE_FORCE_CAP 160
 #endif
