@@ -637,18 +637,25 @@ __device__ inline uint2 wave_lower_bounds(const uint64_t* __restrict__ mov, uint
                     (uint32_t)__builtin_amdgcn_readlane((int)lo, 32));
 }
 
+// One thread per slot (a wavefront covers one mask word, so the moved slots
+// before slot i are word_off[i / 64] plus a popcount), then the moved
+// entries spread over the blocks.
 template <class T4, bool OMEGA, bool PROBE, bool MIXED>
-__global__ __launch_bounds__(kIncThreads) void k_inc_place(
+__global__ __launch_bounds__(kBlock) void k_inc_place(
     uint32_t n, uint32_t n_tiles, const uint32_t* __restrict__ knew, const uint64_t* __restrict__ mask,
-    const uint32_t* __restrict__ tile_off, const uint32_t* __restrict__ word_off, const uint64_t* __restrict__ mov,
-    const uint32_t* __restrict__ count, const uint32_t* __restrict__ lb_old, uint32_t* __restrict__ tile_cnt,
-    uint32_t* __restrict__ kout, uint32_t* __restrict__ perm, ReorderIO<T4> io) {
+    const uint32_t* __restrict__ word_off, const uint64_t* __restrict__ mov, const uint32_t* __restrict__ count,
+    const uint32_t* __restrict__ lb_old, uint32_t* __restrict__ tile_cnt, uint32_t* __restrict__ kout,
+    uint32_t* __restrict__ perm, ReorderIO<T4> io) {
+  static_assert(kBlock % 64 == 0, "wavefronts cover whole mask words");
   const uint32_t m = count[0];
-  if (threadIdx.x == 0) tile_cnt[blockIdx.x] = 0;   // for the next step's counts
-  inc_tile_walk(n, mask, tile_off[blockIdx.x], [&](uint32_t i, uint32_t before, bool moved, bool valid) {
-    const bool kept = valid && !moved;
-    const uint64_t km = __ballot(kept);
-    if (km == 0ull) return;
+  const uint32_t i = blockIdx.x * (uint32_t)kBlock + threadIdx.x;
+  const uint32_t lane = __lane_id();
+  if (i < n_tiles) tile_cnt[i] = 0;   // for the next step's counts
+  const bool valid = i < n;
+  const uint64_t word = valid ? mask[i >> 6] : 0ull;
+  const bool kept = valid && ((word >> lane) & 1ull) == 0ull;
+  const uint64_t km = __ballot(kept);
+  if (km != 0ull) {
     const uint32_t key = kept ? knew[i] : 0u;
     const uint64_t e = ((uint64_t)key << 32) | i;
     // kept slots are in (key, slot) order, so the wavefront's bounds lie
@@ -659,29 +666,32 @@ __global__ __launch_bounds__(kIncThreads) void k_inc_place(
     const uint64_t eb = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)key, last) << 32) |
                         (uint32_t)__builtin_amdgcn_readlane((int)i, last);
     const uint2 bnd = wave_lower_bounds(mov, m, ea, eb);
-    if (!kept) return;
-    uint32_t lo = bnd.x, hi = bnd.y;   // moved entries below (key, i)
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (mov[mid] < e) lo = mid + 1;
-      else hi = mid;
+    if (kept) {
+      const uint32_t before = word_off[i >> 6] + (uint32_t)__popcll(word & lanes_below());
+      uint32_t lo = bnd.x, hi = bnd.y;   // moved entries below (key, i)
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (mov[mid] < e) lo = mid + 1;
+        else hi = mid;
+      }
+      const uint32_t pos = i - before + lo;
+      if (pos < n) {   // always, when the moved counts are consistent
+        kout[pos] = key;
+        perm[pos] = i;
+        move_slot<T4, OMEGA, PROBE, MIXED>(io, i, pos, key);
+      }
     }
-    const uint32_t pos = i - before + lo;
-    if (pos >= n) return;   // never, when the moved counts are consistent
-    kout[pos] = key;
-    perm[pos] = i;
-    move_slot<T4, OMEGA, PROBE, MIXED>(io, i, pos, key);
-  });
+  }
   // moved entries, spread over the blocks
-  for (uint32_t b = threadIdx.x * n_tiles + blockIdx.x; b < m; b += kIncThreads * n_tiles) {
+  for (uint32_t b = threadIdx.x * gridDim.x + blockIdx.x; b < m; b += (uint32_t)kBlock * gridDim.x) {
     const uint64_t e = mov[b];
-    const uint32_t key = (uint32_t)(e >> 32), i = (uint32_t)e;
-    const uint32_t t = i < lb_old[key] ? lb_old[key] : lb_old[key + 1];
+    const uint32_t key = (uint32_t)(e >> 32), s = (uint32_t)e;
+    const uint32_t t = s < lb_old[key] ? lb_old[key] : lb_old[key + 1];
     const uint32_t pos = b + t - inc_moved_before(t, n, m, mask, word_off);
     if (pos >= n) continue;   // never, when the moved counts are consistent
     kout[pos] = key;
-    perm[pos] = i;
-    move_slot<T4, OMEGA, PROBE, MIXED>(io, i, pos, key);
+    perm[pos] = s;
+    move_slot<T4, OMEGA, PROBE, MIXED>(io, s, pos, key);
   }
 }
 
@@ -719,8 +729,8 @@ static void incremental_sort(Ctx& c) {
     uint32_t* perm = c.vals[0].as<uint32_t>();
     with_reorder_form(c, c.cur, c.cur ^ 1, [&](auto form, auto io) {
       using F = decltype(form);
-      k_inc_place<typename F::T, F::omega, F::probe, F::mixed><<<n_tiles, kIncThreads, 0, c.stream>>>(
-          n, n_tiles, knew, mask, tiles, woff, mov, cnt, lb_old, tile_cnt, kold, perm, io);
+      k_inc_place<typename F::T, F::omega, F::probe, F::mixed><<<blocks_for(n), kBlock, 0, c.stream>>>(
+          n, n_tiles, knew, mask, woff, mov, cnt, lb_old, tile_cnt, kold, perm, io);
     });
     PR_HIP(hipGetLastError());
   }
