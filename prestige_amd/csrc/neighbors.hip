@@ -389,9 +389,8 @@ double reorder_bytes(const Ctx& c) {
 constexpr uint32_t kRankMax = 8192;
 constexpr int kIncSortThreads = 1024;
 
-// Gather and placement walk a sort tile with kIncThreads threads, kIncItems
-// slots each: more wavefronts in flight to cover the placement's dependent
-// loads (search, then the state copy).
+// The gather walks a sort tile with kIncThreads threads, kIncItems slots
+// each.
 constexpr int kIncThreads = 1024;
 constexpr int kIncItems = kTile / kIncThreads;
 constexpr uint32_t kIncWaves = kIncThreads / 64;
