@@ -40,6 +40,7 @@ void restore_state(Ctx& c, const Snapshot& s) {
   c.cur = s.cur; c.cur_ids = s.cur_ids; c.cur_key = s.cur_key; c.cur_hist = s.cur_hist;
   c.sorted = s.sorted; c.keys_valid = s.keys_valid; c.built = s.built;
   c.inc_ready = false;   // the previous build's keys and cell table are not restored
+  c.forget_dry();
   copy_back(c, c.pos[c.cur], s.pos, n * e4);
   copy_back(c, c.vel[c.cur], s.vel, n * e4);
   copy_back(c, c.ids[c.cur_ids], s.ids, n * 4);

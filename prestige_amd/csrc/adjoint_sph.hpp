@@ -490,8 +490,18 @@ void record_sph_step(Ctx& c, StepRecord& r) {
   cp(r.ids, c.ids[c.cur_ids].p, c.n * 4);
 }
 
+// Steps taken while this is alive rebuild every particle's row masks (the
+// adjoint reads those of fixed particles too).
+struct RecordAll {
+  Ctx& c;
+  bool old;
+  explicit RecordAll(Ctx& ctx) : c(ctx), old(ctx.record_all) { c.record_all = true; }
+  ~RecordAll() { c.record_all = old; }
+};
+
 template <int F>
 void sph_tape_forward(Ctx& c, uint32_t nsteps, uint32_t every) {
+  RecordAll keep_masks(c);
   Tape& T = c.tape;
   T.clear();
   T.nsteps = nsteps;
@@ -531,6 +541,7 @@ void sph_adjoint(Ctx& c, const double* seed_x, const double* seed_v, double* gra
     const uint32_t len = (T.nsteps - s0) < T.every ? (T.nsteps - s0) : T.every;
     if (!(seg == nseg - 1 && T.last_recorded)) {
       restore_state(c, *T.ckpt[seg]);
+      RecordAll keep_masks(c);
       for (uint32_t t = 0; t < len; ++t) {
         sph_step_impl<F>(c);
         record_sph_step<F>(c, *T.rec[t]);

@@ -206,6 +206,7 @@ static void allocate(Ctx& c) {
   PR_HIP(hipHostMalloc((void**)&c.inc_host, sizeof(uint32_t), hipHostMallocMapped));
   *c.inc_host = 0;
   PR_HIP(hipHostGetDevicePointer((void**)&c.inc_host_dev, c.inc_host, 0));
+  if (const char* dr = std::getenv("PRESTIGE_DRY")) c.dry_reuse = std::string(dr) != "off";
   if (const char* sp = std::getenv("PRESTIGE_SORT")) {
     if (std::string(sp) == "radix") c.sort_policy = Ctx::kSortRadix;
     else if (std::string(sp) == "incremental") c.sort_policy = Ctx::kSortIncremental;
@@ -213,6 +214,12 @@ static void allocate(Ctx& c) {
   c.lb.alloc(((size_t)c.grid.n_cells + 1) * 4);
   c.flags.alloc(FLAG_COUNT * 4);
   c.tile_q.alloc(kQueueWords * 4);
+  if (c.desc.model == PRESTIGE_MODEL_WCSPH) {
+    c.movcell.alloc(((size_t)c.grid.n_cells + 1) * 4);
+    PR_HIP(hipMemsetAsync(c.movcell.p, 0, ((size_t)c.grid.n_cells + 1) * 4, c.stream));
+    c.dry_pass.alloc(nn * 4);
+    PR_HIP(hipMemsetAsync(c.dry_pass.p, 0, nn * 4, c.stream));
+  }
   PR_HIP(hipMemsetAsync(c.tile_q.p, 0, kQueueWords * 4, c.stream));
   PR_HIP(hipDeviceGetAttribute(&c.n_cus, hipDeviceAttributeMultiprocessorCount, c.device));
   PR_HIP(hipMemsetAsync(c.flags.p, 0, FLAG_COUNT * 4, c.stream));
@@ -360,6 +367,7 @@ static void upload(Ctx& c, const std::string& name, const double* host, uint64_t
     }
   }
   c.tape.clear();
+  c.forget_dry();
 }
 
 static void download(Ctx& c, const std::string& name, double* host, uint64_t n) {

@@ -240,6 +240,14 @@ struct Ctx {
   int sort_policy = kSortAuto;   // PRESTIGE_SORT=radix|incremental overrides the choice
   DevBuf lb;                     // n_cells + 1
   DevBuf tile_q;                 // tile-queue counters of the WCSPH gathers (kQueueWords)
+  // WCSPH wall particles far from the fluid (DryWalls): cell stamps (n_cells),
+  // last dry pass per particle id, pass counter, the pass the stamps describe,
+  // and whether steps must rebuild every particle's masks (adjoint records)
+  DevBuf movcell, dry_pass;
+  uint32_t dens_pass = 16, stamp_pass = 0;
+  bool record_all = false;
+  bool dry_reuse = true;         // PRESTIGE_DRY=off turns the reuse off (comparisons and tests)
+  void forget_dry() { dens_pass += 2; }   // state changed outside a step
   int n_cus = 0;                 // compute units of the device
   DevBuf flags;                  // FLAG_COUNT uint32
   // gather outputs

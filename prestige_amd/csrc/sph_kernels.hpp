@@ -519,6 +519,28 @@ __device__ inline void walk_runs_staged(const uint2* __restrict__ meta, const ui
 // lane scans its own range from there.  A union larger than kStageCap slots
 // (a wavefront spread over distant rows) is scanned from global memory.
 // ---------------------------------------------------------------------------
+// Wall particles far from the fluid.  A fixed particle whose 27 cells held
+// no moving particle at this pass and at the previous one has the same
+// candidates as then (fixed particles never move, and the stable sort keeps
+// their order), so its density, pressure and p/rho^2 are the previous pass's
+// bit for bit; a wavefront made only of such particles keeps them (FP64 form:
+// rho and p/rho^2 travel in the .w lanes of the state) instead of scanning.
+// movcell[c] = pass when cell c holds a moving particle (the force pass
+// stamps the next pass), dry_pass[id] = the last pass the particle was seen
+// dry.  Passes count up; an upload or a restore skips one, so nothing from
+// before it is reused.  Its row masks are not rewritten: the forward force
+// pass never walks a fixed particle's masks, and runs that record steps for
+// the adjoint keep every particle (reuse false).
+struct DryWalls {
+  const uint32_t* ids;
+  uint64_t n_moving;
+  const uint32_t* movcell;
+  uint32_t* dry_pass;
+  uint32_t pass;
+  bool stamps;   // movcell describes this pass
+  bool reuse;    // densities may be kept
+};
+
 constexpr int kDensBlock = 64;  // one wavefront: block barriers are wave barriers
 // Wavefronts per SIMD the register budget is held to: the tile loop lets the
 // compiler keep more values live, and these are the budgets that measured
@@ -537,13 +559,43 @@ constexpr int kDensWaves = F == kF32 ? PRESTIGE_F32_DENS_WAVES : 8;
 #endif
 constexpr uint32_t kStageCap = PRESTIGE_STAGE_CAP;
 
+// Tait pressure p = max(B ((rho / rho0)^7 - 1), 0), one operation order for
+// every caller.
+__device__ inline double tait_p(double rho, const SphConst& kc) {
+  const double ratio = rho * kc.inv_rho0;
+  const double r2_ = ratio * ratio;
+  const double r4 = r2_ * r2_;
+  const double r7 = r4 * r2_ * ratio;
+  double p = kc.B * (r7 - 1.0);
+  if (p < 0.0) p = 0.0;
+  return p;
+}
+
+// True when none of the 27 cells around cr held a moving particle at this
+// pass (cell stamps written by the previous step's force pass).
+__device__ inline bool dry_cells(const CellRows& cr, const Grid& g, const DryWalls& dw) {
+  const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
+  bool dry = true;
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int zz = cr.cz + dz;
+    if (zz < 0 || zz >= nz) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int yy = cr.cy + dy;
+      if (yy < 0 || yy >= ny) continue;
+      const uint32_t row = ((uint32_t)zz * (uint32_t)ny + (uint32_t)yy) * (uint32_t)nx;
+      for (int x = cr.x0; x <= cr.x1; ++x) dry &= dw.movcell[row + (uint32_t)x] != dw.pass;
+    }
+  }
+  return dry;
+}
+
 template <int F>
 __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
     uint32_t n, const typename SphForm<F>::S4* __restrict__ spos, const typename SphForm<F>::G4* pos,
     const uint32_t* __restrict__ skeys, const uint32_t* __restrict__ lb, Grid g, SphConst kc, SphUnits u,
     uint2* __restrict__ meta, uint32_t* __restrict__ words, typename SphForm<F>::R2* __restrict__ rp,
     typename SphForm<F>::G4* wpos, typename SphForm<F>::G4* wvel,
-    typename SphForm<F>::real* __restrict__ pres, uint32_t* flags, TileQueue tq) {
+    typename SphForm<F>::real* __restrict__ pres, uint32_t* flags, TileQueue tq, DryWalls dw) {
   using real = typename SphForm<F>::real;
   using G4 = typename SphForm<F>::G4;
   using S4 = typename SphForm<F>::S4;
@@ -570,6 +622,27 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
       sp.z = o.z;
       cr = decode_cell(skeys[i], g);
       pi = pos[i];
+    }
+    if constexpr (F == kF64) {
+      // a wavefront of fixed particles whose densities cannot have changed
+      // since the last pass keeps them (see DryWalls)
+      if (dw.stamps && __ballot(live && (uint64_t)dw.ids[i] < dw.n_moving) == 0ull) {
+        bool keep = true;
+        if (live) {
+          const uint32_t id = dw.ids[i];
+          const bool dry = dry_cells(cr, g, dw);
+          keep = dry && dw.reuse && dw.dry_pass[id] == dw.pass - 1u;
+          if (dry) dw.dry_pass[id] = dw.pass;
+        }
+        if (__ballot(!keep) == 0ull) {
+          if (live) {
+            const double rho = pi.w, pr = reinterpret_cast<const double*>(wvel + i)[3];
+            rp[i] = make_double2(rho, pr);
+            pres[i] = tait_p(rho, kc);
+          }
+          return;
+        }
+      }
     }
     // dot-form scan of staged runs: a = -2 s, and |s|^2 for the threshold
     const float ax = -2.0f * sp.x, ay = -2.0f * sp.y, az = -2.0f * sp.z;
@@ -659,13 +732,8 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
     if (!live) return;
     if (overflow) atomicOr(&flags[FLAG_CAPACITY], 1u);
     if constexpr (F == kF64) {
-      double rho = kc.msig * acc.s;
-      double ratio = rho * kc.inv_rho0;
-      double r2_ = ratio * ratio;
-      double r4 = r2_ * r2_;
-      double r7 = r4 * r2_ * ratio;
-      double p = kc.B * (r7 - 1.0);
-      if (p < 0.0) p = 0.0;
+      const double rho = kc.msig * acc.s;
+      const double p = tait_p(rho, kc);
       const double pr = p / (rho * rho);
       rp[i] = make_double2(rho, pr);
       reinterpret_cast<double*>(wpos + i)[3] = rho;
@@ -774,7 +842,8 @@ __global__ __launch_bounds__(kForceBlock, kForceWaves<F>) void k_sph_force(
     SphUnits u, const typename SphForm<F>::S4* __restrict__ spos,
     const typename SphForm<F>::S4* __restrict__ svel, typename SphForm<F>::S4* __restrict__ opos,
     typename SphForm<F>::S4* __restrict__ ovel, uint32_t* __restrict__ okeys,
-    typename SphForm<F>::S4* __restrict__ oacc, MovedOut moved, uint32_t* flags, TileQueue tq) {
+    typename SphForm<F>::S4* __restrict__ oacc, MovedOut moved, uint32_t* flags, TileQueue tq,
+    uint32_t* __restrict__ movcell, uint32_t next_pass) {
   using real = typename SphForm<F>::real;
   using G4 = typename SphForm<F>::G4;
   using R2 = typename SphForm<F>::R2;
@@ -904,6 +973,7 @@ __global__ __launch_bounds__(kForceBlock, kForceWaves<F>) void k_sph_force(
     else
       atomicOr(&flags[FLAG_DOMAIN], 1u);
     okeys[i] = key;
+    movcell[key] = next_pass;   // this cell holds a moving particle at the next density pass
     moved_word_live(moved, i, key != skeys[i], live_mask);
   });
 }
@@ -966,6 +1036,9 @@ void sph_step_impl(Ctx& c) {
   static_assert(kDensBlock == kForceBlock, "both gathers walk the same tiles");
   const uint32_t n_tiles = blocks_for(n, kDensBlock);
   const TileQueue tq{c.tile_q.as<uint32_t>(), n_tiles, (n_tiles + 7u) / 8u};
+  const uint32_t pass = ++c.dens_pass;
+  const DryWalls dw{c.ids[c.cur_ids].as<uint32_t>(), c.n_moving, c.movcell.as<uint32_t>(),
+                    c.dry_pass.as<uint32_t>(), pass, c.stamp_pass == pass, c.dry_reuse && !c.record_all};
   const double se = (F == kF32) ? 4.0 : 8.0;   // bytes per state scalar
   const double ge = (double)sizeof(real);     // bytes per scalar of the pair arithmetic
   {
@@ -983,7 +1056,7 @@ void sph_step_impl(Ctx& c) {
     k_sph_density<F><<<dgrid, kDensBlock, 0, c.stream>>>(n, c.pos[c.cur].as<S4>(), gpos, skeys, c.lb.as<uint32_t>(),
                                                         c.grid, c.sph, u, c.nmeta.as<uint2>(),
                                                         c.nword.as<uint32_t>(), c.rp.as<R2>(), wpos, wvel,
-                                                        c.pres.as<real>(), c.flags.as<uint32_t>(), tq);
+                                                        c.pres.as<real>(), c.flags.as<uint32_t>(), tq, dw);
     PR_HIP(hipGetLastError());
   }
   {
@@ -1006,13 +1079,14 @@ void sph_step_impl(Ctx& c) {
         c.nmeta.as<uint2>(), c.nword.as<uint32_t>(), c.grid, c.sph, u, c.pos[c.cur].as<S4>(), c.vel[c.cur].as<S4>(),
         c.pos[dst].as<S4>(), c.vel[dst].as<S4>(), c.keys[kdst].as<uint32_t>(),
         c.keep_acc ? c.acc.as<S4>() : nullptr, MovedOut{c.inc_mask.as<uint64_t>(), c.inc_tiles.as<uint32_t>()},
-        c.flags.as<uint32_t>(), tq);
+        c.flags.as<uint32_t>(), tq, c.movcell.as<uint32_t>(), pass + 1u);
     PR_HIP(hipGetLastError());
     c.cur = dst;
     c.cur_key = kdst;
     c.sorted = false;
     c.keys_valid = true;
     c.inc_ready = true;   // keys[cur_key ^ 1] and lb still describe the slots the step kept
+    c.stamp_pass = pass + 1u;
   }
 }
 

@@ -135,3 +135,29 @@ def test_f64_bitexact_unstaged_runs():
     got, ref = _run_pair(d, arrs, 3)
     for k in FIELDS:
         assert np.array_equal(got[k], ref[k]), k
+
+
+def test_f64_bitexact_dry_walls_over_steps(monkeypatch):
+    """Wall particles with no moving particle in their 27 cells at two passes in
+    a row keep their density (the wavefronts made only of them skip the scan);
+    the 8000-particle column collapses over the tank floor for 150 steps, wetting
+    walls that were dry.  FP64 fields stay bit-identical to the oracle, and to a
+    run with the reuse switched off, at 60 and 150 steps."""
+    d, arrs = make_scene(pa.SCENE_DAM_BREAK, n_target=8000, dx=0.005)
+    st = oracle_state(d, arrs)
+    runs = {}
+    for mode in ("on", "off"):
+        monkeypatch.setenv("PRESTIGE_DRY", mode)
+        with pa.Context(d) as ctx:
+            upload_all(ctx, arrs)
+            out = []
+            for n in (60, 90):
+                ctx.step(n)
+                out.append({k: ctx.download(k) for k in FIELDS})
+            runs[mode] = out
+    for idx, n in enumerate((60, 90)):
+        assert st.step(n) == 0
+        ref = {k: st.get(k) for k in FIELDS}
+        for k in FIELDS:
+            assert np.array_equal(runs["on"][idx][k], ref[k]), (idx, k)
+            assert np.array_equal(runs["off"][idx][k], ref[k]), (idx, k)
