@@ -229,6 +229,10 @@ static void allocate(Ctx& c) {
   c.acc.alloc(nn * e4);
   PR_HIP(hipMemsetAsync(c.rp.p, 0, nn * 2 * es, c.stream));
   PR_HIP(hipMemsetAsync(c.pres.p, 0, nn * es, c.stream));
+  if (c.desc.model == PRESTIGE_MODEL_WCSPH && c.desc.precision != PRESTIGE_F64) {
+    c.rp_id.alloc(nn * 2 * es);   // outputs of dry wall particles by id
+    c.pres_id.alloc(nn * es);
+  }
   PR_HIP(hipMemsetAsync(c.acc.p, 0, nn * e4, c.stream));
   if (mixed) c.off32.alloc(nn * sizeof(float4));
   if (mixed) {

@@ -523,8 +523,10 @@ __device__ inline void walk_runs_staged(const uint2* __restrict__ meta, const ui
 // no moving particle at this pass and at the previous one has the same
 // candidates as then (fixed particles never move, and the stable sort keeps
 // their order), so its density, pressure and p/rho^2 are the previous pass's
-// bit for bit; a wavefront made only of such particles keeps them (FP64 form:
-// rho and p/rho^2 travel in the .w lanes of the state) instead of scanning.
+// bit for bit; a wavefront made only of such particles keeps them instead of
+// scanning (FP64 form: rho and p/rho^2 travel in the .w lanes of the state
+// and p follows from rho; FP32 and mixed forms: each pass leaves a copy of
+// the (rho, p/rho^2) and p of its dry particles by particle id).
 // movcell[c] = pass when cell c holds a moving particle (the force pass
 // stamps the next pass), dry_pass[id] = the last pass the particle was seen
 // dry.  Passes count up; an upload or a restore skips one, so nothing from
@@ -536,6 +538,8 @@ struct DryWalls {
   uint64_t n_moving;
   const uint32_t* movcell;
   uint32_t* dry_pass;
+  void* rp_id;     // FP32 / mixed: (rho, p/rho^2) by id (SphForm::R2)
+  void* pres_id;   // FP32 / mixed: p by id (SphForm::real)
   uint32_t pass;
   bool stamps;   // movcell describes this pass
   bool reuse;    // densities may be kept
@@ -623,25 +627,32 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
       cr = decode_cell(skeys[i], g);
       pi = pos[i];
     }
-    if constexpr (F == kF64) {
-      // a wavefront of fixed particles whose densities cannot have changed
-      // since the last pass keeps them (see DryWalls)
-      if (dw.stamps && __ballot(live && (uint64_t)dw.ids[i] < dw.n_moving) == 0ull) {
-        bool keep = true;
+    // a wavefront of fixed particles whose densities cannot have changed
+    // since the last pass keeps them (see DryWalls)
+    using R2 = typename SphForm<F>::R2;
+    uint64_t dry_mask = 0;   // lanes seen dry at this pass
+    if (dw.stamps && __ballot(live && (uint64_t)dw.ids[i] < dw.n_moving) == 0ull) {
+      bool keep = true, dry = false;
+      uint32_t id = 0;
+      if (live) {
+        id = dw.ids[i];
+        dry = dry_cells(cr, g, dw);
+        keep = dry && dw.reuse && dw.dry_pass[id] == dw.pass - 1u;
+        if (dry) dw.dry_pass[id] = dw.pass;
+      }
+      dry_mask = __ballot(dry);
+      if (__ballot(!keep) == 0ull) {
         if (live) {
-          const uint32_t id = dw.ids[i];
-          const bool dry = dry_cells(cr, g, dw);
-          keep = dry && dw.reuse && dw.dry_pass[id] == dw.pass - 1u;
-          if (dry) dw.dry_pass[id] = dw.pass;
-        }
-        if (__ballot(!keep) == 0ull) {
-          if (live) {
+          if constexpr (F == kF64) {
             const double rho = pi.w, pr = reinterpret_cast<const double*>(wvel + i)[3];
             rp[i] = make_double2(rho, pr);
             pres[i] = tait_p(rho, kc);
+          } else {
+            rp[i] = static_cast<const R2*>(dw.rp_id)[id];
+            pres[i] = static_cast<const real*>(dw.pres_id)[id];
           }
-          return;
         }
+        return;
       }
     }
     // dot-form scan of staged runs: a = -2 s, and |s|^2 for the threshold
@@ -752,6 +763,11 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
       reinterpret_cast<float*>(wpos + i)[3] = rho;
       reinterpret_cast<float*>(wvel + i)[3] = pr;
       pres[i] = p;
+      if ((dry_mask >> lane) & 1ull) {
+        const uint32_t id = dw.ids[i];
+        static_cast<float2*>(dw.rp_id)[id] = make_float2(rho, pr);
+        static_cast<float*>(dw.pres_id)[id] = p;
+      }
     } else {
       // EOS in FP64 from the compensated sum: gamma = 7 amplifies the relative
       // error of rho by 7 rho^7 / (rho^7 - rho0^7), so rho is never rounded to
@@ -763,8 +779,14 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
       double r7 = r4 * r2_ * ratio;
       double p = kc.B * (r7 - 1.0);
       if (p < 0.0) p = 0.0;
-      rp[i] = make_float2((float)rho, (float)(p / (rho * rho)));
+      const float2 rpi = make_float2((float)rho, (float)(p / (rho * rho)));
+      rp[i] = rpi;
       pres[i] = (float)p;
+      if ((dry_mask >> lane) & 1ull) {
+        const uint32_t id = dw.ids[i];
+        static_cast<float2*>(dw.rp_id)[id] = rpi;
+        static_cast<float*>(dw.pres_id)[id] = (float)p;
+      }
     }
   });
 }
@@ -1038,7 +1060,8 @@ void sph_step_impl(Ctx& c) {
   const TileQueue tq{c.tile_q.as<uint32_t>(), n_tiles, (n_tiles + 7u) / 8u};
   const uint32_t pass = ++c.dens_pass;
   const DryWalls dw{c.ids[c.cur_ids].as<uint32_t>(), c.n_moving, c.movcell.as<uint32_t>(),
-                    c.dry_pass.as<uint32_t>(), pass, c.stamp_pass == pass, c.dry_reuse && !c.record_all};
+                    c.dry_pass.as<uint32_t>(), c.rp_id.p, c.pres_id.p, pass, c.stamp_pass == pass,
+                    c.dry_reuse && !c.record_all};
   const double se = (F == kF32) ? 4.0 : 8.0;   // bytes per state scalar
   const double ge = (double)sizeof(real);     // bytes per scalar of the pair arithmetic
   {

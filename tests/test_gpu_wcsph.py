@@ -161,3 +161,21 @@ def test_f64_bitexact_dry_walls_over_steps(monkeypatch):
         for k in FIELDS:
             assert np.array_equal(runs["on"][idx][k], ref[k]), (idx, k)
             assert np.array_equal(runs["off"][idx][k], ref[k]), (idx, k)
+
+
+@pytest.mark.parametrize("precision", [pa.MIXED, pa.F32])
+def test_dry_wall_reuse_is_exact_in_reduced_precision(monkeypatch, precision):
+    """The FP32 and mixed forms keep the outputs of dry wall particles by
+    particle id: 150 steps of the collapsing column give the same bits with
+    the reuse on and off."""
+    d, arrs = make_scene(pa.SCENE_DAM_BREAK, n_target=8000, dx=0.005)
+    d.precision = precision
+    runs = {}
+    for mode in ("on", "off"):
+        monkeypatch.setenv("PRESTIGE_DRY", mode)
+        with pa.Context(d) as ctx:
+            upload_all(ctx, arrs)
+            ctx.step(150)
+            runs[mode] = {k: ctx.download(k) for k in FIELDS}
+    for k in FIELDS:
+        assert np.array_equal(runs["on"][k], runs["off"][k]), k
