@@ -501,7 +501,7 @@ struct RecordAll {
 
 template <int F>
 void sph_tape_forward(Ctx& c, uint32_t nsteps, uint32_t every) {
-  RecordAll keep_masks(c);
+  RecordAll keep_masks(c);   // switched off below for the steps not recorded
   Tape& T = c.tape;
   T.clear();
   T.nsteps = nsteps;
@@ -514,6 +514,7 @@ void sph_tape_forward(Ctx& c, uint32_t nsteps, uint32_t every) {
     if (s % every == 0) {
       snapshot_state(c, T.next_ckpt());
     }
+    c.record_all = s >= last0;   // only recorded steps need every particle's masks
     sph_step_impl<F>(c);
     if (s >= last0) record_sph_step<F>(c, *T.rec[s - last0]);
   }
