@@ -490,8 +490,8 @@ void record_sph_step(Ctx& c, StepRecord& r) {
   cp(r.ids, c.ids[c.cur_ids].p, c.n * 4);
 }
 
-// Steps taken while this is alive rebuild every particle's row masks (the
-// adjoint reads those of fixed particles too).
+// Steps taken while this is alive leave every particle usable row masks
+// (the adjoint force pass walks those of fixed particles too).
 struct RecordAll {
   Ctx& c;
   bool old;
@@ -514,7 +514,7 @@ void sph_tape_forward(Ctx& c, uint32_t nsteps, uint32_t every) {
     if (s % every == 0) {
       snapshot_state(c, T.next_ckpt());
     }
-    c.record_all = s >= last0;   // only recorded steps need every particle's masks
+    c.record_all = s >= last0;   // only recorded steps need usable masks for every particle
     sph_step_impl<F>(c);
     if (s >= last0) record_sph_step<F>(c, *T.rec[s - last0]);
   }

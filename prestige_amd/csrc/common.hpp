@@ -242,7 +242,7 @@ struct Ctx {
   DevBuf tile_q;                 // tile-queue counters of the WCSPH gathers (kQueueWords)
   // WCSPH wall particles far from the fluid (DryWalls): cell stamps (n_cells),
   // last dry pass per particle id, pass counter, the pass the stamps describe,
-  // and whether steps must rebuild every particle's masks (adjoint records)
+  // and whether steps must leave every particle usable masks (adjoint records)
   DevBuf movcell, dry_pass, rp_id, pres_id;
   uint32_t dens_pass = 16, stamp_pass = 0;
   bool record_all = false;

@@ -531,8 +531,11 @@ __device__ inline void walk_runs_staged(const uint2* __restrict__ meta, const ui
 // stamps the next pass), dry_pass[id] = the last pass the particle was seen
 // dry.  Passes count up; an upload or a restore skips one, so nothing from
 // before it is reused.  Its row masks are not rewritten: the forward force
-// pass never walks a fixed particle's masks, and runs that record steps for
-// the adjoint keep every particle (reuse false).
+// pass never walks a fixed particle's masks.  Steps recorded for the adjoint
+// give it masks with no candidates (zero_masks): the adjoint force pass walks
+// every slot, and a dry particle's terms there are exactly zero (no moving
+// particle lies within its cells, so its pressure and density reach no
+// moving particle's force and every pair it has is between fixed particles).
 struct DryWalls {
   const uint32_t* ids;
   uint64_t n_moving;
@@ -541,8 +544,9 @@ struct DryWalls {
   void* rp_id;     // FP32 / mixed: (rho, p/rho^2) by id (SphForm::R2)
   void* pres_id;   // FP32 / mixed: p by id (SphForm::real)
   uint32_t pass;
-  bool stamps;   // movcell describes this pass
-  bool reuse;    // densities may be kept
+  bool stamps;       // movcell describes this pass
+  bool reuse;        // densities may be kept
+  bool zero_masks;   // kept particles get empty row masks (recorded steps)
 };
 
 constexpr int kDensBlock = 64;  // one wavefront: block barriers are wave barriers
@@ -642,6 +646,8 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
       }
       dry_mask = __ballot(dry);
       if (__ballot(!keep) == 0ull) {
+        if (live && dw.zero_masks)
+          for (uint32_t r = 0; r < kRuns; ++r) meta[meta_index(r, n, i)] = make_uint2(0u, 0u);
         if (live) {
           if constexpr (F == kF64) {
             const double rho = pi.w, pr = reinterpret_cast<const double*>(wvel + i)[3];
@@ -1061,7 +1067,7 @@ void sph_step_impl(Ctx& c) {
   const uint32_t pass = ++c.dens_pass;
   const DryWalls dw{c.ids[c.cur_ids].as<uint32_t>(), c.n_moving, c.movcell.as<uint32_t>(),
                     c.dry_pass.as<uint32_t>(), c.rp_id.p, c.pres_id.p, pass, c.stamp_pass == pass,
-                    c.dry_reuse && !c.record_all};
+                    c.dry_reuse, c.record_all};
   const double se = (F == kF32) ? 4.0 : 8.0;   // bytes per state scalar
   const double ge = (double)sizeof(real);     // bytes per scalar of the pair arithmetic
   {
