@@ -449,15 +449,23 @@ inline DemAdjConst dem_adj_const(const Ctx& c) {
 
 template <int F>
 void dem_tape_forward(Ctx& c, uint32_t nsteps, uint32_t every) {
-  c.tape.clear();
-  c.tape.nsteps = nsteps;
-  c.tape.every = every;
+  using H = typename std::conditional<F == kF64, double, float>::type;
+  Tape& T = c.tape;
+  T.clear();
+  T.nsteps = nsteps;
+  T.every = every;
+  // the last segment is recorded on the way, so the reverse sweep starts
+  // without recomputing it
+  const uint32_t last0 = nsteps ? ((nsteps - 1) / every) * every : 0;
+  while (T.rec.size() < every) T.rec.emplace_back(new StepRecord());
   for (uint32_t s = 0; s < nsteps; ++s) {
     if (s % every == 0) {
-      snapshot_state(c, c.tape.next_ckpt());
+      snapshot_state(c, T.next_ckpt());
     }
     dem_step_impl<F>(c);
+    if (s >= last0) record_dem_step<H>(c, *T.rec[s - last0]);
   }
+  T.last_recorded = nsteps > 0;
   if (!c.tape.final_state) c.tape.final_state.reset(new Snapshot());
   snapshot_state(c, *c.tape.final_state);
   c.check_flags();
@@ -490,13 +498,16 @@ void dem_adjoint(Ctx& c, const double* seed_x, const double* seed_v, double* gra
   const uint32_t nseg = (T.nsteps + T.every - 1) / T.every;
   while (T.rec.size() < T.every) T.rec.emplace_back(new StepRecord());
   for (uint32_t seg = nseg; seg-- > 0;) {
-    restore_state(c, *T.ckpt[seg]);
     const uint32_t s0 = seg * T.every;
     const uint32_t len = (T.nsteps - s0) < T.every ? (T.nsteps - s0) : T.every;
-    for (uint32_t t = 0; t < len; ++t) {
-      dem_step_impl<F>(c);
-      record_dem_step<H>(c, *T.rec[t]);
+    if (!(seg == nseg - 1 && T.last_recorded)) {
+      restore_state(c, *T.ckpt[seg]);
+      for (uint32_t t = 0; t < len; ++t) {
+        dem_step_impl<F>(c);
+        record_dem_step<H>(c, *T.rec[t]);
+      }
     }
+    T.last_recorded = false;   // records are overwritten by the recompute of earlier segments
     for (uint32_t t = len; t-- > 0;) {
       StepRecord& r = *T.rec[t];
       DemStepView v{r.state_pos.as<double4>(), r.state_vel.as<double4>(), r.state_om.as<double4>(),
