@@ -1076,7 +1076,7 @@ void sph_step_impl(Ctx& c) {
     double per = (F == kMixed ? 12.0 : 3.0 * ge) + 3.0 * ge;
     LaunchScope ls(&c, F == kF64 ? "sph_density_f64" : (F == kF32 ? "sph_density_f32" : "sph_density_mixed"),
                    per * n);
-#ifdef PRESTIGE_DENS_PER_CU
+#ifdef PRESTIGE_DENS_PER_CU   // resident blocks per CU fixed at build time (occupancy sweeps)
     static const int per_cu = PRESTIGE_DENS_PER_CU;
 #else
     static const int per_cu = blocks_per_cu(k_sph_density<F>, kDensBlock);
@@ -1097,7 +1097,7 @@ void sph_step_impl(Ctx& c) {
     LaunchScope ls(&c, F == kF64 ? "sph_force_f64" : (F == kF32 ? "sph_force_f32" : "sph_force_mixed"), per * nm);
     const int dst = c.cur ^ 1;
     const int kdst = c.cur_key ^ 1;
-#ifdef PRESTIGE_FORCE_PER_CU
+#ifdef PRESTIGE_FORCE_PER_CU   // as PRESTIGE_DENS_PER_CU, for the force pass
     static const int per_cu = PRESTIGE_FORCE_PER_CU;
 #else
     static const int per_cu = blocks_per_cu(k_sph_force<F>, kForceBlock);
