@@ -21,6 +21,9 @@
 //            opposite runs in LDS and walk the stored masks: momentum +
 //            viscosity, then symplectic Euler, the next step's cell key and
 //            the moved-slot mask of the next incremental sort.
+// Both passes take their 64-slot tiles from per-XCD queues (for_each_tile),
+// and the density pass keeps the values of wall particles that had no moving
+// particle nearby at this pass and the previous one (DryWalls).
 // Scanning in FP32 halves the bytes each candidate costs, and the expensive
 // pair bodies only run over the set bits.  Runs in the order of sph_run and
 // bits in ascending order are the visiting order of the host oracle, so FP64
