@@ -563,8 +563,14 @@ constexpr int kDensBlock = 64;  // one wavefront: block barriers are wave barrie
 #ifndef PRESTIGE_F32_FORCE_WAVES
 #define PRESTIGE_F32_FORCE_WAVES 5
 #endif
+#ifndef PRESTIGE_MIXED_FORCE_WAVES
+#define PRESTIGE_MIXED_FORCE_WAVES 5
+#endif
+#ifndef PRESTIGE_DENS_WAVES
+#define PRESTIGE_DENS_WAVES 8
+#endif
 template <int F>
-constexpr int kDensWaves = F == kF32 ? PRESTIGE_F32_DENS_WAVES : 8;
+constexpr int kDensWaves = F == kF32 ? PRESTIGE_F32_DENS_WAVES : PRESTIGE_DENS_WAVES;
 #ifndef PRESTIGE_STAGE_CAP
 #define PRESTIGE_STAGE_CAP 256
 #endif
@@ -818,7 +824,7 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
 // ---------------------------------------------------------------------------
 constexpr int kForceBlock = 64;  // one wavefront: block barriers are wave barriers
 template <int F>
-constexpr int kForceWaves = F == kF64 ? 4 : (F == kF32 ? PRESTIGE_F32_FORCE_WAVES : 5);
+constexpr int kForceWaves = F == kF64 ? 4 : (F == kF32 ? PRESTIGE_F32_FORCE_WAVES : PRESTIGE_MIXED_FORCE_WAVES);
 #ifndef PRESTIGE_FORCE_CAP
 #define PRESTIGE_FORCE_CAP 160
 #endif
