@@ -402,14 +402,19 @@ __device__ inline uint32_t wave_sum(uint32_t x) {
   return x;
 }
 
-// Moved slots of a tile in slot order, as (key << 32 | slot), from run (the
-// moved slots before the tile); visit(i, rank_before_i, moved, valid) sees
-// every slot of the tile with the number of moved slots before it, called by
-// every lane of every wavefront (valid: i < n).  All mask words of the tile
-// are read up front, so one barrier serves the whole tile.
-template <class Visit>
-__device__ inline void inc_tile_walk(uint32_t n, const uint64_t* __restrict__ mask, uint32_t run,
-                                     Visit&& visit) {
+// Block b of n_tiles: sums the per-tile moved counts (from the step kernel
+// that wrote the keys) of the tiles before b, writes that to tile_off[b] and
+// the count before each mask word to word_off, and lists its moved slots at
+// their rank, as (key << 32 | slot) in slot order.  Block 0 sums every tile
+// and writes the total to tile_off[n_tiles], *count and the host-mapped word
+// *host_count, which the host reads without waiting when it picks the next
+// build's sort.  The tile's mask words are read before the counts are summed,
+// so one barrier serves both.  The placement clears the tile counts.
+__global__ __launch_bounds__(kIncThreads) void k_inc_gather(
+    uint32_t n, uint32_t n_tiles, const uint32_t* __restrict__ tile_cnt, const uint32_t* __restrict__ knew,
+    const uint64_t* __restrict__ mask, uint32_t* __restrict__ tile_off, uint32_t* __restrict__ word_off,
+    uint32_t* __restrict__ count, uint32_t* __restrict__ host_count, uint64_t* __restrict__ mov) {
+  __shared__ uint32_t s_red[kIncWaves];
   __shared__ uint32_t s_w[kIncItems * kIncWaves];
   const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
   const uint32_t base = blockIdx.x * (uint32_t)kTile;
@@ -418,9 +423,30 @@ __device__ inline void inc_tile_walk(uint32_t n, const uint64_t* __restrict__ ma
   for (int it = 0; it < kIncItems; ++it) {
     const uint32_t i0 = base + (uint32_t)it * kIncThreads + 64u * w;   // the wavefront's first slot
     word[it] = i0 < n ? mask[i0 >> 6] : 0ull;
-    if (lane == 0) s_w[it * kIncWaves + w] = (uint32_t)__popcll(word[it]);
+  }
+  const uint32_t lim = blockIdx.x == 0 ? n_tiles : blockIdx.x;
+  uint32_t part = 0;
+  for (uint32_t t = threadIdx.x; t < lim; t += kIncThreads) part += tile_cnt[t];
+  part = wave_sum(part);
+  if (lane == 0) {
+    s_red[w] = part;
+#pragma unroll
+    for (int it = 0; it < kIncItems; ++it) s_w[it * kIncWaves + w] = (uint32_t)__popcll(word[it]);
   }
   __syncthreads();
+  uint32_t sum = 0;
+  for (uint32_t k = 0; k < kIncWaves; ++k) sum += s_red[k];
+  const uint32_t pre = blockIdx.x == 0 ? 0u : sum;
+  if (threadIdx.x == 0) {
+    tile_off[blockIdx.x] = pre;
+    if (blockIdx.x == 0) {
+      tile_off[n_tiles] = sum;
+      count[0] = sum;
+      *host_count = sum;
+      __threadfence_system();
+    }
+  }
+  uint32_t run = pre;
 #pragma unroll
   for (int it = 0; it < kIncItems; ++it) {
     uint32_t before = run, total = 0;
@@ -430,55 +456,11 @@ __device__ inline void inc_tile_walk(uint32_t n, const uint64_t* __restrict__ ma
       total += cnt;
     }
     const uint32_t i = base + (uint32_t)it * kIncThreads + 64u * w + lane;
-    visit(i, before + (uint32_t)__popcll(word[it] & lanes_below()), ((word[it] >> lane) & 1ull) != 0ull, i < n);
+    before += (uint32_t)__popcll(word[it] & lanes_below());
+    if (i < n && (i & 63u) == 0u) word_off[i >> 6] = before;
+    if (i < n && ((word[it] >> lane) & 1ull) != 0ull) mov[before] = ((uint64_t)knew[i] << 32) | i;
     run += total;
   }
-}
-
-// Block b of n_tiles: sums the per-tile moved counts (from the step kernel
-// that wrote the keys) of the tiles before b, writes that to tile_off[b] and
-// the count before each mask word to word_off, and lists its moved slots at
-// their rank.  Block 0 writes the total to tile_off[n_tiles], *count and the
-// host-mapped word *host_count, which the host reads without waiting when it
-// picks the next build's sort.  The placement clears the tile counts.
-__global__ __launch_bounds__(kIncThreads) void k_inc_gather(
-    uint32_t n, uint32_t n_tiles, const uint32_t* __restrict__ tile_cnt, const uint32_t* __restrict__ knew,
-    const uint64_t* __restrict__ mask, uint32_t* __restrict__ tile_off, uint32_t* __restrict__ word_off,
-    uint32_t* __restrict__ count, uint32_t* __restrict__ host_count, uint64_t* __restrict__ mov) {
-  __shared__ uint32_t s_red[2 * kIncWaves];
-  uint32_t pre = 0, tot = 0;
-  for (uint32_t t = threadIdx.x; t < n_tiles; t += kIncThreads) {
-    const uint32_t v = tile_cnt[t];
-    tot += v;
-    pre += t < blockIdx.x ? v : 0u;
-  }
-  pre = wave_sum(pre);
-  tot = wave_sum(tot);
-  const uint32_t w = threadIdx.x >> 6;
-  if (__lane_id() == 0) {
-    s_red[w] = pre;
-    s_red[kIncWaves + w] = tot;
-  }
-  __syncthreads();
-  pre = 0;
-  tot = 0;
-  for (uint32_t k = 0; k < kIncWaves; ++k) {
-    pre += s_red[k];
-    tot += s_red[kIncWaves + k];
-  }
-  if (threadIdx.x == 0) {
-    tile_off[blockIdx.x] = pre;
-    if (blockIdx.x == 0) {
-      tile_off[n_tiles] = tot;
-      count[0] = tot;
-      *host_count = tot;
-      __threadfence_system();
-    }
-  }
-  inc_tile_walk(n, mask, pre, [&](uint32_t i, uint32_t before, bool moved, bool valid) {
-    if (valid && (i & 63u) == 0u) word_off[i >> 6] = before;
-    if (valid && moved) mov[before] = ((uint64_t)knew[i] << 32) | i;
-  });
 }
 
 // Writes the moved entries sorted by (key, slot) to out.  Up to kRankMax
@@ -707,7 +689,7 @@ static void incremental_sort(Ctx& c) {
   uint32_t* tiles = tile_cnt + n_tiles;                  // tile offsets, n_tiles + 1
   uint32_t* woff = c.inc_woff.as<uint32_t>();            // moved slots before each mask word
   {
-    LaunchScope ls(&c, "sort_inc_gather", 4.0 * n_tiles * n_tiles + 8.5 * n);
+    LaunchScope ls(&c, "sort_inc_gather", 2.0 * n_tiles * n_tiles + 8.5 * n);
     k_inc_gather<<<n_tiles, kIncThreads, 0, c.stream>>>(n, n_tiles, tile_cnt, knew, mask, tiles, woff, cnt,
                                                         c.inc_host_dev, c.inc_mov.as<uint64_t>());
     PR_HIP(hipGetLastError());
