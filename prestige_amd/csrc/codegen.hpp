@@ -29,6 +29,8 @@ struct GenField {
 struct GenPass {
   std::string kernel;            // extern "C" symbol
   std::vector<GenField> fields;  // after the fixed parameters
+  // cell loop: fields read at [j], passed after fields as copies in slot order
+  std::vector<GenField> slot_fields;
 };
 
 struct GenProgram {

@@ -432,6 +432,7 @@ std::vector<Tok> rewrite(std::vector<Tok> t, const std::string& eq) {
 // Field-access and name mapping of one body inside a generated pass.
 struct PassNames {
   std::set<std::string> split;  // fields read at [j] and written in this pass
+  bool slot_j = false;          // [j] reads come from slot-order copies g_<name>[js]
 };
 
 std::string emit_body(const Eq& e, const PassNames& pn, std::set<std::string>* consts_used) {
@@ -460,7 +461,7 @@ std::string emit_body(const Eq& e, const PassNames& pn, std::set<std::string>* c
         } else if (at_i) {
           piece = tk.s + "_i";
         } else {
-          piece = "f_" + tk.s + (pn.split.count(tk.s) ? "_in" : "") + "[jd]";
+          piece = pn.slot_j ? "g_" + tk.s + "[js]" : "f_" + tk.s + (pn.split.count(tk.s) ? "_in" : "") + "[jd]";
         }
         out += piece + " ";
         k = c;
@@ -585,6 +586,7 @@ GenProgram build_program(const int32_t* ids, uint32_t n, int32_t loop, bool f32_
       }
     }
     PassNames pn;
+    pn.slot_j = loop == PRESTIGE_LOOP_CELLS;
     for (auto& kv : fields) {
       kv.second.split = kv.second.written && read_j.count(kv.first);
       if (kv.second.split) pn.split.insert(kv.first);
@@ -606,6 +608,11 @@ GenProgram build_program(const int32_t* ids, uint32_t n, int32_t loop, bool f32_
       else
         params += ",\n    const double* __restrict__ f_" + f.name;
     }
+    if (pn.slot_j)
+      for (const std::string& f : read_j) {
+        params += ",\n    const double* __restrict__ g_" + f;
+        gp.slot_fields.push_back(fields[f]);
+      }
     s += "\n// pass " + std::to_string(p) + ": fields";
     for (auto& kv : fields) s += " " + kv.first + (kv.second.split ? "(in,out)" : "");
     s += "\n";

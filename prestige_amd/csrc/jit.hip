@@ -24,6 +24,14 @@ __global__ __launch_bounds__(kBlock) void k_pos_to_ids(uint32_t n, const T4* __r
 }
 
 // Layout of prestige_consts in the generated source.
+// a field in slot order for the cell loop's [j] reads: out[k] = f[ids[k]]
+__global__ __launch_bounds__(kBlock) void k_field_to_slots(uint32_t n, const uint32_t* __restrict__ ids,
+                                                           const double* __restrict__ f,
+                                                           double* __restrict__ out) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k < n) out[k] = f[ids[k]];
+}
+
 struct GenConsts {
   double h, dt, rho0, c0, alpha, kn, kt, mu, gx, gy, gz;
 };
@@ -153,6 +161,23 @@ void run_equations(Ctx& c, const int32_t* eq_ids, uint32_t neq, int32_t loop, do
         swap_names.push_back(f.name);
       }
     }
+    // slot-order copies of the fields read at [j] (cell loop), taken from the
+    // pass's inputs before it runs
+    std::vector<void*> gptr;
+    gptr.reserve(gp.slot_fields.size());
+    std::vector<std::unique_ptr<DevBuf>> slots;
+    for (const GenField& f : gp.slot_fields) {
+      const double* src = field_ptr(c, f);
+      std::unique_ptr<DevBuf> o(new DevBuf());
+      o->alloc((size_t)n * sizeof(double));
+      {
+        LaunchScope ls(&c, "generated_slot_copy", 20.0 * n);
+        k_field_to_slots<<<blocks_for(n), kBlock, 0, c.stream>>>(n, ids, src, o->as<double>());
+        PR_HIP(hipGetLastError());
+      }
+      gptr.push_back(o->p);
+      slots.push_back(std::move(o));
+    }
     std::vector<void*> args;
     args.push_back(&n);
     args.push_back(&pos);
@@ -167,6 +192,7 @@ void run_equations(Ctx& c, const int32_t* eq_ids, uint32_t neq, int32_t loop, do
     }
     args.push_back(&k);
     for (void*& p : fptr) args.push_back(&p);
+    for (void*& p : gptr) args.push_back(&p);
     {
       LaunchScope ls(&c, "generated_pass", 0.0);
       PR_HIP(hipModuleLaunchKernel(fn, blocks_for(n), 1, 1, kBlock, 1, 1, 0, c.stream, args.data(), nullptr));

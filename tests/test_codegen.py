@@ -51,10 +51,10 @@ def test_schedule():
 def test_generated_text():
     cells = pa.generate_hip([pa.EQ1], pa.LOOP_CELLS)
     assert 'extern "C" __global__ void __launch_bounds__(256) prestige_pass_0(' in cells
-    assert "force_i += f_mass[jd]" in cells
+    assert "force_i += g_mass[js]" in cells and "const double* __restrict__ g_mass" in cells
     assert "< c2" in cells and "lb[row" in cells
     pairs = pa.generate_hip([pa.EQ1], pa.LOOP_ALLPAIRS)
-    assert "for (unsigned jd = 0; jd < n; ++jd)" in pairs
+    assert "for (unsigned jd = 0; jd < n; ++jd)" in pairs and "force_i += f_mass[jd]" in pairs
     e = pa.define_equation("methods", "{ let d = (x[i] - x[j]).abs().sqrt(); w[i] += f64::exp(-d) * d.powi(2) }")
     t = pa.generate_hip([e], pa.LOOP_CELLS)
     assert "sqrt ( fabs ( (" in t and "exp ( - d )" in t and "pow ( d , 2 )" in t
