@@ -2,17 +2,18 @@
 """Throughput of the particle hot path on one MI355X.
 
 A step is one full WCSPH step of the dam-break workload (neighbour rebuild:
-cell keys, radix sort, reorder, cell table; density + Tait EOS gather;
-momentum + viscosity gather fused with the symplectic-Euler update).  The
-workload is 1,000,000 fluid particles (dx = 5 mm, 100^3 block) plus the
-799,320 particles of a 3-layer 1.6 x 0.67 x 1.2 m tank.  `value` is
+cell keys, stable sort by cell key with the state moved into key order, cell
+table; density + Tait EOS gather; momentum + viscosity gather fused with the
+symplectic-Euler update).  The headline workload is the largest single-GPU
+configuration: 4,019,679 fluid particles (dx = 3.15 mm, a 159^3 block) plus
+the 2,006,931 particles of a 3-layer tank, in FP64.  `value` is
 particle-updates/s = fluid particles x steps / wall time of the timed steps,
-inputs resident in HBM.  Also reported, under "forms": the other precision
-forms, the 4M-particle mixed-precision workload with its error against FP64
-after 1, 10, 100 and 1000 steps, and the forward + adjoint cost of the two
-differentiable workloads.
+inputs resident in HBM.  Also reported, under "forms": the 1M dam break in
+FP64, FP32 and mixed precision, the 4M mixed form with its error against
+FP64 after 1, 10, 100 and 1000 steps, the 10,000-sphere DEM lattice in FP64,
+and the forward + adjoint cost of the two differentiable workloads.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f64|f32|mixed]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f64|f32|mixed] [--workload 4M|1M]
 """
 from __future__ import annotations
 
@@ -50,13 +51,24 @@ def _context(pa, scene_id, scene_args, precision):
     return d, arrs, ctx
 
 
-def run_form(pa, scene_args, precision, steps, warmup):
+def run_form(pa, scene_args, precision, steps, warmup, scene_id=None):
     """Throughput over `steps` timed steps with nothing recorded between the
     kernels (a HIP event pair around every launch stretches the stream by
     about 0.15 ms per step).  Then the same steps again from the initial
     state, with HIP events around every kernel on the library's stream: each
     kernel's average duration over the same stretch of the simulation."""
-    d, arrs, ctx = _context(pa, pa.SCENE_DAM_BREAK, scene_args, precision)
+    scene_id = pa.SCENE_DAM_BREAK if scene_id is None else scene_id
+    d, arrs, ctx = _context(pa, scene_id, scene_args, precision)
+    dem = scene_id != pa.SCENE_DAM_BREAK
+    zeros = np.zeros(d.n_particles)
+
+    def reset():
+        for k in STATE:
+            ctx.upload(k, arrs[k])
+        if dem:
+            for k in ("wx", "wy", "wz"):
+                ctx.upload(k, zeros)
+
     with ctx:
         ctx.step(warmup)
         ctx.sync()
@@ -64,8 +76,7 @@ def run_form(pa, scene_args, precision, steps, warmup):
         ctx.step(steps)
         ctx.sync()
         t1 = time.perf_counter()
-        for k in STATE:
-            ctx.upload(k, arrs[k])
+        reset()
         ctx.step(warmup)
         ctx.sync()
         ctx.timing_enable(True)   # per-kernel HIP events on the library's stream
@@ -79,33 +90,47 @@ def run_form(pa, scene_args, precision, steps, warmup):
             "n_total": int(d.n_particles), "kernels": kt}
 
 
-def measured_traffic():
-    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/)."""
-    path = ROOT / "profiles" / "pmc_traffic_wcsph_1M.md"
+def _profile_table(name, tag):
+    """`tag: k=v, ...` line of a committed rocprofv3 summary under profiles/ (empty when absent)."""
+    path = ROOT / "profiles" / name
     if not path.exists():
         return {}
     for line in path.read_text().splitlines():
-        if line.startswith("traffic_bytes:"):
+        if line.startswith(tag + ":"):
             pairs = [p.strip().split("=") for p in line.split(":", 1)[1].split(",") if "=" in p]
-            return {k: int(v) for k, v in pairs}
+            return {k: float(v) for k, v in pairs}
     return {}
 
 
-def roofline(kernels, workload_1m=True):
-    """Dominant kernel: algorithmic bytes per launch / its HIP-event average duration, vs 8 TB/s; traffic
-    is the rocprofv3-measured HBM bytes per launch of the same kernel on the 1M workload."""
+# committed counter summaries per workload: HBM bytes per launch (FETCH_SIZE / WRITE_SIZE runs) and the
+# VALU issue share of each gather (SQ counter runs)
+PROFILES = {"4M": ("pmc_traffic_wcsph_4M.md", "pmc_valu_wcsph_4M.md"),
+            "1M": ("pmc_traffic_wcsph_1M.md", "pmc_valu_wcsph_1M.md")}
+
+
+def roofline(kernels, workload=None):
+    """Dominant kernel: algorithmic bytes per launch / its HIP-event average duration, vs 8 TB/s.  traffic is
+    the rocprofv3-measured HBM bytes per launch of the same kernel on the same workload; `limiter` says
+    whether the counters show it bound by VALU instruction issue or by HBM (MFMA plays no part: nothing
+    here is a dense contraction, so the roof it is priced against is HBM)."""
     if not kernels:
         return None
     dom = max(kernels, key=lambda k: k["total_ms"])
     avg_s = dom["total_ms"] / dom["launches"] / 1e3
     achieved = dom["bytes_per_launch"] / avg_s / 1e9
-    traffic = measured_traffic().get(dom["kernel"]) if workload_1m else None
+    tfile, vfile = PROFILES.get(workload, (None, None))
+    traffic = _profile_table(tfile, "traffic_bytes").get(dom["kernel"]) if tfile else None
+    valu = _profile_table(vfile, "valu_issue").get(dom["kernel"]) if vfile else None
     out = {"kernel": dom["kernel"], "bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
-           "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": traffic,
+           "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5),
+           "traffic": int(traffic) if traffic else None,
            "avg_us": round(avg_s * 1e6, 3), "bytes_per_launch": dom["bytes_per_launch"]}
     if traffic:
         out["traffic_GB_s"] = round(traffic / avg_s / 1e9, 1)
         out["traffic_frac"] = round(traffic / avg_s / 1e9 / PEAK_HBM_GBS, 4)
+    if valu is not None:
+        out["valu_issue_frac"] = valu
+        out["limiter"] = "valu-issue" if valu > out.get("traffic_frac", 0.0) else "hbm"
     return out
 
 
@@ -274,24 +299,74 @@ def copy_probe(nbytes):
     log(f"[bench] copy probe: 3 x {nbytes} bytes device to device")
 
 
-def cpu_baseline(pa, scene_args, budget_s=12.0):
-    """FP64 oracle (OpenMP over particles) on the host cores, bounded sample of the same workload."""
+def _cpu_rate(pa, scene_id, scene_args, threads, budget_s, max_steps=50):
+    """FP64 oracle steps per second on `threads` host threads, over about budget_s of steps after one
+    warm-up step (first touch, thread pool)."""
     import oracle
 
-    d, arrs = pa.scene(pa.SCENE_DAM_BREAK, **scene_args)
-    st = oracle.State(oracle.params_from_desc(d), arrs)
-    st.step(1)   # warm-up (first touch, thread pool)
-    t0 = time.perf_counter()
-    st.step(1)
-    one = time.perf_counter() - t0
-    k = max(1, min(50, int(budget_s / max(one, 1e-6))))
-    t0 = time.perf_counter()
-    st.step(k)
-    sec = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-    return {"value": float(d.n_moving) * k / sec, "unit": "particle-updates/s", "cores": cores, "kind": "port",
-            "sample": f"{k} FP64 steps of the same {d.n_particles}-particle dam break (after 2 warm-up steps), "
-                      f"C oracle, OpenMP {cores} threads"}
+    d, arrs = pa.scene(scene_id, **scene_args)
+    prev = oracle.set_threads(threads)
+    try:
+        st = oracle.State(oracle.params_from_desc(d), arrs)
+        assert st.step(1) == 0
+        t0 = time.perf_counter()
+        assert st.step(1) == 0
+        one = time.perf_counter() - t0
+        k = max(1, min(max_steps, int(budget_s / max(one, 1e-6))))
+        t0 = time.perf_counter()
+        assert st.step(k) == 0
+        sec = time.perf_counter() - t0
+        st.close()
+    finally:
+        oracle.set_threads(prev)
+    return d, k, sec
+
+
+def host_cores():
+    """Threads the host grants this process (the GPU box exports OMP_NUM_THREADS for its CPU share)."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(pa, scene_args, budget_s=10.0):
+    """FP64 oracle (OpenMP over particles) on the host cores, bounded sample of the same workload: all the
+    host threads, and one thread."""
+    cores = host_cores()
+    d, k, sec = _cpu_rate(pa, pa.SCENE_DAM_BREAK, scene_args, cores, budget_s)
+    out = {"value": float(d.n_moving) * k / sec, "unit": "particle-updates/s", "cores": cores, "kind": "port",
+           "sample": f"{k} FP64 steps of the same {d.n_particles}-particle dam break (after 2 warm-up "
+                     f"steps), C oracle, OpenMP {cores} threads"}
+    d1, k1, sec1 = _cpu_rate(pa, pa.SCENE_DAM_BREAK, scene_args, 1, budget_s, max_steps=5)
+    out["one_core"] = {"value": float(d1.n_moving) * k1 / sec1, "cores": 1,
+                       "sample": f"{k1} FP64 steps of the same dam break, 1 thread"}
+    return out
+
+
+def dem_lattice(pa, steps, warmup):
+    """The 10,000-sphere FP64 DEM lattice (1000 steps in its configuration) on the device, next to the
+    FP64 oracle on the host cores for the same scene."""
+    scene = dict(n_target=10_000, seed=0)
+    r = run_form(pa, scene, "f64", steps, warmup, scene_id=pa.SCENE_DEM_LATTICE)
+    cores = host_cores()
+    d, k, sec = _cpu_rate(pa, pa.SCENE_DEM_LATTICE, scene, cores, 5.0, max_steps=1000)
+    return {"workload": f"DEM lattice FP64, {r['n_fluid']} spheres r = 0.01 m, kn 1e5, e 0.3, mu 0.5, "
+                        f"dt 1e-5, {steps} steps",
+            "value": r["value"], "ms_per_step": r["ms_per_step"], "kernels": kernel_table(r["kernels"]),
+            "cpu": {"value": float(d.n_moving) * k / sec, "cores": cores, "kind": "port",
+                    "sample": f"{k} FP64 steps of the same lattice, C oracle, OpenMP {cores} threads"}}
+
+
+WORKLOADS = {
+    "4M": (SCENE_4M, "WCSPH dam break, 4,019,679 fluid (159^3 block) + 2,006,931 tank particles, dx 3.15 mm, "
+                     "h = 1.2 dx, cubic spline, Tait gamma 7, alpha 0.1, dt = 0.25 h / c0"),
+    "1M": (SCENE_1M, "WCSPH dam break, 1,000,000 fluid + 799,320 tank particles, dx 5 mm, "
+                     "h = 1.2 dx, cubic spline, Tait gamma 7, alpha 0.1, dt = 0.25 h / c0"),
+}
 
 
 def main():
@@ -300,6 +375,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", choices=list(PREC), default="f64")
+    ap.add_argument("--workload", choices=list(WORKLOADS), default="4M")
     ap.add_argument("--no-forms", action="store_true", help="skip the other forms, accuracy and adjoint runs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -316,24 +392,29 @@ def main():
     pa.load_library()
     if args.calibrate_copy > 0:
         copy_probe(args.calibrate_copy)
-    log(f"[bench] main form {args.precision}: {args.steps} steps after {args.warmup} warm-up")
-    main_res = run_form(pa, SCENE_1M, args.precision, args.steps, args.warmup)
+    scene_main, workload_text = WORKLOADS[args.workload]
+    log(f"[bench] main form {args.precision} ({args.workload}): {args.steps} steps after {args.warmup} warm-up")
+    main_res = run_form(pa, scene_main, args.precision, args.steps, args.warmup)
     forms = {}
     if not args.no_forms:
         for p in PREC:
-            if p == args.precision:
+            if args.workload == "1M" and p == args.precision:
                 continue
             log(f"[bench] form {p} (1M)")
             r = run_form(pa, SCENE_1M, p, min(args.steps, 500), args.warmup)
-            forms[p] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "roofline": roofline(r["kernels"]),
-                        "kernels": kernel_table(r["kernels"])}
-        log("[bench] form mixed (4M, dx = 3.15 mm)")
-        r = run_form(pa, SCENE_4M, "mixed", min(args.steps, 300), args.warmup)
-        forms["mixed_4M"] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "n_fluid": r["n_fluid"],
-                             "n_total": r["n_total"], "roofline": roofline(r["kernels"], workload_1m=False),
-                             "kernels": kernel_table(r["kernels"])}
+            forms[f"{p}_1M"] = {"value": r["value"], "ms_per_step": r["ms_per_step"],
+                                "roofline": roofline(r["kernels"], "1M"),
+                                "kernels": kernel_table(r["kernels"])}
+        if not (args.workload == "4M" and args.precision == "mixed"):
+            log("[bench] form mixed (4M, dx = 3.15 mm)")
+            r = run_form(pa, SCENE_4M, "mixed", min(args.steps, 300), args.warmup)
+            forms["mixed_4M"] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "n_fluid": r["n_fluid"],
+                                 "n_total": r["n_total"], "roofline": roofline(r["kernels"], "4M"),
+                                 "kernels": kernel_table(r["kernels"])}
         log("[bench] accuracy of mixed / f32 vs f64 at 4M over 1000 steps")
-        forms["mixed_4M"]["error_vs_f64"] = accuracy(pa, SCENE_4M)
+        forms.setdefault("mixed_4M", {})["error_vs_f64"] = accuracy(pa, SCENE_4M)
+        log("[bench] DEM lattice 10k FP64, 1000 steps")
+        forms["dem_lattice_10k"] = dem_lattice(pa, 1000, 10)
         log("[bench] adjoint: WCSPH 1M mixed, 200 steps, K = 20")
         forms["adjoint_wcsph_1M"] = adjoint_wcsph(pa)
         log("[bench] adjoint: DEM pour 100k mixed, 500 steps, K = 50")
@@ -342,8 +423,8 @@ def main():
         forms["generated_density_1M"] = generated_density(pa, SCENE_1M)
     cpu = None
     if not args.no_cpu:
-        log("[bench] CPU baseline (oracle)")
-        cpu = cpu_baseline(pa, SCENE_1M)
+        log(f"[bench] CPU baseline (oracle, {args.workload})")
+        cpu = cpu_baseline(pa, scene_main)
     out = {
         "metric": "particle-updates/s/GPU",
         "value": main_res["value"],
@@ -358,13 +439,12 @@ def main():
         "dtype": DTYPE[args.precision],
         "data": "synthetic (deterministic dam-break lattice)",
         "config": {
-            "workload": "WCSPH dam break, 1,000,000 fluid + 799,320 tank particles, dx 5 mm, "
-                        "h = 1.2 dx, cubic spline, Tait gamma 7, alpha 0.1, dt = 0.25 h / c0",
+            "workload": workload_text,
             "n_fluid": main_res["n_fluid"],
             "n_total": main_res["n_total"],
             "precision": args.precision,
         },
-        "roofline": roofline(main_res["kernels"]),
+        "roofline": roofline(main_res["kernels"], args.workload),
         "cpu_baseline": cpu,
         "kernels": kernel_table(main_res["kernels"]),
         "forms": forms,

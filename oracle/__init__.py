@@ -10,7 +10,12 @@ Parity pins (tests/test_oracle.py): closed forms (spline normalisation,
 lattice density 1.000809548 rho0, 57 neighbours on an interior lattice site,
 discrete free fall, head-on restitution e, Coulomb sliding), brute-force
 all-pairs neighbour sets, momentum conservation, and complex-step vs central
-differences.
+differences.  WCSPH equation of state, pressure force and viscosity
+(tests/test_oracle_pins.py): the Tait closed form on the lattice interior,
+the pressure force as the exact gradient of the EOS internal energy,
+Monaghan's Pi_ij on an approaching pair, dissipation at p = 0 and a
+hydrostatic column; planted mistakes compiled into copies of these sources
+(load()) show that each check fails when its term is wrong.
 """
 from __future__ import annotations
 
@@ -66,41 +71,54 @@ def build() -> Path:
     return LIB_PATH
 
 
+def bind(L):
+    """Declare the C signatures of an oracle library handle (ctypes.CDLL)."""
+    P = C.POINTER(Params)
+    L.orc_cell_key.restype = C.c_int64
+    L.orc_cell_key.argtypes = [P, C.c_double, C.c_double, C.c_double]
+    L.orc_stable_sort.argtypes = [C.c_uint64, _U32, C.c_uint32, _U32]
+    L.orc_cell_lb.argtypes = [C.c_uint64, _U32, C.c_uint32, _U32]
+    L.orc_spline_f.restype = C.c_double
+    L.orc_spline_f.argtypes = [C.c_double]
+    L.orc_spline_df.restype = C.c_double
+    L.orc_spline_df.argtypes = [C.c_double]
+    L.orc_eq1_allpairs.argtypes = [C.c_uint64, _D, _D]
+    L.orc_create.restype = C.c_void_p
+    L.orc_create.argtypes = [P, _D, _D, _D, _D, _D, _D, _D, _D]
+    L.orc_destroy.argtypes = [C.c_void_p]
+    L.orc_build.restype = C.c_int
+    L.orc_build.argtypes = [C.c_void_p]
+    L.orc_neighbors.argtypes = [C.c_void_p, _U32, _U32, _U32, _U32]
+    L.orc_eq1_cells.argtypes = [C.c_void_p, C.c_double, _D, _D]
+    L.orc_density.restype = C.c_int
+    L.orc_density.argtypes = [C.c_void_p]
+    L.orc_step.restype = C.c_int
+    L.orc_step.argtypes = [C.c_void_p, C.c_uint32]
+    L.orc_set_threads.restype = C.c_int
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_get.restype = C.c_int
+    L.orc_get.argtypes = [C.c_void_p, C.c_char_p, _D]
+    L.orc_contact_count.restype = C.c_uint64
+    L.orc_contact_count.argtypes = [C.c_void_p]
+    L.orc_cs_tangent.restype = C.c_int
+    L.orc_cs_tangent.argtypes = [P, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, C.c_uint32, _D, _D, _D, _D]
+    return L
+
+
 def lib():
+    """The oracle built from this directory (compiled on first use)."""
     global _lib
     if _lib is None:
         if not LIB_PATH.exists():
             build()
-        L = C.CDLL(str(LIB_PATH))
-        P = C.POINTER(Params)
-        L.orc_cell_key.restype = C.c_int64
-        L.orc_cell_key.argtypes = [P, C.c_double, C.c_double, C.c_double]
-        L.orc_stable_sort.argtypes = [C.c_uint64, _U32, C.c_uint32, _U32]
-        L.orc_cell_lb.argtypes = [C.c_uint64, _U32, C.c_uint32, _U32]
-        L.orc_spline_f.restype = C.c_double
-        L.orc_spline_f.argtypes = [C.c_double]
-        L.orc_spline_df.restype = C.c_double
-        L.orc_spline_df.argtypes = [C.c_double]
-        L.orc_eq1_allpairs.argtypes = [C.c_uint64, _D, _D]
-        L.orc_create.restype = C.c_void_p
-        L.orc_create.argtypes = [P, _D, _D, _D, _D, _D, _D, _D, _D]
-        L.orc_destroy.argtypes = [C.c_void_p]
-        L.orc_build.restype = C.c_int
-        L.orc_build.argtypes = [C.c_void_p]
-        L.orc_neighbors.argtypes = [C.c_void_p, _U32, _U32, _U32, _U32]
-        L.orc_eq1_cells.argtypes = [C.c_void_p, C.c_double, _D, _D]
-        L.orc_density.restype = C.c_int
-        L.orc_density.argtypes = [C.c_void_p]
-        L.orc_step.restype = C.c_int
-        L.orc_step.argtypes = [C.c_void_p, C.c_uint32]
-        L.orc_get.restype = C.c_int
-        L.orc_get.argtypes = [C.c_void_p, C.c_char_p, _D]
-        L.orc_contact_count.restype = C.c_uint64
-        L.orc_contact_count.argtypes = [C.c_void_p]
-        L.orc_cs_tangent.restype = C.c_int
-        L.orc_cs_tangent.argtypes = [P, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, C.c_uint32, _D, _D, _D, _D]
-        _lib = L
+        _lib = bind(C.CDLL(str(LIB_PATH)))
     return _lib
+
+
+def load(path) -> C.CDLL:
+    """Another build of the oracle sources (a test that plants a mistake compiles one), bound but not
+    made the default."""
+    return bind(C.CDLL(str(path)))
 
 
 def _d(a):
@@ -129,10 +147,11 @@ def params_from_desc(desc) -> Params:
 
 
 class State:
-    """FP64 oracle state (orc_state*)."""
+    """FP64 oracle state (orc_state*), on the default oracle or on the library handle L."""
 
-    def __init__(self, params: Params, arrays: dict):
-        L = lib()
+    def __init__(self, params: Params, arrays: dict, L=None):
+        L = L or lib()
+        self.L = L
         self.p = params
         self.n = int(params.n)
         self._keep = {k: np.ascontiguousarray(v, np.float64) for k, v in arrays.items() if v is not None}
@@ -144,7 +163,7 @@ class State:
 
     def close(self):
         if self._h:
-            lib().orc_destroy(self._h)
+            self.L.orc_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -154,7 +173,7 @@ class State:
             pass
 
     def build(self) -> int:
-        return lib().orc_build(self._h)
+        return self.L.orc_build(self._h)
 
     def neighbors(self) -> dict:
         nc = int(self.p.dims[0]) * int(self.p.dims[1]) * int(self.p.dims[2])
@@ -163,30 +182,35 @@ class State:
         ids = np.empty(self.n, np.uint32)
         lb = np.empty(nc + 1, np.uint32)
         u = lambda a: a.ctypes.data_as(_U32)  # noqa: E731
-        lib().orc_neighbors(self._h, u(keys), u(perm), u(ids), u(lb))
+        self.L.orc_neighbors(self._h, u(keys), u(perm), u(ids), u(lb))
         return {"keys": keys, "perm": perm, "ids": ids, "lb": lb}
 
     def eq1_cells(self, cutoff: float, force, mass) -> np.ndarray:
         f = np.array(force, np.float64, copy=True)
         m = np.ascontiguousarray(mass, np.float64)
-        lib().orc_eq1_cells(self._h, cutoff, _d(f), _d(m))
+        self.L.orc_eq1_cells(self._h, cutoff, _d(f), _d(m))
         return f
 
     def density(self) -> int:
-        return lib().orc_density(self._h)
+        return self.L.orc_density(self._h)
 
     def step(self, nsteps: int = 1) -> int:
-        return lib().orc_step(self._h, int(nsteps))
+        return self.L.orc_step(self._h, int(nsteps))
 
     def get(self, name: str) -> np.ndarray:
         out = np.empty(self.n, np.float64)
-        st = lib().orc_get(self._h, name.encode(), _d(out))
+        st = self.L.orc_get(self._h, name.encode(), _d(out))
         if st:
             raise KeyError(name)
         return out
 
     def contact_count(self) -> int:
-        return int(lib().orc_contact_count(self._h))
+        return int(self.L.orc_contact_count(self._h))
+
+
+def set_threads(n: int) -> int:
+    """OpenMP threads of later oracle calls; returns the previous count."""
+    return int(lib().orc_set_threads(int(n)))
 
 
 def eq1_allpairs(force, mass) -> np.ndarray:

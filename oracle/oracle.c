@@ -162,6 +162,17 @@ int orc_density(orc_state* s) {
 
 int orc_step(orc_state* s, uint32_t nsteps) { return step_d(s, nsteps); }
 
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  int prev = omp_get_max_threads();
+  if (n >= 1) omp_set_num_threads(n);
+  return prev;
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
 static const double* pick_field_d(const orc_state* s, const char* name) {
   struct { const char* n; const double* p; } tab[] = {
       {"x", s->x}, {"y", s->y}, {"z", s->z}, {"vx", s->vx}, {"vy", s->vy},

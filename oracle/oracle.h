@@ -90,6 +90,9 @@ void orc_eq1_cells(const orc_state* s, double cutoff, double* force, const doubl
 int orc_density(orc_state* s);
 /* nsteps full steps; OpenMP threads = omp_get_max_threads(). */
 int orc_step(orc_state* s, uint32_t nsteps);
+/* Set the OpenMP thread count of later calls (n >= 1); returns the previous
+ * omp_get_max_threads(). */
+int orc_set_threads(int n);
 /* Field in id order: x y z vx vy vz wx wy wz rho p ax ay az radius mass key. */
 int orc_get(const orc_state* s, const char* name, double* out);
 /* Number of active DEM contacts summed over particles (pairs counted twice). */
