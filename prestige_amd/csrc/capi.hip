@@ -70,7 +70,7 @@ Timing::~Timing() {
 }
 
 LaunchScope::LaunchScope(Ctx* c, const char* name, double bytes) : ctx(c), stat(0) {
-  if (!c->timing.enabled) return;
+  if (!c || !c->timing.enabled) return;
   stat = c->timing.stat_index(name, bytes);
   a = c->timing.get_event();
   b = c->timing.get_event();
@@ -111,20 +111,7 @@ void Ctx::check_flags() {
 static void derive_constants(Ctx& c) {
   const prestige_desc& d = c.desc;
   Grid& g = c.grid;
-  for (int a = 0; a < 3; ++a) {
-    g.lo[a] = d.grid_lo[a];
-    double span = d.grid_hi[a] - d.grid_lo[a];
-    PR_CHECK(span > 0.0, PRESTIGE_E_INVALID, "grid_hi must exceed grid_lo on every axis");
-    double cells = std::ceil(span / d.cell_size);
-    PR_CHECK(cells >= 1.0 && cells < 2.0e9, PRESTIGE_E_INVALID, "grid has too many cells along an axis");
-    g.dims[a] = (int32_t)cells;
-  }
-  g.cs = d.cell_size;
-  g.inv_cs = 1.0 / d.cell_size;
-  double nc = (double)g.dims[0] * (double)g.dims[1] * (double)g.dims[2];
-  PR_CHECK(nc < 4.0e9, PRESTIGE_E_INVALID, "grid has more than 4e9 cells");
-  g.n_cells = (uint32_t)nc;
-  g.off_scale = 1.0;
+  g = make_grid(d.grid_lo, d.grid_hi, d.cell_size);
 
   SphConst& s = c.sph;
   s.dt = d.dt;

@@ -49,6 +49,29 @@ struct Grid {
   uint32_t n_cells;
 };
 
+// Grid of the box [lo, hi) in cells of size cs: dims = ceil((hi - lo) / cs)
+// per axis (every context and the allocation-free entry point build it
+// here, so their keys agree bit for bit).
+inline Grid make_grid(const double lo[3], const double hi[3], double cs) {
+  Grid g{};
+  PR_CHECK(cs > 0.0 && cs < 1.0e300, PRESTIGE_E_INVALID, "cell_size must be finite and > 0");
+  for (int a = 0; a < 3; ++a) {
+    g.lo[a] = lo[a];
+    const double span = hi[a] - lo[a];
+    PR_CHECK(span > 0.0, PRESTIGE_E_INVALID, "grid_hi must exceed grid_lo on every axis");
+    const double cells = __builtin_ceil(span / cs);
+    PR_CHECK(cells >= 1.0 && cells < 2.0e9, PRESTIGE_E_INVALID, "grid has too many cells along an axis");
+    g.dims[a] = (int32_t)cells;
+  }
+  g.cs = cs;
+  g.inv_cs = 1.0 / cs;
+  const double nc = (double)g.dims[0] * (double)g.dims[1] * (double)g.dims[2];
+  PR_CHECK(nc < 4.0e9, PRESTIGE_E_INVALID, "grid has more than 4e9 cells");
+  g.n_cells = (uint32_t)nc;
+  g.off_scale = 1.0;
+  return g;
+}
+
 // Constants of the WCSPH bodies, derived once on the host from the
 // descriptor.  Every device form reads the same FP64 values (FP32 forms
 // round them once).
@@ -377,6 +400,9 @@ __device__ inline int cell_shift(uint32_t wi, uint32_t wj, int axis) {
   uint32_t t = ((wj >> s) - (wi >> s) + 1u) & 3u;
   return (int)t - 1;
 }
+
+// Error message of calls without a context (prestige_last_error(NULL)).
+void set_context_free_error(const std::string& msg);
 
 // Per-form kernel entry points (defined in the .hip files).
 void neighbors_build(Ctx& c);

@@ -217,51 +217,68 @@ static int key_bits(uint32_t n_cells) {
   return b;
 }
 
-// Sorts keys[cur_key]; afterwards keys[cur_key] holds sorted keys and
-// vals[perm_buf] the permutation (sorted slot k <- old slot perm[k]).
-static void radix_sort(Ctx& c) {
-  const uint32_t n = (uint32_t)c.n;
-  const int bits = key_bits(c.grid.n_cells);
+// Buffers of one radix sort: key and value ping-pong pairs, the per-tile digit
+// histogram (kBuckets x tiles), the digit totals, and optionally the moved-slot
+// tile counts the sort clears for the next incremental sort.
+struct SortBufs {
+  uint32_t* keys[2];
+  uint32_t* vals[2];
+  uint32_t* hist;
+  uint32_t* tot;
+  uint32_t* tile_cnt;
+};
+
+// Sorts the n keys (< n_cells) in keys[src]; *key_out and *val_out receive
+// the buffers holding the sorted keys and the permutation (sorted slot k <-
+// old slot perm[k]).  c (may be null) records the launches when timing is on.
+static void radix_sort_bufs(Ctx* c, const SortBufs& b, int src, uint32_t n, uint32_t n_cells, hipStream_t st,
+                            int* key_out, int* val_out) {
+  const int bits = key_bits(n_cells);
   const int passes = (bits + kRadixBits - 1) / kRadixBits;
   const uint32_t n_tiles = (uint32_t)sort_tiles(n);
+  *key_out = src;
+  *val_out = 0;
   if (passes == 0) {
-    k_iota<<<blocks_for(n), kBlock, 0, c.stream>>>(n, c.vals[0].as<uint32_t>());
+    k_iota<<<blocks_for(n), kBlock, 0, st>>>(n, b.vals[0]);
     PR_HIP(hipGetLastError());
-    c.perm_buf = 0;
     return;
   }
-  int src = c.cur_key;
   int vsrc = -1;   // first pass generates slot indices itself
   for (int p = 0; p < passes; ++p) {
     const int shift = p * kRadixBits;
     const int dst = src ^ 1;
     const int vdst = (vsrc < 0) ? 0 : (vsrc ^ 1);
     {
-      LaunchScope ls(&c, "sort_hist", 4.0 * n);
-      k_sort_hist<<<n_tiles, kSortThreads, 0, c.stream>>>(c.keys[src].as<uint32_t>(), n, shift,
-                                                          n_tiles, c.sort_hist.as<uint32_t>(),
-                                                          p == 0 ? c.inc_tiles.as<uint32_t>() : nullptr);
+      LaunchScope ls(c, "sort_hist", 4.0 * n);
+      k_sort_hist<<<n_tiles, kSortThreads, 0, st>>>(b.keys[src], n, shift, n_tiles, b.hist,
+                                                    p == 0 ? b.tile_cnt : nullptr);
       PR_HIP(hipGetLastError());
     }
     {
-      LaunchScope ls(&c, "sort_scan", 8.0 * kBuckets * n_tiles);
-      k_sort_scan<<<kBuckets, kSortThreads, 0, c.stream>>>(c.sort_hist.as<uint32_t>(), n_tiles,
-                                                           c.sort_tot.as<uint32_t>());
+      LaunchScope ls(c, "sort_scan", 8.0 * kBuckets * n_tiles);
+      k_sort_scan<<<kBuckets, kSortThreads, 0, st>>>(b.hist, n_tiles, b.tot);
       PR_HIP(hipGetLastError());
     }
     {
-      LaunchScope ls(&c, "sort_scatter", (vsrc < 0 ? 12.0 : 16.0) * n);
-      k_sort_scatter<<<n_tiles, kSortThreads, 0, c.stream>>>(
-          c.keys[src].as<uint32_t>(), vsrc < 0 ? nullptr : c.vals[vsrc].as<uint32_t>(),
-          c.keys[dst].as<uint32_t>(), c.vals[vdst].as<uint32_t>(), n, shift, n_tiles,
-          c.sort_hist.as<uint32_t>(), c.sort_tot.as<uint32_t>());
+      LaunchScope ls(c, "sort_scatter", (vsrc < 0 ? 12.0 : 16.0) * n);
+      k_sort_scatter<<<n_tiles, kSortThreads, 0, st>>>(b.keys[src], vsrc < 0 ? nullptr : b.vals[vsrc], b.keys[dst],
+                                                       b.vals[vdst], n, shift, n_tiles, b.hist, b.tot);
       PR_HIP(hipGetLastError());
     }
     src = dst;
     vsrc = vdst;
   }
-  c.cur_key = src;
-  c.perm_buf = vsrc;
+  *key_out = src;
+  *val_out = vsrc;
+}
+
+// Sorts keys[cur_key]; afterwards keys[cur_key] holds sorted keys and
+// vals[perm_buf] the permutation (sorted slot k <- old slot perm[k]).
+static void radix_sort(Ctx& c) {
+  const SortBufs b{{c.keys[0].as<uint32_t>(), c.keys[1].as<uint32_t>()},
+                   {c.vals[0].as<uint32_t>(), c.vals[1].as<uint32_t>()},
+                   c.sort_hist.as<uint32_t>(), c.sort_tot.as<uint32_t>(), c.inc_tiles.as<uint32_t>()};
+  radix_sort_bufs(&c, b, c.cur_key, (uint32_t)c.n, c.grid.n_cells, c.stream, &c.cur_key, &c.perm_buf);
 }
 
 // ---------------------------------------------------------------------------
@@ -959,3 +976,4 @@ void eq1_eval(Ctx& c, int loop, double cutoff) {
 }
 
 }  // namespace prestige
+
