@@ -141,30 +141,40 @@ def kernel_table(kernels):
 
 
 def accuracy(pa, scene_args, marks=(1, 10, 100, 1000)):
-    """Norm-wise relative error of the mixed and FP32 forms against the FP64 form (bit-identical to the
-    FP64 oracle) over fluid particles, at full size."""
+    """Norm-wise relative error of the reduced forms against the FP64 form (bit-identical to the FP64
+    oracle) over fluid particles, at full size: mixed, mixed with the density pass in FP64
+    (PRESTIGE_FLAG_DENSITY_F64), and FP32."""
+    forms = {"f64": ("f64", 0), "mixed": ("mixed", 0), "mixed_rho64": ("mixed", 1), "f32": ("f32", 0)}
     ctxs = {}
-    for p in ("f64", "mixed", "f32"):
-        d, _, ctxs[p] = _context(pa, pa.SCENE_DAM_BREAK, scene_args, p)
+    for name, (p, flags) in forms.items():
+        d, arrs = pa.scene(pa.SCENE_DAM_BREAK, **scene_args)
+        d.precision = PREC[p]
+        d.flags = flags
+        ctxs[name] = pa.Context(d)
+        for k in STATE:
+            ctxs[name].upload(k, arrs[k])
     nf = int(d.n_moving)
-    out = {"steps": list(marks), "mixed": {"rho": [], "p": [], "v": [], "x": []},
-           "f32": {"rho": [], "p": [], "v": [], "x": []}}
+    out = {"steps": list(marks)}
+    out.update({name: {"rho": [], "p": [], "v": [], "x": []} for name in forms if name != "f64"})
     done = 0
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
     try:
         for m in marks:
             for c in ctxs.values():
                 c.step(m - done)
             done = m
             ref = {k: ctxs["f64"].download(k)[:nf] for k in ("rho", "p", "vx", "vy", "vz", "x", "y", "z")}
-            for p in ("mixed", "f32"):
-                got = {k: ctxs[p].download(k)[:nf] for k in ref}
-                rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
-                out[p]["rho"].append(rel(got["rho"], ref["rho"]))
-                out[p]["p"].append(rel(got["p"], ref["p"]))
-                out[p]["v"].append(rel(np.stack([got[k] for k in ("vx", "vy", "vz")]),
-                                       np.stack([ref[k] for k in ("vx", "vy", "vz")])))
-                out[p]["x"].append(rel(np.stack([got[k] for k in "xyz"]), np.stack([ref[k] for k in "xyz"])))
-            log(f"[bench]   errors at step {m}: mixed v {out['mixed']['v'][-1]:.2e} p {out['mixed']['p'][-1]:.2e}")
+            for name in out:
+                if name == "steps":
+                    continue
+                got = {k: ctxs[name].download(k)[:nf] for k in ref}
+                out[name]["rho"].append(rel(got["rho"], ref["rho"]))
+                out[name]["p"].append(rel(got["p"], ref["p"]))
+                out[name]["v"].append(rel(np.stack([got[k] for k in ("vx", "vy", "vz")]),
+                                          np.stack([ref[k] for k in ("vx", "vy", "vz")])))
+                out[name]["x"].append(rel(np.stack([got[k] for k in "xyz"]), np.stack([ref[k] for k in "xyz"])))
+            log(f"[bench]   errors at step {m}: mixed v {out['mixed']['v'][-1]:.2e} p {out['mixed']['p'][-1]:.2e}; "
+                f"mixed_rho64 v {out['mixed_rho64']['v'][-1]:.2e} p {out['mixed_rho64']['p'][-1]:.2e}")
     finally:
         for c in ctxs.values():
             c.close()
@@ -251,6 +261,111 @@ def adjoint_dem(pa, steps=500, every=50):
             "forward_value": n * steps / t_fwd,
             "loss": L, "dL_dkn": float(gp[0]), "dL_de": float(gp[1]), "dL_dmu": float(gp[2]),
             "dL_dv0_norm": float(np.linalg.norm(gv0))}
+
+
+def _grad_fd_check(pa, scene_id, scene_args, steps, every, loss_of, params, ndir=8, rel_eps=(1e-9, 1e-10, 1e-11),
+                   seed=5, v_scale=0.1, events=False):
+    """Directional derivatives of the loss along `ndir` random directions in (parameters, initial
+    velocities): the FP64 and mixed adjoints against central differences of the FP64 device forward.
+    Each difference uses the largest relative step in rel_eps whose two runs take the same branches as
+    the unperturbed run (DEM: identical contact and slip event fingerprints; WCSPH: smooth, the first
+    step is used).  Returns per-direction records and the worst relative errors."""
+    d0, arrs = pa.scene(scene_id, **scene_args)
+    n = int(d0.n_particles)
+    rng = np.random.default_rng(seed)
+    dirs = []
+    for _ in range(ndir):
+        dp = rng.standard_normal(len(params)) * np.array([abs(getattr(d0, k)) for k in params])
+        dv = rng.standard_normal((3, n)) * v_scale
+        dirs.append((dp, dv))
+    zeros = np.zeros(n)
+
+    def forward(dp, dv, eps):
+        d = d0.copy()
+        d.precision = PREC["f64"]
+        if events:
+            d.flags = pa.FLAG_DEM_EVENTS
+        for k, name in enumerate(params):
+            setattr(d, name, getattr(d0, name) + eps * dp[k])
+            if name == "kn":   # kt = 2/7 kn moves with kn
+                d.kt = d0.kt * d.kn / d0.kn
+        with pa.Context(d) as ctx:
+            for k in STATE + ("radius", "mass"):
+                if k in arrs:
+                    a = arrs[k]
+                    if k in ("vx", "vy", "vz"):
+                        a = a + eps * dv["xyz".index(k[1])]
+                    ctx.upload(k, a)
+            if scene_id != pa.SCENE_DAM_BREAK:
+                for k in ("wx", "wy", "wz"):
+                    ctx.upload(k, zeros)
+            ctx.step(steps)
+            x = np.stack([ctx.download(k) for k in "xyz"])
+            v = np.stack([ctx.download(k) for k in ("vx", "vy", "vz")])
+            ev = (np.stack([ctx.download("contact_events"), ctx.download("slip_events")]) if events else None)
+        return loss_of(d, x, v)[0], ev
+
+    grads = {}
+    for prec in ("f64", "mixed"):
+        d, ar, ctx = _context(pa, scene_id, scene_args, prec)
+        with ctx:
+            if scene_id != pa.SCENE_DAM_BREAK:
+                for k in ("wx", "wy", "wz"):
+                    ctx.upload(k, zeros)
+            ctx.forward_tape(steps, every)
+            x = np.stack([ctx.download(k) for k in "xyz"])
+            v = np.stack([ctx.download(k) for k in ("vx", "vy", "vz")])
+            _, gx, gv = loss_of(d, x, v)
+            grads[prec] = ctx.adjoint(gx, gv)
+    _, ev0 = forward(np.zeros(len(params)), np.zeros((3, n)), 0.0)
+    recs = []
+    for k, (dp, dv) in enumerate(dirs):
+        rec = {"direction": k}
+        for eps in rel_eps:
+            lp, evp = forward(dp, dv, eps)
+            lm, evm = forward(dp, dv, -eps)
+            same = (not events) or (np.array_equal(evp, ev0) and np.array_equal(evm, ev0))
+            rec.update({"eps": eps, "same_events": bool(same), "fd": (lp - lm) / (2 * eps)})
+            if same:
+                break
+        for prec, (gp, gv0) in grads.items():
+            pred = float(gp @ dp) + float((gv0 * dv).sum())
+            rec[f"adjoint_{prec}"] = pred
+            rec[f"rel_err_{prec}"] = abs(pred - rec["fd"]) / max(abs(rec["fd"]), 1e-300)
+        log(f"[bench]   direction {k}: fd {rec['fd']:.6e} f64 {rec['rel_err_f64']:.2e} "
+            f"mixed {rec['rel_err_mixed']:.2e} eps {rec['eps']:.0e} same events {rec['same_events']}")
+        recs.append(rec)
+    ok = [r for r in recs if r["same_events"]]
+    return {"directions": recs, "valid_directions": len(ok),
+            "max_rel_err_f64": max((r["rel_err_f64"] for r in ok), default=None),
+            "max_rel_err_mixed": max((r["rel_err_mixed"] for r in ok), default=None)}
+
+
+def fd_dem(pa, steps=500, every=50):
+    """Pour of 100,000 spheres, 500 steps: gradients w.r.t. kn, e, mu and the initial velocities."""
+    from prestige_amd import losses
+
+    scene = dict(n_target=100_000, seed=0)
+    _, arrs = pa.scene(pa.SCENE_DEM_POUR, **scene)
+    target = losses.heap_target(np.stack([arrs[k] for k in "xyz"]), seed=11)
+    return _grad_fd_check(pa, pa.SCENE_DEM_POUR, scene, steps, every,
+                          lambda d, x, v: losses.heap_loss(x, target), ("kn", "restitution", "mu"),
+                          v_scale=0.05, events=True)
+
+
+def fd_wcsph(pa, steps=200, every=20):
+    """Dam break of 1,000,000 fluid particles, 200 steps: gradients w.r.t. alpha, c0 and the initial
+    velocities (dt fixed)."""
+    from prestige_amd import losses
+
+    d0, arrs = pa.scene(pa.SCENE_DAM_BREAK, **SCENE_1M)
+    nf = int(d0.n_moving)
+    dx = d0.h / 1.2
+    extent = (d0.grid_hi[0] - 3 * dx, d0.grid_hi[1] - 3 * dx)
+    target = losses.dam_break_target(np.stack([arrs[k] for k in "xyz"])[:, :nf], extent, seed=7)
+    return _grad_fd_check(pa, pa.SCENE_DAM_BREAK, SCENE_1M, steps, every,
+                          lambda d, x, v: losses.dam_break_loss(x, v, d.mass, nf, extent, target),
+                          ("alpha", "c0"), rel_eps=(1e-7,), v_scale=0.01)
 
 
 def generated_density(pa, scene_args, reps=20):
@@ -377,6 +492,9 @@ def main():
     ap.add_argument("--precision", choices=list(PREC), default="f64")
     ap.add_argument("--workload", choices=list(WORKLOADS), default="4M")
     ap.add_argument("--no-forms", action="store_true", help="skip the other forms, accuracy and adjoint runs")
+    ap.add_argument("--forms", default="all",
+                    help="comma-separated subset of the forms to run (f64_1M, f32_1M, mixed_1M, mixed_4M, accuracy, "
+                         "dem_lattice_10k, adjoint_wcsph_1M, adjoint_dem_100k, fd_dem, fd_wcsph, generated)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="first run a plain device-to-device copy of BYTES (a reference kernel of known "
@@ -396,29 +514,41 @@ def main():
     log(f"[bench] main form {args.precision} ({args.workload}): {args.steps} steps after {args.warmup} warm-up")
     main_res = run_form(pa, scene_main, args.precision, args.steps, args.warmup)
     forms = {}
-    if not args.no_forms:
-        for p in PREC:
-            if args.workload == "1M" and p == args.precision:
-                continue
+    want = set(args.forms.split(",")) if args.forms != "all" else None
+    run = lambda name: not args.no_forms and (want is None or name in want)  # noqa: E731
+    for p in PREC:
+        if args.workload == "1M" and p == args.precision:
+            continue
+        if run(f"{p}_1M"):
             log(f"[bench] form {p} (1M)")
             r = run_form(pa, SCENE_1M, p, min(args.steps, 500), args.warmup)
             forms[f"{p}_1M"] = {"value": r["value"], "ms_per_step": r["ms_per_step"],
-                                "roofline": roofline(r["kernels"], "1M"),
-                                "kernels": kernel_table(r["kernels"])}
-        if not (args.workload == "4M" and args.precision == "mixed"):
-            log("[bench] form mixed (4M, dx = 3.15 mm)")
-            r = run_form(pa, SCENE_4M, "mixed", min(args.steps, 300), args.warmup)
-            forms["mixed_4M"] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "n_fluid": r["n_fluid"],
-                                 "n_total": r["n_total"], "roofline": roofline(r["kernels"], "4M"),
-                                 "kernels": kernel_table(r["kernels"])}
-        log("[bench] accuracy of mixed / f32 vs f64 at 4M over 1000 steps")
+                                "roofline": roofline(r["kernels"], "1M"), "kernels": kernel_table(r["kernels"])}
+    if run("mixed_4M") and not (args.workload == "4M" and args.precision == "mixed"):
+        log("[bench] form mixed (4M, dx = 3.15 mm)")
+        r = run_form(pa, SCENE_4M, "mixed", min(args.steps, 300), args.warmup)
+        forms["mixed_4M"] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "n_fluid": r["n_fluid"],
+                             "n_total": r["n_total"], "roofline": roofline(r["kernels"], "4M"),
+                             "kernels": kernel_table(r["kernels"])}
+    if run("accuracy"):
+        log("[bench] accuracy of the reduced forms vs f64 at 4M over 1000 steps")
         forms.setdefault("mixed_4M", {})["error_vs_f64"] = accuracy(pa, SCENE_4M)
+    if run("dem_lattice_10k"):
         log("[bench] DEM lattice 10k FP64, 1000 steps")
         forms["dem_lattice_10k"] = dem_lattice(pa, 1000, 10)
+    if run("adjoint_wcsph_1M"):
         log("[bench] adjoint: WCSPH 1M mixed, 200 steps, K = 20")
         forms["adjoint_wcsph_1M"] = adjoint_wcsph(pa)
+    if run("adjoint_dem_100k"):
         log("[bench] adjoint: DEM pour 100k mixed, 500 steps, K = 50")
         forms["adjoint_dem_100k"] = adjoint_dem(pa)
+    if run("fd_dem"):
+        log("[bench] adjoint vs central differences: DEM pour 100k, 500 steps, 8 directions")
+        forms.setdefault("adjoint_dem_100k", {})["fd_check"] = fd_dem(pa)
+    if run("fd_wcsph"):
+        log("[bench] adjoint vs central differences: WCSPH 1M, 200 steps, 8 directions")
+        forms.setdefault("adjoint_wcsph_1M", {})["fd_check"] = fd_wcsph(pa)
+    if run("generated"):
         log("[bench] generated density + EOS pass (1M)")
         forms["generated_density_1M"] = generated_density(pa, SCENE_1M)
     cpu = None

@@ -43,11 +43,14 @@ typedef enum prestige_status {
   PRESTIGE_E_INVALID = 1,     /* bad argument, unknown field name, wrong length */
   PRESTIGE_E_HIP = 2,         /* HIP runtime error (message in prestige_last_error) */
   PRESTIGE_E_OOM = 3,         /* device allocation failed */
-  PRESTIGE_E_CAPACITY = 4,    /* more than max_contacts DEM partners on one particle */
+  PRESTIGE_E_CAPACITY = 4,    /* more than max_contacts DEM partners on one particle, or (WCSPH)
+                                 more than 128 particles in one row of three cells */
   PRESTIGE_E_NONFINITE = 5,   /* NaN or Inf found in a state field */
   PRESTIGE_E_DOMAIN = 6,      /* particle outside the cell grid */
   PRESTIGE_E_UNSUPPORTED = 7, /* valid call that this model/precision does not provide */
-  PRESTIGE_E_STATE = 8        /* call out of order (e.g. adjoint before a taped forward) */
+  PRESTIGE_E_STATE = 8,       /* call out of order (e.g. adjoint before a taped forward) */
+  PRESTIGE_E_CFL = 9          /* a particle outran the time step: WCSPH dt (c0 + |v|) > h / 2
+                                 (or dt c0 > h / 2 before any step), DEM dt |v| > r / 4 */
 } prestige_status;
 
 typedef enum prestige_model {
@@ -115,8 +118,24 @@ typedef struct prestige_desc {
   double wall_lo[3];          /* half-space walls x>=lo, x<=hi per axis */
   double wall_hi[3];
   uint32_t wall_mask;         /* bit 2a: wall at lo of axis a; bit 2a+1: wall at hi */
-  uint32_t flags;             /* reserved, 0 */
+  uint32_t flags;             /* PRESTIGE_FLAG_* bits, 0 for none */
 } prestige_desc;
+
+/* prestige_desc.flags.  PRESTIGE_FLAG_DENSITY_F64: with PRESTIGE_MIXED WCSPH,
+ * the density pass runs the F64 form's pair arithmetic and sums (its rho and p
+ * are the F64 form's bit for bit from the same positions) and only the force
+ * pass is FP32 with Kahan sums; "rho" and "p" download the FP64 values.  Not
+ * available for prestige_forward_tape (E_UNSUPPORTED). */
+#define PRESTIGE_FLAG_DENSITY_F64 1u
+/* PRESTIGE_FLAG_DEM_EVENTS: DEM contexts keep two event fingerprints per
+ * particle over the prestige_step calls since the last position upload:
+ * "contact_events" = sum over steps s (1, 2, ... counted from that upload) of
+ * s times the particle's contact count (pairs and walls) at step s, and
+ * "slip_events" = the same sum over its contacts at the Coulomb cap, both
+ * modulo 2^32, downloaded as doubles.  Two runs whose fingerprints agree
+ * opened, closed and slipped the same contacts at the same steps (the check
+ * a finite-difference gradient of the piecewise-smooth DEM loss needs). */
+#define PRESTIGE_FLAG_DEM_EVENTS 2u
 
 /* Opaque context: owns every device buffer and one HIP stream. */
 typedef struct prestige_ctx prestige_ctx;
@@ -166,7 +185,11 @@ prestige_status prestige_eval_equation(prestige_ctx* ctx, int32_t eq_id,
                                        int32_t loop, double cutoff);
 
 /* Advance nsteps full steps (neighbour rebuild, gathers, integrator) on the
- * device; one host synchronisation at the end to collect error flags. */
+ * device.  The kernels' error flags (E_NONFINITE, E_CFL, E_CAPACITY, E_DOMAIN,
+ * reported in that order of precedence) are read every 16 steps and after
+ * the last: a bad step ends the call within 16 steps, the state left is the
+ * state after the step that was last read, and prestige_last_error names the
+ * stretch of steps. */
 prestige_status prestige_step(prestige_ctx* ctx, uint32_t nsteps);
 
 /* Reverse mode.  prestige_forward_tape runs nsteps like prestige_step from

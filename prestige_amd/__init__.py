@@ -40,10 +40,11 @@ LOOP_ALLPAIRS, LOOP_CELLS = 0, 1
 EQ1, EQ_SPH_DENSITY, EQ_SPH_EOS, EQ_SPH_MOMENTUM, EQ_INTEGRATE, EQ_DEM_CONTACT = range(6)
 SCENE_DEM_LATTICE, SCENE_DAM_BREAK, SCENE_DEM_POUR = 0, 1, 2
 ROLE_I, ROLE_J = 1, 2
+FLAG_DENSITY_F64, FLAG_DEM_EVENTS = 1, 2
 
 STATUS = {
     0: "OK", 1: "E_INVALID", 2: "E_HIP", 3: "E_OOM", 4: "E_CAPACITY", 5: "E_NONFINITE",
-    6: "E_DOMAIN", 7: "E_UNSUPPORTED", 8: "E_STATE",
+    6: "E_DOMAIN", 7: "E_UNSUPPORTED", 8: "E_STATE", 9: "E_CFL",
 }
 
 

@@ -127,6 +127,8 @@ void tape_forward(Ctx& c, uint32_t nsteps, uint32_t every) {
   PR_CHECK(every >= 1, PRESTIGE_E_INVALID, "checkpoint_every must be >= 1");
   PR_CHECK(nsteps >= 1, PRESTIGE_E_INVALID, "nsteps must be >= 1");
   if (c.desc.model == PRESTIGE_MODEL_WCSPH) {
+    PR_CHECK(!(c.desc.flags & PRESTIGE_FLAG_DENSITY_F64), PRESTIGE_E_UNSUPPORTED,
+             "the adjoint does not cover PRESTIGE_FLAG_DENSITY_F64");
     if (c.desc.precision == PRESTIGE_F64) sph_tape_forward<kF64>(c, nsteps, every);
     else if (c.desc.precision == PRESTIGE_MIXED) sph_tape_forward_mixed(c, nsteps, every);
     else throw Error(PRESTIGE_E_UNSUPPORTED, "the WCSPH adjoint runs in F64 or MIXED precision");

@@ -238,14 +238,14 @@ __global__ __launch_bounds__(kAdjBlock, kAdjForceWaves<F>) void k_adj_force(
         }
       };
       walk_runs_staged<kUnr, kForceCap>(
-          rec.meta, rec.words, n, i, live, lane,
+          rec.meta, rec.words, n, i, live, lane, [](uint32_t) {},
           [&](uint32_t k, uint32_t j) {
             s_p[k] = gpos[j];
             s_v[k] = gvel[j];
             s_rp[k] = rp[j];
             s_c[k] = cvec[j];
           },
-          [&](const uint32_t* jj) {
+          [&](const uint32_t* jj, const uint32_t*) {
             body(jj, [&](uint32_t k, G4& p, G4& v, R2& r, C4& cc) {
               p = s_p[k];
               v = s_v[k];
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(kAdjBlock, kAdjForceWaves<F>) void k_adj_force(
               cc = s_c[k];
             });
           },
-          [&](const uint32_t* jj) {
+          [&](const uint32_t* jj, const uint32_t*) {
             body(jj, [&](uint32_t j, G4& p, G4& v, R2& r, C4& cc) {
               p = gpos[j];
               v = gvel[j];
@@ -342,18 +342,18 @@ __global__ __launch_bounds__(kAdjBlock, kAdjDensityWaves<F>) void k_adj_density(
       }
     };
     walk_runs_staged<kUnr, kForceCap>(
-        rec.meta, rec.words, n, i, live, lane,
+        rec.meta, rec.words, n, i, live, lane, [](uint32_t) {},
         [&](uint32_t k, uint32_t j) {
           s_p[k] = gpos[j];
           s_rb[k] = rhob[j];
         },
-        [&](const uint32_t* jj) {
+        [&](const uint32_t* jj, const uint32_t*) {
           body(jj, [&](uint32_t k, G4& p, ar& rb) {
             p = s_p[k];
             rb = s_rb[k];
           });
         },
-        [&](const uint32_t* jj) {
+        [&](const uint32_t* jj, const uint32_t*) {
           body(jj, [&](uint32_t j, G4& p, ar& rb) {
             p = gpos[j];
             rb = rhob[j];

@@ -90,19 +90,23 @@ LaunchScope::~LaunchScope() noexcept(false) {
 // ---------------------------------------------------------------------------
 // flags
 // ---------------------------------------------------------------------------
-void Ctx::check_flags() {
+void Ctx::check_flags(const std::string& where) {
   uint32_t h[FLAG_COUNT] = {0, 0, 0, 0};
   PR_HIP(hipMemcpyAsync(h, flags.p, sizeof(h), hipMemcpyDeviceToHost, stream));
   PR_HIP(hipStreamSynchronize(stream));
   if (timing.enabled) timing.flush();
   bool any = h[0] | h[1] | h[2] | h[3];
   if (any) PR_HIP(hipMemsetAsync(flags.p, 0, sizeof(h), stream));
-  if (h[FLAG_DOMAIN]) throw Error(PRESTIGE_E_DOMAIN, "a particle left the cell grid");
+  const std::string at = where.empty() ? std::string() : " (" + where + ")";
+  // a non-finite or runaway particle usually also leaves the grid: name the cause first
+  if (h[FLAG_NONFINITE]) throw Error(PRESTIGE_E_NONFINITE, "non-finite acceleration" + at);
+  if (h[FLAG_CFL])
+    throw Error(PRESTIGE_E_CFL, "a particle outran the time step: WCSPH dt (c0 + |v|) > h / 2, DEM dt |v| > r / 4" + at);
   if (h[FLAG_CAPACITY])
     throw Error(PRESTIGE_E_CAPACITY,
                 "a row of three cells holds more than 128 candidates, or a particle has more DEM "
-                "contacts than max_contacts");
-  if (h[FLAG_NONFINITE]) throw Error(PRESTIGE_E_NONFINITE, "non-finite acceleration");
+                "contacts than max_contacts" + at);
+  if (h[FLAG_DOMAIN]) throw Error(PRESTIGE_E_DOMAIN, "a particle left the cell grid" + at);
 }
 
 // ---------------------------------------------------------------------------
@@ -131,6 +135,8 @@ static void derive_constants(Ctx& c) {
     s.msh = (d.mass * sigma) * s.inv_h;
     s.alpha_c0 = d.alpha * d.c0;
     s.B = ((d.rho0 * d.c0) * d.c0) / 7.0;
+    const double vmax = 0.5 * h / d.dt - d.c0;
+    s.cfl_v2 = vmax > 0.0 ? vmax * vmax : -1.0;
     g.off_scale = s.inv_h;
   }
   DemConst& k = c.dem;
@@ -216,6 +222,16 @@ static void allocate(Ctx& c) {
   c.acc.alloc(nn * e4);
   PR_HIP(hipMemsetAsync(c.rp.p, 0, nn * 2 * es, c.stream));
   PR_HIP(hipMemsetAsync(c.pres.p, 0, nn * es, c.stream));
+  if (c.desc.flags & PRESTIGE_FLAG_DENSITY_F64) {
+    c.rp64.alloc(nn * 16);
+    c.pres64.alloc(nn * 8);
+    PR_HIP(hipMemsetAsync(c.rp64.p, 0, nn * 16, c.stream));
+    PR_HIP(hipMemsetAsync(c.pres64.p, 0, nn * 8, c.stream));
+  }
+  if (c.desc.flags & PRESTIGE_FLAG_DEM_EVENTS) {
+    c.events.alloc(nn * 8);
+    PR_HIP(hipMemsetAsync(c.events.p, 0, nn * 8, c.stream));
+  }
   if (c.desc.model == PRESTIGE_MODEL_WCSPH && c.desc.precision != PRESTIGE_F64) {
     c.rp_id.alloc(nn * 2 * es);   // outputs of dry wall particles by id
     c.pres_id.alloc(nn * es);
@@ -277,6 +293,10 @@ static FieldRef find_field(Ctx& c, const std::string& name) {
   if (dem && name == "wy") return {c.omega[c.cur].p, sk, 1, false, true};
   if (dem && name == "wz") return {c.omega[c.cur].p, sk, 2, false, true};
   if (dem && name == "radius") return {c.pos[c.cur].p, sk, 3, false, true};
+  if (c.desc.flags & PRESTIGE_FLAG_DENSITY_F64) {
+    if (name == "rho") return {c.rp64.p, 0, 100, false, false};
+    if (name == "p") return {c.pres64.p, 0, -1, false, false};
+  }
   if (name == "rho") return {c.rp.p, rk, 100, false, false};
   if (name == "p") return {c.pres.p, rk, -1, false, false};
   if (name == "ax") return {c.acc.p, sk, 0, false, false};
@@ -301,6 +321,20 @@ bool generic_field_name(const Ctx& c, const std::string& name) {
   if (c.desc.model != PRESTIGE_MODEL_PAIRS)
     for (const char* m : {"rho", "p", "ax", "ay", "az"})
       if (name == m) return false;
+  return true;
+}
+
+// FP64 copy, in particle-id order, of the model field `name` (state or step
+// output) into the device array d_out; false when the context has no such field.
+bool model_field_ids(Ctx& c, const std::string& name, double* d_out) {
+  if (name == "key" || name == "force" || name == "mass") return false;
+  FieldRef f;
+  try {
+    f = find_field(c, name);
+  } catch (const Error&) {
+    return false;
+  }
+  if (c.n) scatter_to_ids(c, f.buf, f.kind, f.comp, d_out);
   return true;
 }
 
@@ -349,6 +383,10 @@ static void upload(Ctx& c, const std::string& name, const double* host, uint64_t
     c.keys_valid = false;
     c.sorted = false;
     c.inc_ready = false;
+    if (c.events.p) {
+      PR_HIP(hipMemsetAsync(c.events.p, 0, c.n * 8, c.stream));
+      c.ev_steps = 0;
+    }
     if (c.desc.model == PRESTIGE_MODEL_DEM) {
       // new positions invalidate the tangential contact history
       const size_t hr = c.desc.precision == PRESTIGE_MIXED ? 4 : 8;
@@ -364,6 +402,14 @@ static void upload(Ctx& c, const std::string& name, const double* host, uint64_t
 static void download(Ctx& c, const std::string& name, double* host, uint64_t n) {
   PR_CHECK(n == c.n, PRESTIGE_E_INVALID, "field length must equal n_particles");
   PR_CHECK(host != nullptr || n == 0, PRESTIGE_E_INVALID, "null output");
+  if (c.events.p && (name == "contact_events" || name == "slip_events")) {
+    std::vector<uint32_t> u(n ? n : 1);
+    const size_t off = name == "contact_events" ? 0 : c.n;
+    if (n) PR_HIP(hipMemcpyAsync(u.data(), c.events.as<uint32_t>() + off, n * 4, hipMemcpyDeviceToHost, c.stream));
+    PR_HIP(hipStreamSynchronize(c.stream));
+    for (uint64_t i = 0; i < n; ++i) host[i] = (double)u[i];
+    return;
+  }
   if (generic_field_name(c, name)) {
     auto it = c.gfields.find(name);
     PR_CHECK(it != c.gfields.end(), PRESTIGE_E_STATE, "field '" + name + "' was never uploaded or written");
@@ -387,22 +433,37 @@ static void download(Ctx& c, const std::string& name, double* host, uint64_t n) 
   PR_HIP(hipStreamSynchronize(c.stream));
 }
 
+// Device flags are read every kFlagEvery steps (one stream synchronisation),
+// so a step that raises one ends the call within that many steps, and the
+// error names the stretch in which it was raised.
+constexpr uint32_t kFlagEvery = 16;
+
 static void step(Ctx& c, uint32_t nsteps) {
   const int m = c.desc.model, p = c.desc.precision;
+  if (m == PRESTIGE_MODEL_WCSPH && nsteps)
+    PR_CHECK(c.sph.cfl_v2 > 0.0, PRESTIGE_E_CFL, "dt c0 > h / 2: the time step breaks the CFL bound at rest");
+  uint32_t first = 0;
   for (uint32_t s = 0; s < nsteps; ++s) {
     if (m == PRESTIGE_MODEL_WCSPH) {
       if (p == PRESTIGE_F64) sph_step_f64(c);
       else if (p == PRESTIGE_F32) sph_step_f32(c);
       else sph_step_mixed(c);
     } else if (m == PRESTIGE_MODEL_DEM) {
+      if (c.events.p) c.ev_weight = ++c.ev_steps;
       if (p == PRESTIGE_F64) dem_step_f64(c);
       else if (p == PRESTIGE_MIXED) dem_step_mixed(c);
       else throw Error(PRESTIGE_E_UNSUPPORTED, "DEM runs in F64 or MIXED precision");
+      c.ev_weight = 0;
     } else {
       throw Error(PRESTIGE_E_UNSUPPORTED, "model PAIRS has no step; use prestige_eval_equation");
     }
+    if ((s + 1) % kFlagEvery == 0 || s + 1 == nsteps) {
+      c.check_flags("raised in steps " + std::to_string(first + 1) + "-" + std::to_string(s + 1) + " of " +
+                    std::to_string(nsteps) + " of this call");
+      first = s + 1;
+    }
   }
-  c.check_flags();
+  if (!nsteps) c.check_flags();
 }
 
 // Tape entry points live in adjoint code when built; declared weak-free here.
@@ -483,6 +544,13 @@ prestige_status prestige_create(const prestige_desc* desc, prestige_ctx** out) {
     if (desc->precision < 0 || desc->precision > PRESTIGE_MIXED)
       throw Error(PRESTIGE_E_INVALID, "unknown precision");
     if (desc->n_particles >= 0xFFFFFFFFull) throw Error(PRESTIGE_E_INVALID, "n_particles must be < 2^32 - 1");
+    if (desc->flags & ~(PRESTIGE_FLAG_DENSITY_F64 | PRESTIGE_FLAG_DEM_EVENTS))
+      throw Error(PRESTIGE_E_INVALID, "unknown bits in flags");
+    if ((desc->flags & PRESTIGE_FLAG_DEM_EVENTS) && desc->model != PRESTIGE_MODEL_DEM)
+      throw Error(PRESTIGE_E_INVALID, "PRESTIGE_FLAG_DEM_EVENTS applies to DEM contexts only");
+    if ((desc->flags & PRESTIGE_FLAG_DENSITY_F64) &&
+        !(desc->model == PRESTIGE_MODEL_WCSPH && desc->precision == PRESTIGE_MIXED))
+      throw Error(PRESTIGE_E_INVALID, "PRESTIGE_FLAG_DENSITY_F64 applies to the MIXED WCSPH form only");
     if (desc->n_moving > desc->n_particles) throw Error(PRESTIGE_E_INVALID, "n_moving > n_particles");
     if (!(desc->cell_size > 0.0) || !std::isfinite(desc->cell_size))
       throw Error(PRESTIGE_E_INVALID, "cell_size must be finite and > 0");

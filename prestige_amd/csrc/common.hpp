@@ -37,7 +37,9 @@ struct Error : std::runtime_error {
   } while (0)
 
 // Device flags written by kernels on error paths only.
-enum FlagSlot : int { FLAG_DOMAIN = 0, FLAG_CAPACITY = 1, FLAG_NONFINITE = 2, FLAG_COUNT = 4 };
+// FLAG_CFL: a moving particle outran the step (WCSPH: dt (c0 + |v|) > h / 2;
+// DEM: dt |v| > r / 4).
+enum FlagSlot : int { FLAG_DOMAIN = 0, FLAG_CAPACITY = 1, FLAG_NONFINITE = 2, FLAG_CFL = 3, FLAG_COUNT = 4 };
 
 // Cell grid: key = (cz*ny + cy)*nx + cx, c = floor((x - lo) * inv_cs).
 struct Grid {
@@ -78,6 +80,7 @@ inline Grid make_grid(const double lo[3], const double hi[3], double cs) {
 struct SphConst {
   double h, inv_h, H2, eps2, inv_rho0, B, alpha_c0, msig, msh, dt;
   double g[3];
+  double cfl_v2;   // largest |v|^2 with dt (c0 + |v|) <= h / 2 (negative: dt alone breaks it)
 };
 
 struct DemConst {
@@ -276,6 +279,12 @@ struct Ctx {
   // gather outputs
   DevBuf rp;                     // (rho, p/rho^2) per slot: double2 or float2
   DevBuf pres;                   // p per slot (double or float)
+  DevBuf rp64, pres64;           // MIXED with PRESTIGE_FLAG_DENSITY_F64: the FP64 density outputs
+  // PRESTIGE_FLAG_DEM_EVENTS: per particle id, step-weighted sums of contact
+  // and slip counts (2 n uint32); steps counted since the last position
+  // upload; weight of the step being run (0: not recording)
+  DevBuf events;
+  uint32_t ev_steps = 0, ev_weight = 0;
   DevBuf acc;                    // a per slot (double4 / float4)
   // MIXED form gather copies
   DevBuf off32, vel32, om32;     // float4: offset from cell origin (cell x in .w); velocity
@@ -297,6 +306,10 @@ struct Ctx {
   // modules compiled for them, keyed by source text
   std::map<std::string, std::unique_ptr<DevBuf>> gfields;
   DevBuf posid;   // positions in id order for the all-pairs form of generated passes
+  // read-only model fields bound by generated passes (id-order FP64 copies,
+  // refreshed per call), the slot-order copies of [j] fields and the separate
+  // outputs of fields a pass reads at [j] and writes, kept between calls
+  std::map<std::string, std::unique_ptr<DevBuf>> gmodel, gslots, gsplit;
   struct JitCache {
     std::map<std::string, hipModule_t> mods;
     ~JitCache() {
@@ -304,7 +317,9 @@ struct Ctx {
     }
   } jit;
 
-  void check_flags();            // syncs; throws on a raised flag
+  // Syncs the stream; throws on a raised flag.  `where` names the stretch of
+  // the call the flags cover (e.g. "steps 17-32 of 100").
+  void check_flags(const std::string& where = std::string());
 };
 
 // Launch helpers -------------------------------------------------------------
@@ -401,9 +416,6 @@ __device__ inline int cell_shift(uint32_t wi, uint32_t wj, int axis) {
   return (int)t - 1;
 }
 
-// Error message of calls without a context (prestige_last_error(NULL)).
-void set_context_free_error(const std::string& msg);
-
 // Per-form kernel entry points (defined in the .hip files).
 void neighbors_build(Ctx& c);
 uint64_t sort_tiles(uint64_t n);   // radix-sort tiles of n keys (histogram rows per digit)
@@ -412,10 +424,12 @@ void eq1_eval(Ctx& c, int loop, double cutoff);
 void sph_step_f64(Ctx& c);
 void sph_step_f32(Ctx& c);
 void sph_step_mixed(Ctx& c);
+void sph_density_f64_for_mixed(Ctx& c);   // density of PRESTIGE_FLAG_DENSITY_F64 (FP64 translation unit)
 void dem_step_f64(Ctx& c);
 void dem_step_mixed(Ctx& c);
 void run_equations(Ctx& c, const int32_t* eq_ids, uint32_t n, int32_t loop, double cutoff);
 bool generic_field_name(const Ctx& c, const std::string& name);
+bool model_field_ids(Ctx& c, const std::string& name, double* d_out);
 void scatter_to_ids(Ctx& c, const void* src, int src_kind, int comp, double* d_out);
 void gather_from_ids(Ctx& c, const double* d_in, void* dst, int dst_kind, int comp);
 

@@ -24,9 +24,10 @@ template <int F> struct DemForm;
 template <> struct DemForm<kF64> { using real = double; using G4 = double4; };
 template <> struct DemForm<kMixed> { using real = float; using G4 = float4; };
 
-// Tangential spring update (shared by pair and wall contacts).
+// Tangential spring update (shared by pair and wall contacts); true when the
+// spring is at the Coulomb cap (sliding).
 template <class real>
-__device__ inline void dem_tangential(real fn, real nx, real ny, real nz, real vtx, real vty, real vtz,
+__device__ inline bool dem_tangential(real fn, real nx, real ny, real nz, real vtx, real vty, real vtz,
                                       real xox, real xoy, real xoz, real kt, real mu, real neg_inv_kt,
                                       real dt, real* ft, real* xn) {
   real xdn = xox * nx + xoy * ny + xoz * nz;
@@ -35,13 +36,15 @@ __device__ inline void dem_tangential(real fn, real nx, real ny, real nz, real v
   real ftx = -kt * xnx, fty = -kt * xny, ftz = -kt * xnz;
   real ftn = sqrt(ftx * ftx + fty * fty + ftz * ftz);
   real fcap = mu * (fn < real(0.0) ? -fn : fn);
-  if (ftn > fcap) {
+  const bool slip = ftn > fcap;
+  if (slip) {
     real sc = fcap / ftn;
     ftx = ftx * sc; fty = fty * sc; ftz = ftz * sc;
     xnx = ftx * neg_inv_kt; xny = fty * neg_inv_kt; xnz = ftz * neg_inv_kt;
   }
   ft[0] = ftx; ft[1] = fty; ft[2] = ftz;
   xn[0] = xnx; xn[1] = xny; xn[2] = xnz;
+  return slip;
 }
 
 struct DemHist {
@@ -57,7 +60,8 @@ __global__ __launch_bounds__(kBlock) void k_dem_force(
     const uint32_t* __restrict__ lb, Grid g, DemConst kc, const double4* __restrict__ spos,
     const double4* __restrict__ svel, const double4* __restrict__ som, double4* __restrict__ opos,
     double4* __restrict__ ovel, double4* __restrict__ oom, uint32_t* __restrict__ okeys,
-    double4* __restrict__ oforce, DemHist hist, MovedOut moved, uint32_t* flags) {
+    double4* __restrict__ oforce, DemHist hist, MovedOut moved, uint32_t* flags,
+    uint32_t* __restrict__ events, uint32_t ev_weight) {
   using real = typename DemForm<F>::real;
   using G4 = typename DemForm<F>::G4;
   const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -87,7 +91,7 @@ __global__ __launch_bounds__(kBlock) void k_dem_force(
              neg_inv_kt = (real)kc.neg_inv_kt, dt = (real)kc.dt;
   const float cs_u = (float)g.cs;
   real Fx = 0, Fy = 0, Fz = 0, Tx = 0, Ty = 0, Tz = 0;
-  uint32_t nc = 0;
+  uint32_t nc = 0, nwall = 0, nslip = 0;
   const uint32_t ocnt = hist.cnt_in[me];
   for_each_run(cr, g, lb, [&](uint32_t j0, uint32_t j1, int ddy, int ddz) {
     for (uint32_t j = j0; j < j1; ++j) {
@@ -133,7 +137,7 @@ __global__ __launch_bounds__(kBlock) void k_dem_force(
         }
       }
       real ft[3], xn[3];
-      dem_tangential<real>(fn, nx, ny, nz, vtx, vty, vtz, xox, xoy, xoz, kt, mu, neg_inv_kt, dt, ft, xn);
+      nslip += dem_tangential<real>(fn, nx, ny, nz, vtx, vty, vtz, xox, xoy, xoz, kt, mu, neg_inv_kt, dt, ft, xn);
       Fx = Fx + (fn * nx + ft[0]);
       Fy = Fy + (fn * ny + ft[1]);
       Fz = Fz + (fn * nz + ft[2]);
@@ -184,7 +188,8 @@ __global__ __launch_bounds__(kBlock) void k_dem_force(
     real xoy = wxi_in[(size_t)(w * 3 + 1) * n + me];
     real xoz = wxi_in[(size_t)(w * 3 + 2) * n + me];
     real ft[3], xn[3];
-    dem_tangential<real>(fn, nx, ny, nz, vtx, vty, vtz, xox, xoy, xoz, kt, mu, neg_inv_kt, dt, ft, xn);
+    nslip += dem_tangential<real>(fn, nx, ny, nz, vtx, vty, vtz, xox, xoy, xoz, kt, mu, neg_inv_kt, dt, ft, xn);
+    ++nwall;
     Fx = Fx + (fn * nx + ft[0]);
     Fy = Fy + (fn * ny + ft[1]);
     Fz = Fz + (fn * nz + ft[2]);
@@ -196,6 +201,10 @@ __global__ __launch_bounds__(kBlock) void k_dem_force(
   }
   if (nc > mc) atomicOr(&flags[FLAG_CAPACITY], 1u);
   hist.cnt_out[me] = nc < mc ? nc : mc;
+  if (events) {   // step-weighted fingerprints of this particle's contact and slip events
+    events[me] += ev_weight * (nc + nwall);
+    events[(size_t)n + me] += ev_weight * nslip;
+  }
   // symplectic Euler in FP64
   const double4 vs = svel[i], ws = som[i];
   const double m64 = vs.w, r64 = xs.w;
@@ -218,6 +227,8 @@ __global__ __launch_bounds__(kBlock) void k_dem_force(
   oom[i] = w2;
   if (oforce) oforce[i] = make_double4(axx, ayy, azz, 0.0);
   if (!isfinite(axx) || !isfinite(ayy) || !isfinite(azz)) atomicOr(&flags[FLAG_NONFINITE], 1u);
+  if ((v2.x * v2.x + v2.y * v2.y + v2.z * v2.z) * (kc.dt * kc.dt) > 0.0625 * (r64 * r64))
+    atomicOr(&flags[FLAG_CFL], 1u);
   double fx = floor((x2.x - g.lo[0]) * g.inv_cs);
   double fy = floor((x2.y - g.lo[1]) * g.inv_cs);
   double fz = floor((x2.z - g.lo[2]) * g.inv_cs);
@@ -259,7 +270,8 @@ void dem_step_impl(Ctx& c) {
       c.lb.as<uint32_t>(), c.grid, c.dem, c.pos[c.cur].as<double4>(), c.vel[c.cur].as<double4>(),
       c.omega[c.cur].as<double4>(), c.pos[dst].as<double4>(), c.vel[dst].as<double4>(),
       c.omega[dst].as<double4>(), c.keys[kdst].as<uint32_t>(), c.keep_acc ? c.acc.as<double4>() : nullptr,
-      hist, MovedOut{c.inc_mask.as<uint64_t>(), c.inc_tiles.as<uint32_t>()}, c.flags.as<uint32_t>());
+      hist, MovedOut{c.inc_mask.as<uint64_t>(), c.inc_tiles.as<uint32_t>()}, c.flags.as<uint32_t>(),
+      c.ev_weight ? c.events.as<uint32_t>() : nullptr, c.ev_weight);
   PR_HIP(hipGetLastError());
   c.cur = dst;
   c.cur_key = kdst;

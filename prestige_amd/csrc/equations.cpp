@@ -88,7 +88,7 @@ std::vector<Eq> builtins() {
         {"z", PRESTIGE_ROLE_I}},
        {"vx", "vy", "vz", "x", "y", "z"}},
       {"dem_contact",
-       "{ let c = contact (x [i] - x [j] , y [i] - y [j] , z [i] - z [j] , radius [i] + radius [j] , "
+       "{ let c = contact (x [i] - x [j] , y [i] - y [j] , z [i] - z [j] , radius [i] , radius [j] , "
        "vx [i] - vx [j] , vy [i] - vy [j] , vz [i] - vz [j] , wx [i] , wy [i] , wz [i] , wx [j] , wy [j] , "
        "wz [j] , mass [i] , mass [j]) ; fx [i] += c . 0 ; fy [i] += c . 1 ; fz [i] += c . 2 ; "
        "tx [i] += c . 3 ; ty [i] += c . 4 ; tz [i] += c . 5 ; }",
@@ -246,6 +246,18 @@ size_t index_close(const std::vector<Tok>& t, size_t k) {
   throw GenError{PRESTIGE_E_INVALID, "unbalanced '[' in equation body"};
 }
 
+// Name the macro records for the indexed expression ending at identifier k
+// (prestige_macros/src/lib.rs:75-77, 105-107, 125-127): only a path counts,
+// and of a path only its first segment, so `x[i]` records x,
+// `crate::m::x[i]` records crate and a field `self.x[i]` (not a path)
+// records nothing (empty result).
+std::string indexed_path_root(const std::vector<Tok>& t, size_t k) {
+  if (k > 0 && t[k - 1].s == ".") return std::string();
+  size_t s = k;
+  while (s >= 2 && t[s - 1].s == "::" && t[s - 2].kind == kIdent) s -= 2;
+  return t[s].s;
+}
+
 // The macro's rule (prestige_macros/src/lib.rs:71-133): every indexed path is
 // a read, an indexed path on the left of `=` or `op=` is a write; plus the
 // role of each access: the index inside the brackets (i, j, or other).
@@ -255,11 +267,13 @@ void infer(const std::vector<Tok>& t, Eq& e) {
   for (size_t k = 0; k < t.size(); ++k) {
     const size_t c = index_close(t, k);
     if (!c) continue;
+    const std::string name = indexed_path_root(t, k);
+    if (name.empty()) continue;
     unsigned role = 0;
     if (c == k + 3 && t[k + 2].s == "i") role = PRESTIGE_ROLE_I;
     if (c == k + 3 && t[k + 2].s == "j") role = PRESTIGE_ROLE_J;
-    roles[t[k].s] |= role;
-    if (c + 1 < t.size() && is_assign_op(t[c + 1].s)) writes.insert(t[k].s);
+    roles[name] |= role;
+    if (c + 1 < t.size() && is_assign_op(t[c + 1].s)) writes.insert(name);
   }
   e.reads.assign(roles.begin(), roles.end());
   e.writes.assign(writes.begin(), writes.end());
@@ -288,7 +302,7 @@ const std::map<std::string, std::string>& fn_map() {
 
 // Device helpers provided to generated code.
 const std::set<std::string>& helper_names() {
-  static const std::set<std::string> s = {"kernel_w", "tait"};
+  static const std::set<std::string> s = {"kernel_w", "tait", "momentum", "contact", "pt_make", "pt_get"};
   return s;
 }
 
@@ -386,26 +400,173 @@ std::vector<Tok> rewrite(std::vector<Tok> t, const std::string& eq) {
     t.erase(t.begin() + (long)start, t.begin() + (long)close + 1);
     t.insert(t.begin() + (long)start, repl.begin(), repl.end());
   }
-  // tuple fields have no translation
-  for (size_t k = 0; k + 1 < t.size(); ++k)
-    if (t[k].s == "." && t[k + 1].kind == kNumber) throw bad("tuple field access");
-  // let
+  // if expressions (an `if` where a value is expected) become immediately
+  // invoked lambdas whose branches return their blocks' last expressions
+  for (;;) {
+    size_t k = 0;
+    bool found = false;
+    for (; k < t.size(); ++k) {
+      if (t[k].s != "if") continue;
+      const std::string prev = k ? t[k - 1].s : "{";
+      if (prev != "{" && prev != ";" && prev != "}" && prev != "else") {
+        found = true;
+        break;
+      }
+    }
+    if (!found) break;
+    std::vector<Tok> lam = {{kPunct, "("}, {kPunct, "["}, {kPunct, "&"}, {kPunct, "]"}, {kPunct, "("},
+                            {kPunct, ")"}, {kPunct, "{"}};
+    size_t at = k;
+    for (;;) {   // one `if cond { block }` of the chain per turn
+      size_t b = at + 1;
+      int depth = 0;
+      for (; b < t.size(); ++b) {
+        if (t[b].s == "(" || t[b].s == "[") ++depth;
+        if (t[b].s == ")" || t[b].s == "]") --depth;
+        if (t[b].s == "{" && depth == 0) break;
+      }
+      if (b >= t.size()) throw bad("if without block");
+      const size_t e = match_close(t, b);
+      lam.push_back({kIdent, "if"});
+      lam.push_back({kPunct, "("});
+      lam.insert(lam.end(), t.begin() + (long)at + 1, t.begin() + (long)b);
+      lam.push_back({kPunct, ")"});
+      // the block with its last expression returned
+      size_t last = b + 1;
+      int d2 = 0;
+      for (size_t m = b + 1; m < e; ++m) {
+        if (t[m].s == "(" || t[m].s == "[" || t[m].s == "{") ++d2;
+        if (t[m].s == ")" || t[m].s == "]" || t[m].s == "}") --d2;
+        if (t[m].s == ";" && d2 == 0) last = m + 1;
+      }
+      if (last >= e) throw bad("if expression branch without a value");
+      lam.push_back({kPunct, "{"});
+      lam.insert(lam.end(), t.begin() + (long)b + 1, t.begin() + (long)last);
+      lam.push_back({kIdent, "return"});
+      lam.insert(lam.end(), t.begin() + (long)last, t.begin() + (long)e);
+      lam.push_back({kPunct, ";"});
+      lam.push_back({kPunct, "}"});
+      if (e + 1 >= t.size() || t[e + 1].s != "else") throw bad("if expression without else");
+      lam.push_back({kIdent, "else"});
+      if (e + 2 < t.size() && t[e + 2].s == "if") {
+        at = e + 2;
+        continue;
+      }
+      if (e + 2 >= t.size() || t[e + 2].s != "{") throw bad("else without block");
+      const size_t e2 = match_close(t, e + 2);
+      size_t last2 = e + 3;
+      int d3 = 0;
+      for (size_t m = e + 3; m < e2; ++m) {
+        if (t[m].s == "(" || t[m].s == "[" || t[m].s == "{") ++d3;
+        if (t[m].s == ")" || t[m].s == "]" || t[m].s == "}") --d3;
+        if (t[m].s == ";" && d3 == 0) last2 = m + 1;
+      }
+      if (last2 >= e2) throw bad("if expression branch without a value");
+      lam.push_back({kPunct, "{"});
+      lam.insert(lam.end(), t.begin() + (long)e + 3, t.begin() + (long)last2);
+      lam.push_back({kIdent, "return"});
+      lam.insert(lam.end(), t.begin() + (long)last2, t.begin() + (long)e2);
+      lam.push_back({kPunct, ";"});
+      lam.push_back({kPunct, "}"});
+      lam.push_back({kPunct, "}"});
+      lam.push_back({kPunct, ")"});
+      lam.push_back({kPunct, "("});
+      lam.push_back({kPunct, ")"});
+      t.erase(t.begin() + (long)k, t.begin() + (long)e2 + 1);
+      t.insert(t.begin() + (long)k, lam.begin(), lam.end());
+      break;
+    }
+  }
+  // tuple fields: t.0 -> pt_get<0>(t) (receiver: a name or a parenthesised /
+  // called expression)
+  for (size_t k = 1; k + 1 < t.size(); ++k) {
+    if (!(t[k].s == "." && t[k + 1].kind == kNumber)) continue;
+    if (t[k + 1].s.find_first_not_of("0123456789") != std::string::npos) throw bad("tuple field '" + t[k + 1].s + "'");
+    size_t start = k - 1;
+    if (t[start].s == ")") {
+      start = match_open(t, start);
+      if (start > 0 && t[start - 1].kind == kIdent) --start;
+    } else if (t[start].kind != kIdent) {
+      throw bad("tuple field of '" + t[start].s + "'");
+    }
+    std::vector<Tok> repl = {{kIdent, "pt_get"}, {kPunct, "<"}, t[k + 1], {kPunct, ">"}, {kPunct, "("}};
+    repl.insert(repl.end(), t.begin() + (long)start, t.begin() + (long)k);
+    repl.push_back({kPunct, ")"});
+    t.erase(t.begin() + (long)start, t.begin() + (long)k + 2);
+    t.insert(t.begin() + (long)start, repl.begin(), repl.end());
+    k = start + repl.size() - 1;
+  }
+  // tuple expressions (a, b, ..) -> pt_make(a, b, ..): a parenthesis that is
+  // no call and holds a comma at its own level
+  for (size_t k = 0; k < t.size(); ++k) {
+    if (t[k].s != "(") continue;
+    if (k > 0 && (t[k - 1].kind == kIdent || t[k - 1].s == ">" || t[k - 1].s == ")" || t[k - 1].s == "]")) continue;
+    if (k > 0 && t[k - 1].s == "let") continue;   // destructuring pattern, below
+    const size_t e = match_close(t, k);
+    int depth = 0;
+    bool comma = false;
+    for (size_t m = k + 1; m < e; ++m) {
+      if (t[m].s == "(" || t[m].s == "[" || t[m].s == "{") ++depth;
+      if (t[m].s == ")" || t[m].s == "]" || t[m].s == "}") --depth;
+      if (t[m].s == "," && depth == 0) comma = true;
+    }
+    if (comma) t.insert(t.begin() + (long)k, {kIdent, "pt_make"});
+  }
+  // let: `let (a, b) = e;` -> auto __tN = e; double a = pt_get<0>(__tN); ...
+  // and `let v = e;` -> double v = e (auto when e is a tuple)
+  int tmp_id = 0;
   for (size_t k = 0; k < t.size(); ++k) {
     if (t[k].s != "let") continue;
     size_t v = k + 1;
     if (v < t.size() && t[v].s == "mut") t.erase(t.begin() + (long)v);
+    if (v < t.size() && t[v].s == "(") {
+      const size_t e = match_close(t, v);
+      std::vector<std::string> names;
+      for (size_t m = v + 1; m < e; ++m) {
+        if (t[m].s == "mut") continue;
+        if (t[m].kind == kIdent) names.push_back(t[m].s);
+        else if (t[m].s != ",") throw bad("pattern in let");
+      }
+      if (e + 1 >= t.size() || t[e + 1].s != "=") throw bad("let pattern without value");
+      size_t semi = e + 2;
+      int depth = 0;
+      for (; semi < t.size(); ++semi) {
+        if (t[semi].s == "(" || t[semi].s == "[" || t[semi].s == "{") ++depth;
+        if (t[semi].s == ")" || t[semi].s == "]" || t[semi].s == "}") --depth;
+        if (t[semi].s == ";" && depth == 0) break;
+      }
+      if (semi >= t.size()) throw bad("let without ';'");
+      const std::string tn = "__t" + std::to_string(tmp_id++);
+      std::vector<Tok> repl = {{kIdent, "auto"}, {kIdent, tn}, {kPunct, "="}};
+      repl.insert(repl.end(), t.begin() + (long)e + 2, t.begin() + (long)semi);
+      repl.push_back({kPunct, ";"});
+      for (size_t q = 0; q < names.size(); ++q) {
+        if (names[q] == "_") continue;
+        const std::vector<Tok> d = {{kIdent, "double"}, {kIdent, names[q]}, {kPunct, "="}, {kIdent, "pt_get"},
+                                    {kPunct, "<"}, {kNumber, std::to_string(q)}, {kPunct, ">"}, {kPunct, "("},
+                                    {kIdent, tn}, {kPunct, ")"}, {kPunct, ";"}};
+        repl.insert(repl.end(), d.begin(), d.end());
+      }
+      t.erase(t.begin() + (long)k, t.begin() + (long)semi + 1);
+      t.insert(t.begin() + (long)k, repl.begin(), repl.end());
+      k += repl.size() - 1;
+      continue;
+    }
     if (v >= t.size() || t[v].kind != kIdent) throw bad("pattern in let");
     if (v + 1 < t.size() && t[v + 1].s == ":") {
       if (v + 2 >= t.size() || (t[v + 2].s != "f64" && t[v + 2].s != "f32")) throw bad("non-float let binding");
       t.erase(t.begin() + (long)v + 1, t.begin() + (long)v + 3);
     }
-    t[k].s = "double";
+    // a value that is a tuple keeps its type: a tuple expression, a helper
+    // returning one, or an if expression (whose branches decide)
+    const bool tuple = v + 2 < t.size() && t[v + 1].s == "=" &&
+                       (t[v + 2].s == "pt_make" || t[v + 2].s == "momentum" || t[v + 2].s == "contact" ||
+                        (t[v + 2].s == "(" && v + 3 < t.size() && t[v + 3].s == "["));
+    t[k].s = tuple ? "auto" : "double";
   }
   // if statements
   for (size_t k = 0; k < t.size(); ++k) {
     if (t[k].s != "if") continue;
-    const std::string prev = k ? t[k - 1].s : "{";
-    if (prev != "{" && prev != ";" && prev != "}" && prev != "else") throw bad("if expression");
     size_t b = k + 1;
     int depth = 0;
     for (; b < t.size(); ++b) {
@@ -416,6 +577,15 @@ std::vector<Tok> rewrite(std::vector<Tok> t, const std::string& eq) {
     if (b >= t.size()) throw bad("if without block");
     t.insert(t.begin() + (long)b, {kPunct, ")"});
     t.insert(t.begin() + (long)k + 1, {kPunct, "("});
+    ++k;
+  }
+  // helpers that read the descriptor constants get them as a last argument
+  for (size_t k = 0; k + 1 < t.size(); ++k) {
+    if ((t[k].s == "momentum" || t[k].s == "contact") && t[k + 1].s == "(") {
+      const size_t e = match_close(t, k + 1);
+      t.insert(t.begin() + (long)e, {kIdent, "prestige_k"});
+      t.insert(t.begin() + (long)e, {kPunct, ","});
+    }
   }
   // a block's last expression may omit its ';'
   for (size_t k = 1; k < t.size(); ++k) {
@@ -442,7 +612,7 @@ std::string emit_body(const Eq& e, const PassNames& pn, std::set<std::string>* c
   std::vector<Tok> t = rewrite(tokenize(e.body), e.name);
   std::set<std::string> locals;
   for (size_t k = 0; k + 1 < t.size(); ++k)
-    if (t[k].s == "double" && t[k + 1].kind == kIdent) locals.insert(t[k + 1].s);
+    if ((t[k].s == "double" || t[k].s == "auto") && t[k + 1].kind == kIdent) locals.insert(t[k + 1].s);
   std::string out;
   for (size_t k = 0; k < t.size(); ++k) {
     const Tok& tk = t[k];
@@ -471,8 +641,9 @@ std::string emit_body(const Eq& e, const PassNames& pn, std::set<std::string>* c
       else if (tk.s == "j") piece = "jd";
       else if (const_names().count(tk.s)) {
         if (consts_used) consts_used->insert(tk.s);
-      } else if (tk.s == "double" || tk.s == "if" || tk.s == "else" || tk.s == "true" || tk.s == "false" ||
-                 locals.count(tk.s) || helper_names().count(tk.s)) {
+      } else if (tk.s == "double" || tk.s == "auto" || tk.s == "return" || tk.s == "prestige_k" ||
+                 tk.s == "if" || tk.s == "else" || tk.s == "true" || tk.s == "false" || locals.count(tk.s) ||
+                 helper_names().count(tk.s)) {
       } else {
         bool fn = false;
         for (const auto& kv : fn_map())
@@ -541,7 +712,110 @@ __device__ inline double tait(double rho, double rho0, double c0) {
 }
 struct prestige_consts {
   double h, dt, rho0, c0, alpha, kn, kt, mu, gx, gy, gz;
+  double two_zeta, neg_inv_kt;   // DEM: 2 zeta from the restitution, -1 / kt
 };
+// Cell-loop passes run one wavefront per block over 64 consecutive slots.  Per row
+// run the wavefront stages the union of its lanes' candidate ranges in LDS as FP32
+// offsets from its first slot in units of the cutoff (|q|^2 in .w); each lane scans
+// its own range 32 candidates at a time with the dot form q.w - 2 s.q < t and walks
+// the set bits in ascending slot order, applying the exact FP64 test before the
+// bodies.  The threshold t carries the rounding margin of the FP32 form, so the
+// scan keeps a superset of the exact neighbours and the bodies see exactly the
+// pairs and the order of the plain loop.
+__device__ inline unsigned pg_wave_min(unsigned v) {
+  for (int d = 32; d > 0; d >>= 1) {
+    const unsigned t = __shfl_xor(v, d, 64);
+    v = t < v ? t : v;
+  }
+  return (unsigned)__builtin_amdgcn_readfirstlane((int)v);
+}
+__device__ inline unsigned pg_wave_max(unsigned v) {
+  for (int d = 32; d > 0; d >>= 1) {
+    const unsigned t = __shfl_xor(v, d, 64);
+    v = t > v ? t : v;
+  }
+  return (unsigned)__builtin_amdgcn_readfirstlane((int)v);
+}
+// Tuples of doubles: (a, b, ..) in a body is pt_make(a, b, ..), t.k is pt_get<k>(t).
+template <int N> struct pt_tuple { double v[N]; };
+template <class... A> __device__ inline pt_tuple<(int)sizeof...(A)> pt_make(A... a) {
+  return pt_tuple<(int)sizeof...(A)>{{(double)a...}};
+}
+template <int K, int N> __device__ inline double pt_get(const pt_tuple<N>& t) { return t.v[K]; }
+// momentum: acceleration of i from j, -m_j sigma / h (P_i + P_j + Pi_ij) f'(q) / r x_ij,
+// for 0 < r < 2h (else 0), with P = p / rho^2 summed by the caller (psum), rho_i + rho_j
+// (rsum) and Monaghan's Pi_ij = -alpha c0 h (v.r) / ((r^2 + 0.01 h^2) rhobar) for
+// approaching pairs, over one common denominator; the WCSPH force pass's operation order.
+__device__ inline pt_tuple<3> momentum(double dx, double dy, double dz, double dvx, double dvy, double dvz,
+                                       double psum, double rsum, double mj, const prestige_consts& k) {
+  const double inv_h = 1.0 / k.h;
+  const double H2 = (2.0 * k.h) * (2.0 * k.h);
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  if (!(r2 < H2 && r2 > 0.0)) return pt_tuple<3>{{0.0, 0.0, 0.0}};
+  const double eps2 = (0.01 * k.h) * k.h;
+  const double rr = sqrt(r2);
+  const double q = rr * inv_h;
+  double dw;
+  if (q < 1.0) {
+    dw = -3.0 * q + 2.25 * q * q;
+  } else {
+    const double t = 2.0 - q;
+    dw = -0.75 * t * t;
+  }
+  const double vdotr = dvx * dx + dvy * dy + dvz * dz;
+  const double ach = (k.alpha * k.c0) * k.h;
+  double num, den;
+  if (vdotr < 0.0) {
+    const double drb = (r2 + eps2) * (0.5 * rsum);
+    num = (psum * drb - ach * vdotr) * dw;
+    den = drb * rr;
+  } else {
+    num = psum * dw;
+    den = rr;
+  }
+  const double sc = num / den;
+  const double sigma = 1.0 / (((3.141592653589793 * k.h) * k.h) * k.h);
+  const double msh = (mj * sigma) * inv_h;
+  return pt_tuple<3>{{-(msh * (sc * dx)), -(msh * (sc * dy)), -(msh * (sc * dz))}};
+}
+// contact: force and torque on sphere i from sphere j (linear spring-dashpot normal
+// force, tangential spring from a fresh history advanced by v_t dt and Coulomb-capped,
+// torque (-r_i n) x F_t), zero when the spheres do not overlap; the DEM contact pass's
+// operation order.  Returns (Fx, Fy, Fz, Tx, Ty, Tz).
+__device__ inline pt_tuple<6> contact(double dx, double dy, double dz, double ri, double rj, double dvx,
+                                      double dvy, double dvz, double wxi, double wyi, double wzi, double wxj,
+                                      double wyj, double wzj, double mi, double mj, const prestige_consts& k) {
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  const double R = ri + rj;
+  if (!(r2 < R * R && r2 > 0.0)) return pt_tuple<6>{{0.0, 0.0, 0.0, 0.0, 0.0, 0.0}};
+  const double d = sqrt(r2);
+  const double invd = 1.0 / d;
+  const double nx = dx * invd, ny = dy * invd, nz = dz * invd;
+  const double delta = R - d;
+  const double wsx = ri * wxi + rj * wxj, wsy = ri * wyi + rj * wyj, wsz = ri * wzi + rj * wzj;
+  const double cx = wsy * nz - wsz * ny, cy = wsz * nx - wsx * nz, cz = wsx * ny - wsy * nx;
+  const double vrx = dvx - cx, vry = dvy - cy, vrz = dvz - cz;
+  const double vn = vrx * nx + vry * ny + vrz * nz;
+  const double vtx = vrx - vn * nx, vty = vry - vn * ny, vtz = vrz - vn * nz;
+  const double meff = mi * mj / (mi + mj);
+  const double cn = k.two_zeta * sqrt(meff * k.kn);
+  const double fn = k.kn * delta - cn * vn;
+  const double xo = 0.0;
+  const double xdn = xo * nx + xo * ny + xo * nz;
+  const double xpx = xo - xdn * nx, xpy = xo - xdn * ny, xpz = xo - xdn * nz;
+  double xnx = xpx + vtx * k.dt, xny = xpy + vty * k.dt, xnz = xpz + vtz * k.dt;
+  double ftx = -k.kt * xnx, fty = -k.kt * xny, ftz = -k.kt * xnz;
+  const double ftn = sqrt(ftx * ftx + fty * fty + ftz * ftz);
+  const double fcap = k.mu * (fn < 0.0 ? -fn : fn);
+  if (ftn > fcap) {
+    const double sc = fcap / ftn;
+    ftx = ftx * sc;
+    fty = fty * sc;
+    ftz = ftz * sc;
+  }
+  const double tx = ny * ftz - nz * fty, ty = nz * ftx - nx * ftz, tz = nx * fty - ny * ftx;
+  return pt_tuple<6>{{fn * nx + ftx, fn * ny + fty, fn * nz + ftz, -(ri * tx), -(ri * ty), -(ri * tz)}};
+}
 )";
 
 GenProgram build_program(const int32_t* ids, uint32_t n, int32_t loop, bool f32_pos) {
@@ -617,46 +891,98 @@ GenProgram build_program(const int32_t* ids, uint32_t n, int32_t loop, bool f32_
     for (auto& kv : fields) s += " " + kv.first + (kv.second.split ? "(in,out)" : "");
     s += "\n";
     if (loop == PRESTIGE_LOOP_CELLS)
-      s += "extern \"C\" __global__ void __launch_bounds__(256) " + gp.kernel +
+      s += "extern \"C\" __global__ void __launch_bounds__(64) " + gp.kernel +
            "(\n    unsigned n, const prestige_pos* __restrict__ pos, const unsigned* __restrict__ ids,\n"
            "    const unsigned* __restrict__ skeys, const unsigned* __restrict__ lb, int nx, int ny, int nz,\n"
-           "    double c2, prestige_consts k" + params + ") {\n";
+           "    double c2, double inv_cut, float thr, prestige_consts prestige_k" + params + ") {\n";
     else
       s += "extern \"C\" __global__ void __launch_bounds__(256) " + gp.kernel +
-           "(\n    unsigned n, const prestige_pos* __restrict__ pos, prestige_consts k" + params + ") {\n";
+           "(\n    unsigned n, const prestige_pos* __restrict__ pos, prestige_consts prestige_k" + params + ") {\n";
     s += "#pragma clang fp contract(off)\n";
     if (loop == PRESTIGE_LOOP_CELLS) {
-      s += "  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;\n  if (i >= n) return;\n";
-      s += "  const unsigned me = ids[i];\n  const prestige_pos pi = pos[i];\n";
+      s += "  __shared__ float4 s_q[256];\n"
+           "  const unsigned lane = threadIdx.x, wbase = blockIdx.x * 64u, i = wbase + lane;\n"
+           "  const bool live = i < n;\n"
+           "  const unsigned me = live ? ids[i] : 0u;\n"
+           "  const prestige_pos pi = pos[live ? i : wbase];\n"
+           "  const prestige_pos p0 = pos[wbase];\n"
+           "  const double xr = (double)p0.x, yr = (double)p0.y, zr = (double)p0.z;\n"
+           "  auto offset = [&](const prestige_pos& p) {\n"
+           "    return make_float4((float)(((double)p.x - xr) * inv_cut), (float)(((double)p.y - yr) * inv_cut),\n"
+           "                       (float)(((double)p.z - zr) * inv_cut), 0.0f);\n"
+           "  };\n";
     } else {
       s += "  const unsigned me = blockIdx.x * blockDim.x + threadIdx.x;\n  if (me >= n) return;\n";
       s += "  const prestige_pos pi = pos[me];\n";
     }
-    for (const auto& c : consts) s += "  const double " + c + " = k." + c + ";\n";
+    for (const auto& c : consts) s += "  const double " + c + " = prestige_k." + c + ";\n";
     for (const auto& f : at_i) {
       const bool sp = pn.split.count(f) > 0;
       s += "  double " + f + "_i = f_" + f + (sp ? "_in" : "") + "[me];\n";
     }
+    if (loop == PRESTIGE_LOOP_CELLS && jb.empty()) s += "  if (!live) return;\n";
     if (!jb.empty()) {
       if (loop == PRESTIGE_LOOP_CELLS) {
-        s += "  const unsigned key = skeys[i];\n"
+        s += "  const unsigned key = live ? skeys[i] : 0u;\n"
              "  const int cx = (int)(key % (unsigned)nx), cy = (int)((key / (unsigned)nx) % (unsigned)ny),\n"
              "            cz = (int)(key / ((unsigned)nx * (unsigned)ny));\n"
              "  const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < nx - 1 ? cx + 1 : nx - 1;\n"
+             "  const float4 so = offset(pi);\n"
+             "  const float ax_ = -2.0f * so.x, ay_ = -2.0f * so.y, az_ = -2.0f * so.z;\n"
+             "  const float ss = __builtin_fmaf(so.x, so.x, __builtin_fmaf(so.y, so.y, so.z * so.z));\n"
+             "  const unsigned ss_max = pg_wave_max(live ? __float_as_uint(ss) : 0u);\n"
              "  for (int dz = -1; dz <= 1; ++dz) {\n"
-             "    const int zz = cz + dz;\n    if (zz < 0 || zz >= nz) continue;\n"
              "    for (int dy = -1; dy <= 1; ++dy) {\n"
-             "      const int yy = cy + dy;\n      if (yy < 0 || yy >= ny) continue;\n"
-             "      const unsigned row = ((unsigned)zz * (unsigned)ny + (unsigned)yy) * (unsigned)nx;\n"
-             "      const unsigned j1 = lb[row + (unsigned)x1 + 1];\n"
-             "      for (unsigned js = lb[row + (unsigned)x0]; js < j1; ++js) {\n"
-             "      const prestige_pos pj = pos[js];\n"
-             "      const double ex = (double)pi.x - (double)pj.x, ey = (double)pi.y - (double)pj.y,\n"
-             "                   ez = (double)pi.z - (double)pj.z;\n"
-             "      if (!(ex * ex + ey * ey + ez * ez < c2)) continue;\n"
-             "      const unsigned jd = ids[js];\n";
+             "      const int zz = cz + dz, yy = cy + dy;\n"
+             "      unsigned j0 = 0u, len = 0u;\n"
+             "      if (live && zz >= 0 && zz < nz && yy >= 0 && yy < ny) {\n"
+             "        const unsigned row = ((unsigned)zz * (unsigned)ny + (unsigned)yy) * (unsigned)nx;\n"
+             "        j0 = lb[row + (unsigned)x0];\n"
+             "        len = lb[row + (unsigned)x1 + 1] - j0;\n"
+             "      }\n"
+             "      const unsigned base = pg_wave_min(len ? j0 : 0xFFFFFFFFu);\n"
+             "      const unsigned top = pg_wave_max(len ? j0 + len : 0u);\n"
+             "      if (top <= base) continue;\n"
+             "      const bool staged = top - base <= 256u;\n"
+             "      float t = 0.0f;\n"
+             "      if (staged) {\n"
+             "        float mq = 0.0f;\n"
+             "        for (unsigned kk = lane; kk < top - base; kk += 64u) {\n"
+             "          float4 q = offset(pos[base + kk]);\n"
+             "          q.w = __builtin_fmaf(q.x, q.x, __builtin_fmaf(q.y, q.y, q.z * q.z));\n"
+             "          s_q[kk] = q;\n"
+             "          mq = q.w > mq ? q.w : mq;\n"
+             "        }\n"
+             "        const float m2 = fmaxf(__uint_as_float(ss_max), __uint_as_float(pg_wave_max(__float_as_uint(mq))));\n"
+             "        t = thr + 0x1.0p-19f * (m2 + __builtin_sqrtf(m2) + 1.0f) - ss;\n"
+             "        __syncthreads();\n"
+             "      }\n"
+             "      for (unsigned w0 = 0u; w0 < len; w0 += 32u) {\n"
+             "        const unsigned cnt = len - w0 < 32u ? len - w0 : 32u;\n"
+             "        unsigned word = 0u;\n"
+             "        for (unsigned b = 0u; b < cnt; ++b) {\n"
+             "          const unsigned jc = j0 + w0 + b;\n"
+             "          bool hit;\n"
+             "          if (staged) {\n"
+             "            const float4 q = s_q[jc - base];\n"
+             "            hit = __builtin_fmaf(ax_, q.x, __builtin_fmaf(ay_, q.y, __builtin_fmaf(az_, q.z, q.w))) < t;\n"
+             "          } else {\n"
+             "            const float4 q = offset(pos[jc]);\n"
+             "            const float ddx = so.x - q.x, ddy = so.y - q.y, ddz = so.z - q.z;\n"
+             "            hit = __builtin_fmaf(ddx, ddx, __builtin_fmaf(ddy, ddy, ddz * ddz)) < thr;\n"
+             "          }\n"
+             "          word |= (hit ? 1u : 0u) << b;\n"
+             "        }\n"
+             "        while (word) {\n"
+             "          const unsigned js = j0 + w0 + (unsigned)__builtin_ctz(word);\n"
+             "          word &= word - 1u;\n"
+             "          const prestige_pos pj = pos[js];\n"
+             "          const double ex = (double)pi.x - (double)pj.x, ey = (double)pi.y - (double)pj.y,\n"
+             "                       ez = (double)pi.z - (double)pj.z;\n"
+             "          if (!(ex * ex + ey * ey + ez * ez < c2)) continue;\n"
+             "          const unsigned jd = ids[js];\n";
         s += jcode;
-        s += "      }\n    }\n  }\n";
+        s += "        }\n      }\n      if (staged) __syncthreads();\n    }\n  }\n  if (!live) return;\n";
       } else {
         s += "  for (unsigned jd = 0; jd < n; ++jd) {\n      const prestige_pos pj = pos[jd];\n";
         s += jcode;

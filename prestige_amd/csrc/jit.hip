@@ -5,6 +5,8 @@
 // prestige/src/codegen/simple_cpu.rs:3-22 prints.
 #include <hip/hiprtc.h>
 
+#include <cmath>
+
 #include <string>
 #include <vector>
 
@@ -23,7 +25,6 @@ __global__ __launch_bounds__(kBlock) void k_pos_to_ids(uint32_t n, const T4* __r
   if (k < n) out[ids[k]] = pos[k];
 }
 
-// Layout of prestige_consts in the generated source.
 // a field in slot order for the cell loop's [j] reads: out[k] = f[ids[k]]
 __global__ __launch_bounds__(kBlock) void k_field_to_slots(uint32_t n, const uint32_t* __restrict__ ids,
                                                            const double* __restrict__ f,
@@ -32,9 +33,18 @@ __global__ __launch_bounds__(kBlock) void k_field_to_slots(uint32_t n, const uin
   if (k < n) out[k] = f[ids[k]];
 }
 
+// Layout of prestige_consts in the generated source.
 struct GenConsts {
   double h, dt, rho0, c0, alpha, kn, kt, mu, gx, gy, gz;
+  double two_zeta, neg_inv_kt;
 };
+
+DevBuf& kept_buf(std::map<std::string, std::unique_ptr<DevBuf>>& m, const std::string& name, size_t bytes) {
+  std::unique_ptr<DevBuf>& b = m[name];
+  if (!b) b.reset(new DevBuf());
+  b->alloc(bytes);
+  return *b;
+}
 
 hipModule_t module_for(Ctx& c, const std::string& src) {
   auto it = c.jit.mods.find(src);
@@ -85,6 +95,11 @@ double* field_ptr(Ctx& c, const GenField& f) {
     return b.as<double>();
   }
   auto it = c.gfields.find(f.name);
+  if (it == c.gfields.end() && !f.written) {
+    // a field of the context's model (velocities, radii, WCSPH outputs, ...), read only
+    DevBuf& b = kept_buf(c.gmodel, f.name, bytes);
+    if (model_field_ids(c, f.name, b.as<double>())) return b.as<double>();
+  }
   if (it == c.gfields.end()) {
     PR_CHECK(f.written, PRESTIGE_E_STATE, "field '" + f.name + "' was never uploaded");
     PR_CHECK(generic_field_name(c, f.name), PRESTIGE_E_INVALID,
@@ -137,27 +152,39 @@ void run_equations(Ctx& c, const int32_t* eq_ids, uint32_t neq, int32_t loop, do
     pos = c.posid.p;
   }
   const prestige_desc& d = c.desc;
-  GenConsts k{d.h, d.dt, d.rho0, d.c0, d.alpha, d.kn, d.kt, d.mu, d.gravity[0], d.gravity[1], d.gravity[2]};
+  GenConsts k{d.h,  d.dt, d.rho0, d.c0, d.alpha, d.kn, d.kt, d.mu, d.gravity[0], d.gravity[1], d.gravity[2],
+              c.dem.two_zeta, c.dem.neg_inv_kt};
   const uint32_t* ids = c.ids[c.cur_ids].as<uint32_t>();
   const uint32_t* skeys = c.keys[c.cur_key].as<uint32_t>();
   const uint32_t* lb = c.lb.as<uint32_t>();
   int nx = c.grid.dims[0], ny = c.grid.dims[1], nz = c.grid.dims[2];
   double c2 = cutoff * cutoff;
+  // scan threshold of the cell walk in units of the cutoff: 1 plus the margin
+  // that keeps the FP32 scan a superset of the exact FP64 test (rounding of
+  // offsets up to the grid's extent E, in cutoffs, and of the test itself)
+  double inv_cut = loop == PRESTIGE_LOOP_CELLS ? 1.0 / cutoff : 0.0;
+  float thr = 0.0f;
+  if (loop == PRESTIGE_LOOP_CELLS) {
+    double E = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      const double lo = c.grid.lo[a], hi = lo + c.grid.dims[a] * c.grid.cs;
+      const double e = (std::fabs(lo) > std::fabs(hi) ? std::fabs(lo) : std::fabs(hi)) + (hi - lo);
+      E = e > E ? e : E;
+    }
+    E *= inv_cut;
+    thr = (float)(1.0 + 1.0 / 1024.0 + 16.0 * E * 0x1.0p-24);
+  }
   for (const GenPass& gp : prog.passes) {
     hipFunction_t fn;
     PR_HIP(hipModuleGetFunction(&fn, mod, gp.kernel.c_str()));
     // argument values must stay at fixed addresses until the launch
     std::vector<void*> fptr;
     fptr.reserve(2 * gp.fields.size());
-    std::vector<std::unique_ptr<DevBuf>> outs;  // separate outputs of split fields
     std::vector<std::string> swap_names;
     for (const GenField& f : gp.fields) {
       fptr.push_back(field_ptr(c, f));
       if (f.split) {
-        std::unique_ptr<DevBuf> o(new DevBuf());
-        o->alloc((size_t)n * sizeof(double));
-        fptr.push_back(o->p);
-        outs.push_back(std::move(o));
+        fptr.push_back(kept_buf(c.gsplit, f.name, (size_t)n * sizeof(double)).p);
         swap_names.push_back(f.name);
       }
     }
@@ -165,18 +192,15 @@ void run_equations(Ctx& c, const int32_t* eq_ids, uint32_t neq, int32_t loop, do
     // pass's inputs before it runs
     std::vector<void*> gptr;
     gptr.reserve(gp.slot_fields.size());
-    std::vector<std::unique_ptr<DevBuf>> slots;
     for (const GenField& f : gp.slot_fields) {
       const double* src = field_ptr(c, f);
-      std::unique_ptr<DevBuf> o(new DevBuf());
-      o->alloc((size_t)n * sizeof(double));
+      DevBuf& o = kept_buf(c.gslots, f.name, (size_t)n * sizeof(double));
       {
         LaunchScope ls(&c, "generated_slot_copy", 20.0 * n);
-        k_field_to_slots<<<blocks_for(n), kBlock, 0, c.stream>>>(n, ids, src, o->as<double>());
+        k_field_to_slots<<<blocks_for(n), kBlock, 0, c.stream>>>(n, ids, src, o.as<double>());
         PR_HIP(hipGetLastError());
       }
-      gptr.push_back(o->p);
-      slots.push_back(std::move(o));
+      gptr.push_back(o.p);
     }
     std::vector<void*> args;
     args.push_back(&n);
@@ -189,17 +213,21 @@ void run_equations(Ctx& c, const int32_t* eq_ids, uint32_t neq, int32_t loop, do
       args.push_back(&ny);
       args.push_back(&nz);
       args.push_back(&c2);
+      args.push_back(&inv_cut);
+      args.push_back(&thr);
     }
     args.push_back(&k);
     for (void*& p : fptr) args.push_back(&p);
     for (void*& p : gptr) args.push_back(&p);
     {
       LaunchScope ls(&c, "generated_pass", 0.0);
-      PR_HIP(hipModuleLaunchKernel(fn, blocks_for(n), 1, 1, kBlock, 1, 1, 0, c.stream, args.data(), nullptr));
+      // cell passes: one wavefront per block (their scan stages in LDS per wavefront)
+      const int bs = loop == PRESTIGE_LOOP_CELLS ? 64 : kBlock;
+      PR_HIP(hipModuleLaunchKernel(fn, blocks_for(n, bs), 1, 1, bs, 1, 1, 0, c.stream, args.data(), nullptr));
     }
     PR_HIP(hipStreamSynchronize(c.stream));
     // the pass wrote the split fields' new values to their outputs
-    for (size_t s = 0; s < outs.size(); ++s) std::swap(field_buf(c, swap_names[s]).p, outs[s]->p);
+    for (const std::string& nm : swap_names) std::swap(field_buf(c, nm).p, c.gsplit.at(nm)->p);
   }
 }
 

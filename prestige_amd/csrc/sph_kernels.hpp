@@ -149,7 +149,10 @@ __device__ inline double sqrt_pos(double x) {
   d = __builtin_fma(-g, g, x);
   return __builtin_fma(d, hh, g);
 }
-__device__ inline float sqrt_pos(float x) { return sqrtf(x); }
+// FP32 forms: the hardware square root (1 ulp) instead of the correctly
+// rounded sequence (16 instructions); their pair terms carry ~1e-7 relative
+// error from the FP32 geometry anyway.
+__device__ inline float sqrt_pos(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // Correctly rounded a / b for a positive normal b and an a whose quotient
 // stays normal (the pair terms: b = r or r^2-scaled density, a bounded):
@@ -170,7 +173,13 @@ __device__ inline double div_pos(double a, double b) {
   const double rem = __builtin_fma(-b, q, a);
   return __builtin_fma(rem, r, q);
 }
-__device__ inline float div_pos(float a, float b) { return a / b; }
+// FP32 forms: hardware reciprocal refined by one Newton step, times a (about
+// 1.5 ulp; the correctly rounded division costs 10 instructions).
+__device__ inline float div_pos(float a, float b) {
+  float r = __builtin_amdgcn_rcpf(b);
+  r = __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
+  return a * r;
+}
 
 // Mixed-form pair vector x_i - x_j from two probes (units of h).
 __device__ inline void probe_pair(const float4& qi, const float4& qj, float cs_u, float& dx, float& dy,
@@ -432,19 +441,21 @@ __device__ inline float wave_max_pos(float v) { return __uint_as_float(wave_max(
 
 // Walk of a wavefront over its lanes' stored masks (one wavefront per block;
 // every lane calls it, live or not), run groups in the WCSPH order.  For each
-// group the wavefront copies the gather data of the union of its lanes' hit
-// ranges of the group's runs into LDS with stage(k, j) (slot j -> stage
-// entry k), then each lane walks its own masks B at a time with lds(jj) (jj:
-// stage entries, kPad for unused batch entries, jj[0] always valid).  A group
-// whose unions do not fit CAP entries together is walked run by run, and a
-// run whose union alone exceeds CAP with glob(jj) (jj: slots).  Lanes visit
-// their neighbours in group order and scan order either way; end_run()
-// follows each group.
-template <int B, uint32_t CAP, class Stage, class Lds, class Glob, class End>
+// group begin(grp) runs first; then the wavefront copies the gather data of
+// the union of its lanes' hit ranges of the group's runs into LDS with
+// stage(k, j) (slot j -> stage entry k), then each lane walks its own masks B
+// at a time with lds(jj, tt) (jj: stage entries, kPad for unused batch
+// entries, jj[0] always valid; tt: which run of the group, 0 or 1, each entry
+// belongs to).  A group whose unions do not fit CAP entries together is
+// walked run by run, and a run whose union alone exceeds CAP with glob(jj,
+// tt) (jj: slots).  Lanes visit their neighbours in group order and scan
+// order either way; end_run() follows each group.
+template <int B, uint32_t CAP, class Begin, class Stage, class Lds, class Glob, class End>
 __device__ inline void walk_runs_staged(const uint2* __restrict__ meta, const uint32_t* __restrict__ words,
-                                        uint32_t n, uint32_t i, bool live, uint32_t lane, Stage&& stage, Lds&& lds,
-                                        Glob&& glob, End&& end_run) {
+                                        uint32_t n, uint32_t i, bool live, uint32_t lane, Begin&& begin,
+                                        Stage&& stage, Lds&& lds, Glob&& glob, End&& end_run) {
   for (uint32_t grp = 0; grp < kGroups; ++grp) {
+    begin(grp);
     const uint32_t nrun = grp + 1u == kGroups ? 1u : 2u;
     uint64_t m[4] = {0u, 0u, 0u, 0u};
     uint32_t j0[2] = {0u, 0u}, base[2] = {0u, 0u}, span[2] = {0u, 0u};
@@ -473,19 +484,23 @@ __device__ inline void walk_runs_staged(const uint2* __restrict__ meta, const ui
         stage(k, k < span[0] ? base[0] + k : base[1] + (k - span[0]));
       __syncthreads();
       walk_bits_w<B>(m, [&](const uint32_t* jb) {
-        uint32_t jj[B];
+        uint32_t jj[B], tt[B];
 #pragma unroll
         for (int v = 0; v < B; ++v) {
           const uint32_t q = jb[v];
+          tt[v] = q != kPad && q >= 128u ? 1u : 0u;
           jj[v] = q == kPad ? kPad : (q < 128u ? j0[0] + q - base[0] : span[0] + j0[1] + (q - 128u) - base[1]);
         }
-        lds(jj);
+        lds(jj, tt);
       });
       __syncthreads();  // the next group's staging overwrites the stage
     } else {
 #pragma unroll
       for (uint32_t t = 0; t < 2u; ++t) {
         if (span[t] == 0u) continue;
+        uint32_t tt[B];
+#pragma unroll
+        for (int v = 0; v < B; ++v) tt[v] = t;
         if (span[t] <= CAP) {
           for (uint32_t k = lane; k < span[t]; k += 64u) stage(k, base[t] + k);
           __syncthreads();
@@ -493,7 +508,7 @@ __device__ inline void walk_runs_staged(const uint2* __restrict__ meta, const ui
             uint32_t jj[B];
 #pragma unroll
             for (int v = 0; v < B; ++v) jj[v] = jb[v] == kPad ? kPad : j0[t] + jb[v] - base[t];
-            lds(jj);
+            lds(jj, tt);
           });
           __syncthreads();
         } else {
@@ -501,7 +516,7 @@ __device__ inline void walk_runs_staged(const uint2* __restrict__ meta, const ui
             uint32_t jj[B];
 #pragma unroll
             for (int v = 0; v < B; ++v) jj[v] = jb[v] == kPad ? kPad : j0[t] + jb[v];
-            glob(jj);
+            glob(jj, tt);
           });
         }
       }
@@ -633,12 +648,39 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
     CellRows cr{};
     G4 pi{};
     if (live) {
-      const float4 o = offset(spos[i]);
-      sp.x = o.x;
-      sp.y = o.y;
-      sp.z = o.z;
       cr = decode_cell(skeys[i], g);
       pi = pos[i];
+    }
+    // Mixed form, wavefront within one row of cells (the common case): scan
+    // offsets are taken from the 16-byte probes and the cell keys, in a frame
+    // at the origin of cell (cx_ref, row) in units of h, instead of from the
+    // 32-byte FP64 state.  Every candidate of run r then lies in the run's
+    // row, cells cx_ref + (key - row base), shifted by the run's (dy, dz).
+    bool probe_frame = false;
+    uint32_t cx_ref = 0, row_ref = 0;
+    if constexpr (F == kMixed) {
+      const uint32_t rowid = (uint32_t)cr.cz * (uint32_t)g.dims[1] + (uint32_t)cr.cy;
+      const uint32_t rmin = wave_min(live ? rowid : 0xFFFFFFFFu);
+      probe_frame = rmin == wave_max(live ? rowid : 0u);
+      row_ref = rmin;
+      cx_ref = wave_min(live ? (uint32_t)cr.cx : 0xFFFFFFFFu);
+    }
+    auto probe_offset = [&](uint32_t j, uint32_t rowbase, float oy, float oz) {
+      const float4 q = reinterpret_cast<const float4*>(pos)[j];
+      const int cx = (int)(skeys[j] - rowbase) - (int)cx_ref;
+      return make_float4(q.x + (float)cx * u.cs_u, q.y + oy, q.z + oz, 0.0f);
+    };
+    if (live) {
+      if (probe_frame) {
+        sp.x = (float)pi.x + (float)(cr.cx - (int)cx_ref) * u.cs_u;
+        sp.y = (float)pi.y;
+        sp.z = (float)pi.z;
+      } else {
+        const float4 o = offset(spos[i]);
+        sp.x = o.x;
+        sp.y = o.y;
+        sp.z = o.z;
+      }
     }
     // a wavefront of fixed particles whose densities cannot have changed
     // since the last pass keeps them (see DryWalls)
@@ -688,14 +730,32 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
       const int zz = cr.cz + (int)(r / 3u) - 1, yy = cr.cy + (int)(r % 3u) - 1;
       const bool row_ok = live && zz >= 0 && zz < nz && yy >= 0 && yy < ny;
       uint32_t j0 = 0, len = 0;
+      // mixed form: the run's candidates lie in row (dy, dz) of cells cx - 1,
+      // cx, cx + 1; slots [b0, b1) are cell cx, so a candidate's cell shift
+      // and with it the probe pair vector come from its slot (pair_base)
+      uint32_t b0 = 0, b1 = 0;
       if (row_ok) {
         const uint32_t row = ((uint32_t)zz * (uint32_t)ny + (uint32_t)yy) * (uint32_t)nx;
         j0 = lb[row + (uint32_t)cr.x0];
         len = lb[row + (uint32_t)cr.x1 + 1] - j0;
+        if constexpr (F == kMixed) {
+          b0 = lb[row + (uint32_t)cr.cx];
+          b1 = lb[row + (uint32_t)cr.cx + 1];
+        }
         if (len > 128u) {
           overflow = true;
           len = 128u;
         }
+      }
+      // probe pair vector (o_i - o_j) - (c_j - c_i) cs/h as (base - o_j): the
+      // y and z shifts are the run's, x is -1, 0, +1 by slot
+      float bxm = 0.0f, bx0 = 0.0f, bxp = 0.0f, by = 0.0f, bz = 0.0f;
+      if constexpr (F == kMixed) {
+        bx0 = (float)pi.x;
+        bxm = bx0 + u.cs_u;
+        bxp = bx0 - u.cs_u;
+        by = (float)pi.y - (float)((int)(r % 3u) - 1) * u.cs_u;
+        bz = (float)pi.z - (float)((int)(r / 3u) - 1) * u.cs_u;
       }
       if (live) meta[meta_index(r, n, i)] = make_uint2(j0, len);
       // union of the wavefront's candidate ranges of this run
@@ -703,11 +763,18 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
       const uint32_t top = wave_max(len ? j0 + len : 0u);
       if (top <= base) continue;
       const bool staged = top - base <= kStageCap;
+      // scan offset of candidate j in this wavefront's frame
+      const int rdy = (int)(r % 3u) - 1, rdz = (int)(r / 3u) - 1;
+      const uint32_t rowbase = (row_ref + (uint32_t)(rdz * g.dims[1] + rdy)) * (uint32_t)nx;
+      const float ofy = (float)rdy * u.cs_u, ofz = (float)rdz * u.cs_u;
+      auto cand = [&](uint32_t j) {
+        return probe_frame ? probe_offset(j, rowbase, ofy, ofz) : offset(spos[j]);
+      };
       float t = 0.0f;
       if (staged) {
         float mq = 0.0f;
         for (uint32_t kk = lane; kk < top - base; kk += kDensBlock) {
-          float4 q = offset(spos[base + kk]);
+          float4 q = cand(base + kk);
           q.w = __builtin_fmaf(q.x, q.x, __builtin_fmaf(q.y, q.y, q.z * q.z));
           s_q[kk] = q;
           mq = q.w > mq ? q.w : mq;
@@ -731,7 +798,7 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
                    j0, len, r, words, n, i, m0, m1);
         } else {
           scan_run([&](uint32_t k0, uint32_t cnt) {
-                     return scan_word([&](uint32_t j) { return offset(spos[j < n ? j : n - 1]); }, k0, cnt, sp);
+                     return scan_word([&](uint32_t j) { return cand(j < n ? j : n - 1); }, k0, cnt, sp);
                    },
                    j0, len, r, words, n, i, m0, m1);
         }
@@ -743,7 +810,14 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
           for (int v = 0; v < kDensUnr; ++v) {
             if (jj[v] == kPad) continue;
             real dx, dy, dz;
-            form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
+            if constexpr (F == kMixed) {
+              const uint32_t js = j0 + jj[v];
+              dx = (js < b0 ? bxm : (js < b1 ? bx0 : bxp)) - pj[v].x;
+              dy = by - pj[v].y;
+              dz = bz - pj[v].z;
+            } else {
+              form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
+            }
             real r2 = dx * dx + dy * dy + dz * dz;
             if (r2 < H2) {
               // the self pair (and coincident slots) has r2 = 0, q = 0
@@ -836,17 +910,15 @@ constexpr uint32_t kForceCap = PRESTIGE_FORCE_CAP;
 // put over one common denominator: a single division per pair.  ach is
 // alpha c0 times hv.
 template <int F>
-__device__ inline bool force_pair(const typename SphForm<F>::G4& pi, const typename SphForm<F>::G4& vi,
-                                  typename SphForm<F>::real rhoi, typename SphForm<F>::real pri,
-                                  const typename SphForm<F>::G4& pj, const typename SphForm<F>::G4& vj,
-                                  typename SphForm<F>::real rhoj, typename SphForm<F>::real prj,
-                                  const SphUnits& u, typename SphForm<F>::real ach,
-                                  typename SphForm<F>::real& rx, typename SphForm<F>::real& ry,
-                                  typename SphForm<F>::real& rz) {
+__device__ inline bool force_pair_vec(typename SphForm<F>::real dx, typename SphForm<F>::real dy,
+                                      typename SphForm<F>::real dz, const typename SphForm<F>::G4& vi,
+                                      typename SphForm<F>::real rhoi, typename SphForm<F>::real pri,
+                                      const typename SphForm<F>::G4& vj, typename SphForm<F>::real rhoj,
+                                      typename SphForm<F>::real prj, const SphUnits& u,
+                                      typename SphForm<F>::real ach, typename SphForm<F>::real& rx,
+                                      typename SphForm<F>::real& ry, typename SphForm<F>::real& rz) {
   using real = typename SphForm<F>::real;
   const real H2 = (real)u.H2, inv_h = (real)u.inv_h, eps2 = (real)u.eps2;
-  real dx, dy, dz;
-  form_pair<F>(pi, pj, u.cs_u, dx, dy, dz);
   real r2 = dx * dx + dy * dy + dz * dz;
   if (!(r2 < H2 && r2 > real(0.0))) return false;
   real rr = sqrt_pos(r2);
@@ -868,6 +940,19 @@ __device__ inline bool force_pair(const typename SphForm<F>::G4& pi, const typen
   ry = sc * dy;
   rz = sc * dz;
   return true;
+}
+
+template <int F>
+__device__ inline bool force_pair(const typename SphForm<F>::G4& pi, const typename SphForm<F>::G4& vi,
+                                  typename SphForm<F>::real rhoi, typename SphForm<F>::real pri,
+                                  const typename SphForm<F>::G4& pj, const typename SphForm<F>::G4& vj,
+                                  typename SphForm<F>::real rhoj, typename SphForm<F>::real prj,
+                                  const SphUnits& u, typename SphForm<F>::real ach,
+                                  typename SphForm<F>::real& rx, typename SphForm<F>::real& ry,
+                                  typename SphForm<F>::real& rz) {
+  typename SphForm<F>::real dx, dy, dz;
+  form_pair<F>(pi, pj, u.cs_u, dx, dy, dz);
+  return force_pair_vec<F>(dx, dy, dz, vi, rhoi, pri, vj, rhoj, prj, u, ach, rx, ry, rz);
 }
 
 template <int F>
@@ -910,8 +995,22 @@ __global__ __launch_bounds__(kForceBlock, kForceWaves<F>) void k_sph_force(
     }
     const real alpha_c0 = (real)kc.alpha_c0, ach = alpha_c0 * (real)u.hv;
     Sum<real, F == kMixed> sx, sy, sz;
+    // mixed form: probe pair vector (o_i - o_j) - (c_j - c_i) cs/h with the
+    // y and z cell shifts of the group's two runs folded into per-run bases
+    // (by, bz) and only the x shift read from the cell codes
+    float by[2] = {0.0f, 0.0f}, bz[2] = {0.0f, 0.0f};
+    auto begin_group = [&](uint32_t grp) {
+      if constexpr (F == kMixed) {
+#pragma unroll
+        for (uint32_t t = 0; t < 2u; ++t) {
+          const uint32_t r = sph_run(2u * grp + t);
+          by[t] = (float)pi.y - (float)((int)(r % 3u) - 1) * u.cs_u;
+          bz[t] = (float)pi.z - (float)((int)(r / 3u) - 1) * u.cs_u;
+        }
+      }
+    };
     // pair bodies over a batch of neighbours fetched by get(j, p, v, rho, pr)
-    auto body = [&](const uint32_t* jj, auto&& get) {
+    auto body = [&](const uint32_t* jj, const uint32_t* tt, auto&& get) {
       G4 pj[kUnr], vj[kUnr];
       real rhoj[kUnr], prj[kUnr];
   #pragma unroll
@@ -920,7 +1019,17 @@ __global__ __launch_bounds__(kForceBlock, kForceWaves<F>) void k_sph_force(
       for (int v = 0; v < kUnr; ++v) {
         if (jj[v] == kPad) continue;
         real rx, ry, rz;
-        if (force_pair<F>(pi, vi, rhoi, pri, pj[v], vj[v], rhoj[v], prj[v], u, ach, rx, ry, rz)) {
+        bool hit;
+        if constexpr (F == kMixed) {
+          const real dx = (pi.x - pj[v].x) -
+                          (float)cell_shift(__float_as_uint(pi.w), __float_as_uint(pj[v].w), 0) * u.cs_u;
+          const real dy = (tt[v] ? by[1] : by[0]) - pj[v].y;
+          const real dz = (tt[v] ? bz[1] : bz[0]) - pj[v].z;
+          hit = force_pair_vec<F>(dx, dy, dz, vi, rhoi, pri, vj[v], rhoj[v], prj[v], u, ach, rx, ry, rz);
+        } else {
+          hit = force_pair<F>(pi, vi, rhoi, pri, pj[v], vj[v], rhoj[v], prj[v], u, ach, rx, ry, rz);
+        }
+        if (hit) {
           sx.add(rx);
           sy.add(ry);
           sz.add(rz);
@@ -928,14 +1037,14 @@ __global__ __launch_bounds__(kForceBlock, kForceWaves<F>) void k_sph_force(
       }
     };
     walk_runs_staged<kUnr, kForceCap>(
-        meta, words, n, i, live, lane,
+        meta, words, n, i, live, lane, begin_group,
         [&](uint32_t k, uint32_t j) {
           s_p[k] = gpos[j];
           s_v[k] = gvel[j];
           if constexpr (F == kMixed) s_rp[k] = rp[j];
         },
-        [&](const uint32_t* jj) {
-          body(jj, [&](uint32_t k, G4& p, G4& v, real& rho, real& pr) {
+        [&](const uint32_t* jj, const uint32_t* tt) {
+          body(jj, tt, [&](uint32_t k, G4& p, G4& v, real& rho, real& pr) {
             p = s_p[k];
             v = s_v[k];
             if constexpr (F == kMixed) {
@@ -948,8 +1057,8 @@ __global__ __launch_bounds__(kForceBlock, kForceWaves<F>) void k_sph_force(
             }
           });
         },
-        [&](const uint32_t* jj) {
-          body(jj, [&](uint32_t j, G4& p, G4& v, real& rho, real& pr) {
+        [&](const uint32_t* jj, const uint32_t* tt) {
+          body(jj, tt, [&](uint32_t j, G4& p, G4& v, real& rho, real& pr) {
             p = gpos[j];
             v = gvel[j];
             if constexpr (F == kMixed) {
@@ -998,6 +1107,8 @@ __global__ __launch_bounds__(kForceBlock, kForceWaves<F>) void k_sph_force(
     ovel[i] = v2;
     if (oacc) oacc[i] = S4{ax, ay, az, 0};
     if (!isfinite(ax) || !isfinite(ay) || !isfinite(az)) atomicOr(&flags[FLAG_NONFINITE], 1u);
+    if ((double)v2.x * (double)v2.x + (double)v2.y * (double)v2.y + (double)v2.z * (double)v2.z > kc.cfl_v2)
+      atomicOr(&flags[FLAG_CFL], 1u);
     // next step's cell key, same FP64 floor as the key kernel
     double fx = floor(((double)x2.x - g.lo[0]) * g.inv_cs);
     double fy = floor(((double)x2.y - g.lo[1]) * g.inv_cs);
@@ -1053,6 +1164,64 @@ SphUnits sph_units(const Ctx& c) {
   return u;
 }
 
+// Per-step inputs of both gathers of form F.
+template <int F>
+struct SphStepIO {
+  SphUnits u;
+  const typename SphForm<F>::G4* gpos;
+  const typename SphForm<F>::G4* gvel;
+  const uint32_t* skeys;
+  TileQueue tq;
+  DryWalls dw;
+  uint32_t pass;
+};
+
+template <int F>
+SphStepIO<F> sph_step_io(Ctx& c) {
+  using G4 = typename SphForm<F>::G4;
+  SphStepIO<F> io;
+  io.u = sph_units<F>(c);
+  io.gpos = (F == kMixed) ? c.off32.as<G4>() : c.pos[c.cur].as<G4>();
+  io.gvel = (F == kMixed) ? c.vel32.as<G4>() : c.vel[c.cur].as<G4>();
+  io.skeys = c.keys[c.cur_key].as<uint32_t>();
+  static_assert(kDensBlock == kForceBlock, "both gathers walk the same tiles");
+  const uint32_t n_tiles = blocks_for((uint32_t)c.n, kDensBlock);
+  io.tq = TileQueue{c.tile_q.as<uint32_t>(), n_tiles, (n_tiles + 7u) / 8u};
+  io.pass = c.dens_pass;
+  io.dw = DryWalls{c.ids[c.cur_ids].as<uint32_t>(), c.n_moving, c.movcell.as<uint32_t>(),
+                   c.dry_pass.as<uint32_t>(), c.rp_id.p, c.pres_id.p, io.pass, c.stamp_pass == io.pass,
+                   c.dry_reuse, c.record_all};
+  return io;
+}
+
+// Density + EOS pass of form F into rp / pres (and, for F64 / F32, the .w
+// lanes of the state's gather copies wpos / wvel).
+template <int F>
+void sph_density_launch(Ctx& c, const SphStepIO<F>& io, typename SphForm<F>::R2* rp,
+                        typename SphForm<F>::real* pres, typename SphForm<F>::G4* wpos,
+                        typename SphForm<F>::G4* wvel) {
+  using S4 = typename SphForm<F>::S4;
+  using real = typename SphForm<F>::real;
+  const uint32_t n = (uint32_t)c.n;
+  const double ge = (double)sizeof(real);     // bytes per scalar of the pair arithmetic
+  // algorithmic bytes per particle: position (3 scalars; the FP32 probe for
+  // the mixed form) read once, (rho, p/rho^2, p) written once
+  double per = (F == kMixed ? 12.0 : 3.0 * ge) + 3.0 * ge;
+  LaunchScope ls(&c, F == kF64 ? "sph_density_f64" : (F == kF32 ? "sph_density_f32" : "sph_density_mixed"),
+                 per * n);
+#ifdef PRESTIGE_DENS_PER_CU   // resident blocks per CU fixed at build time (occupancy sweeps)
+  static const int per_cu = PRESTIGE_DENS_PER_CU;
+#else
+  static const int per_cu = blocks_per_cu(k_sph_density<F>, kDensBlock);
+#endif
+  const uint32_t dgrid = queue_grid(per_cu, c.n_cus, io.tq.n_tiles);
+  k_sph_density<F><<<dgrid, kDensBlock, 0, c.stream>>>(n, c.pos[c.cur].as<S4>(), io.gpos, io.skeys,
+                                                      c.lb.as<uint32_t>(), c.grid, c.sph, io.u, c.nmeta.as<uint2>(),
+                                                      c.nword.as<uint32_t>(), rp, wpos, wvel, pres,
+                                                      c.flags.as<uint32_t>(), io.tq, io.dw);
+  PR_HIP(hipGetLastError());
+}
+
 // Host driver of one WCSPH step in form F: rebuild, density, force.
 template <int F>
 void sph_step_impl(Ctx& c) {
@@ -1063,39 +1232,23 @@ void sph_step_impl(Ctx& c) {
   neighbors_build(c);
   const uint32_t n = (uint32_t)c.n;
   if (!n) return;
-  const SphUnits u = sph_units<F>(c);
-  const G4* gpos = (F == kMixed) ? c.off32.as<G4>() : c.pos[c.cur].as<G4>();
-  const G4* gvel = (F == kMixed) ? c.vel32.as<G4>() : c.vel[c.cur].as<G4>();
-  const uint32_t* skeys = c.keys[c.cur_key].as<uint32_t>();
-  // .w lanes of the FP64 / FP32 gather copies receive (rho, p/rho^2)
-  G4* wpos = (F == kMixed) ? nullptr : c.pos[c.cur].as<G4>();
-  G4* wvel = (F == kMixed) ? nullptr : c.vel[c.cur].as<G4>();
-  static_assert(kDensBlock == kForceBlock, "both gathers walk the same tiles");
-  const uint32_t n_tiles = blocks_for(n, kDensBlock);
-  const TileQueue tq{c.tile_q.as<uint32_t>(), n_tiles, (n_tiles + 7u) / 8u};
-  const uint32_t pass = ++c.dens_pass;
-  const DryWalls dw{c.ids[c.cur_ids].as<uint32_t>(), c.n_moving, c.movcell.as<uint32_t>(),
-                    c.dry_pass.as<uint32_t>(), c.rp_id.p, c.pres_id.p, pass, c.stamp_pass == pass,
-                    c.dry_reuse, c.record_all};
+  ++c.dens_pass;
+  const SphStepIO<F> io = sph_step_io<F>(c);
+  const SphUnits& u = io.u;
+  const G4* gpos = io.gpos;
+  const G4* gvel = io.gvel;
+  const uint32_t* skeys = io.skeys;
+  const TileQueue& tq = io.tq;
+  const uint32_t pass = io.pass;
   const double se = (F == kF32) ? 4.0 : 8.0;   // bytes per state scalar
   const double ge = (double)sizeof(real);     // bytes per scalar of the pair arithmetic
-  {
-    // algorithmic bytes per particle: position (3 scalars; the FP32 probe for
-    // the mixed form) read once, (rho, p/rho^2, p) written once
-    double per = (F == kMixed ? 12.0 : 3.0 * ge) + 3.0 * ge;
-    LaunchScope ls(&c, F == kF64 ? "sph_density_f64" : (F == kF32 ? "sph_density_f32" : "sph_density_mixed"),
-                   per * n);
-#ifdef PRESTIGE_DENS_PER_CU   // resident blocks per CU fixed at build time (occupancy sweeps)
-    static const int per_cu = PRESTIGE_DENS_PER_CU;
-#else
-    static const int per_cu = blocks_per_cu(k_sph_density<F>, kDensBlock);
-#endif
-    const uint32_t dgrid = queue_grid(per_cu, c.n_cus, tq.n_tiles);
-    k_sph_density<F><<<dgrid, kDensBlock, 0, c.stream>>>(n, c.pos[c.cur].as<S4>(), gpos, skeys, c.lb.as<uint32_t>(),
-                                                        c.grid, c.sph, u, c.nmeta.as<uint2>(),
-                                                        c.nword.as<uint32_t>(), c.rp.as<R2>(), wpos, wvel,
-                                                        c.pres.as<real>(), c.flags.as<uint32_t>(), tq, dw);
-    PR_HIP(hipGetLastError());
+  if (F == kMixed && (c.desc.flags & PRESTIGE_FLAG_DENSITY_F64)) {
+    sph_density_f64_for_mixed(c);
+  } else {
+    // .w lanes of the FP64 / FP32 gather copies receive (rho, p/rho^2)
+    G4* wpos = (F == kMixed) ? nullptr : c.pos[c.cur].as<G4>();
+    G4* wvel = (F == kMixed) ? nullptr : c.vel[c.cur].as<G4>();
+    sph_density_launch<F>(c, io, c.rp.as<R2>(), c.pres.as<real>(), wpos, wvel);
   }
   {
     // algorithmic bytes per moving particle: position, velocity (3 + 3

@@ -33,6 +33,20 @@ def test_read_write_rule_and_roles():
     assert e["roles"] == {"a": I, "b": J, "c": I, "d": I, "e": I, "f": I, "g": J, "h2": I}
 
 
+def test_only_indexed_paths_count_and_only_their_first_segment():
+    """The macro records an indexed expression only when it is a path, and then
+    path.segments[0] (prestige_macros/src/lib.rs:75-79, 105-109, 125-129): a
+    field `self.x[i]` is neither read nor written, `crate::m::x[i]` records
+    `crate`, and of `x[i][j]` only the inner path counts, as a read."""
+    e = pa.equation_describe(pa.define_equation(
+        "paths", "{ self.x[i] += m[j]; crate::m::y[i] = w[j]; z[i][j] += 1.0; ::q[i] -= 2.0; }"))
+    assert e["reads"] == ["crate", "m", "q", "w", "z"]
+    assert e["writes"] == ["crate", "q"]
+    with pytest.raises(pa.PrestigeError) as ex:
+        pa.generate_hip([pa.define_equation("fieldref", "{ self.x[i] += m[j]; }")], pa.LOOP_CELLS)
+    assert ex.value.name == "E_UNSUPPORTED"
+
+
 def test_schedule():
     # built-in WCSPH list: density + EOS, then momentum + integrate
     sph = [pa.EQ_SPH_DENSITY, pa.EQ_SPH_EOS, pa.EQ_SPH_MOMENTUM, pa.EQ_INTEGRATE]
@@ -50,9 +64,9 @@ def test_schedule():
 
 def test_generated_text():
     cells = pa.generate_hip([pa.EQ1], pa.LOOP_CELLS)
-    assert 'extern "C" __global__ void __launch_bounds__(256) prestige_pass_0(' in cells
+    assert 'extern "C" __global__ void __launch_bounds__(64) prestige_pass_0(' in cells
     assert "force_i += g_mass[js]" in cells and "const double* __restrict__ g_mass" in cells
-    assert "< c2" in cells and "lb[row" in cells
+    assert "< c2" in cells and "lb[row" in cells and "s_q[kk] = q" in cells
     pairs = pa.generate_hip([pa.EQ1], pa.LOOP_ALLPAIRS)
     assert "for (unsigned jd = 0; jd < n; ++jd)" in pairs and "force_i += f_mass[jd]" in pairs
     e = pa.define_equation("methods", "{ let d = (x[i] - x[j]).abs().sqrt(); w[i] += f64::exp(-d) * d.powi(2) }")
@@ -61,11 +75,27 @@ def test_generated_text():
     assert "double d =" in t
 
 
+def test_tuples_and_if_expressions():
+    """Tuple values (a, b), their fields t.0, destructuring lets and if expressions (chains included)
+    translate: tuples to pt_make / pt_get, if expressions to lambdas returning their branches' values."""
+    e = pa.define_equation("tup", "{ let (a, b) = (x[i] - x[j], y[i] - y[j]); "
+                                  "let s = if a > 0.0 { let t = a * 2.0; t + b } else if b > 0.0 { b } else { 0.0 }; "
+                                  "let p = (s, s * s); w[i] += p.1 - p.0; }")
+    t = pa.generate_hip([e], pa.LOOP_CELLS)
+    assert "auto __t0 = pt_make (" in t and "double a = pt_get < 0 > ( __t0 )" in t
+    assert "( [ & ] ( ) {" in t and "return t + b ;" in t and "else if" in t
+    assert "pt_get < 1 > ( p )" in t and "auto p = pt_make ( s , s * s )" in t
+    m = pa.generate_hip([pa.EQ_SPH_MOMENTUM], pa.LOOP_CELLS)
+    assert "auto f = momentum (" in m and ", prestige_k )" in m and "pt_get < 2 > ( f )" in m
+
+
 @pytest.mark.parametrize("body,what", [
-    ("{ let f = g(x[i]); a[i] += f . 0 ; }", "tuple"),
+    ("{ let f = g(x[i]); a[i] += f . 0 ; }", "unknown name"),
     ("{ a[j] += 1.0; }", "through [i] only"),
     ("{ x[i] += 1.0; }", "position"),
-    ("{ let t = if x[i] > 0.0 { 1.0 } else { 2.0 }; a[i] += t; }", "if expression"),
+    ("{ let t = if x[i] > 0.0 { 1.0 }; a[i] += t; }", "without else"),
+    ("{ let t = if x[i] > 0.0 { a[i] += 1.0; } else { 2.0 }; }", "without a value"),
+    ("{ for k in 0..3 { a[i] += 1.0; } }", "translation"),
     ("{ a[i] += undefined_name; }", "unknown name"),
     ("{ a[i + 1] += 1.0; }", "other than i or j"),
 ])
@@ -118,7 +148,11 @@ def test_generated_programs_compile_for_gfx950():
     t = pa.define_equation("twice", "{ m[i] = 2.0 * s[i]; }")
     q = pa.define_equation("nb", "{ if i != j { q[i] += m[j] * (y[i] - y[j]).abs().sqrt() } }")
     sm = pa.define_equation("smooth", "{ a[i] += 0.5 * a[j] * kernel_w(x[i] - x[j], y[i] - y[j], z[i] - z[j], h) }")
-    lists = [[pa.EQ1], [pa.EQ_SPH_DENSITY, pa.EQ_SPH_EOS], [m, t, q], [sm]]
+    tp = pa.define_equation("tup2", "{ let (a, b) = (x[i] - x[j], y[i] - y[j]); "
+                                    "let s = if a > 0.0 { let t = a * 2.0; t + b } else if b > 0.0 { b } else { 0.0 }; "
+                                    "w[i] += s; }")
+    lists = [[pa.EQ1], [pa.EQ_SPH_DENSITY, pa.EQ_SPH_EOS], [m, t, q], [sm], [pa.EQ_SPH_MOMENTUM],
+             [pa.EQ_DEM_CONTACT], [tp]]
     for eqs in lists:
         for loop in (pa.LOOP_ALLPAIRS, pa.LOOP_CELLS):
             r, log = _compile(lib, pa.generate_hip(eqs, loop))

@@ -96,19 +96,30 @@ def test_defined_equations_match_numpy(loop):
         assert np.all(got["c"] == n)
 
 
-def test_field_read_and_written_in_one_pass_uses_pre_pass_values():
+@pytest.mark.parametrize("loop", [pa.LOOP_ALLPAIRS, pa.LOOP_CELLS])
+def test_field_read_and_written_in_one_pass_uses_pre_pass_values(loop):
     """a[i] += 0.5 a[j] reads a at [j] and writes it: every j sees the values
-    from before the pass (a separate output array), not partial updates."""
+    from before the pass (a separate output array), not partial updates; a
+    second call starts from the first call's result."""
     n = 1500
     d, arrs = _pairs_ctx_desc(n, box=1.0, cell=0.1, seed=3)
     a0 = np.random.default_rng(4).random(n)
     e = pa.define_equation("smooth", "{ a[i] += 0.5 * a[j] }")
+    P = np.stack([arrs["x"], arrs["y"], arrs["z"]], 1)
+    if loop == pa.LOOP_CELLS:
+        nb = ((P[:, None, :] - P[None, :, :]) ** 2).sum(2) < d.cell_size ** 2
+    else:
+        nb = np.ones((n, n), dtype=bool)
+    step = lambda a: a + 0.5 * np.where(nb, a[None, :], 0.0).sum(1)  # noqa: E731
     with pa.Context(d) as ctx:
         upload_all(ctx, arrs)
         ctx.upload("a", a0)
-        ctx.run_equations([e], pa.LOOP_ALLPAIRS)
-        got = ctx.download("a")
-    assert np.allclose(got, a0 + 0.5 * a0.sum(), rtol=1e-12, atol=0.0)
+        ctx.run_equations([e], loop, d.cell_size)
+        got1 = ctx.download("a")
+        ctx.run_equations([e], loop, d.cell_size)
+        got2 = ctx.download("a")
+    assert np.allclose(got1, step(a0), rtol=1e-12, atol=0.0)
+    assert np.allclose(got2, step(step(a0)), rtol=1e-12, atol=0.0)
 
 
 def test_missing_input_field_is_reported():
@@ -142,3 +153,68 @@ def test_generated_density_and_eos_match_the_wcsph_kernel():
     assert np.allclose(rho, rho_ref, rtol=1e-13, atol=0.0)
     scale = np.max(np.abs(p_ref))
     assert np.max(np.abs(p - p_ref)) <= 1e-10 * scale
+
+
+def test_generated_momentum_matches_the_force_pass():
+    """The built-in momentum body (tuple-returning helper, generated over the cell iterator) against
+    the hand-written WCSPH force pass on the same positions, velocities, rho and p: the acceleration
+    minus gravity, norm-wise to 1e-13 (the sums run in another order and association)."""
+    d, arrs = make_scene(pa.SCENE_DAM_BREAK, n_target=8000, dx=0.005)
+    rng = np.random.default_rng(11)
+    arrs = dict(arrs)
+    nf = int(d.n_moving)
+    for k in ("vx", "vy", "vz"):
+        v = np.zeros(d.n_particles)
+        v[:nf] = rng.uniform(-0.05, 0.05, nf)
+        arrs[k] = v
+    with pa.Context(d) as ctx:
+        upload_all(ctx, arrs)
+        ctx.step(1)
+        rho, p = ctx.download("rho"), ctx.download("p")
+        a_ref = np.stack([ctx.download(k) for k in ("ax", "ay", "az")])
+    dp = d.copy()
+    dp.model = pa.MODEL_PAIRS
+    with pa.Context(dp) as ctx:
+        upload_all(ctx, arrs)
+        ctx.upload("mass", np.full(d.n_particles, d.mass))
+        ctx.upload("rho", rho)
+        ctx.upload("p", p)
+        ctx.run_equations([pa.EQ_SPH_MOMENTUM], pa.LOOP_CELLS, d.cell_size)
+        a = np.stack([ctx.download(k) for k in ("ax", "ay", "az")])
+    g = np.array(d.gravity[:])[:, None]
+    want = (a_ref - g)[:, :nf]
+    err = np.linalg.norm(a[:, :nf] - want) / np.linalg.norm(want)
+    print(f"generated momentum vs force pass: {err:.3e}")
+    assert err < 1e-13
+
+
+def test_generated_contact_matches_the_dem_pass():
+    """The built-in contact body (six-tuple helper) against the hand-written DEM pass on the first step
+    (fresh tangential history), without walls or gravity: force / m and dt torque / I bit for bit."""
+    d, arrs = make_scene(pa.SCENE_DEM_POUR, n_target=3000, seed=2)
+    d.wall_mask = 0
+    d.gravity[2] = 0.0
+    rng = np.random.default_rng(5)
+    n = int(d.n_particles)
+    arrs = dict(arrs)
+    # squeeze the spheres so that many pairs touch, and spin them
+    for k in ("x", "y", "z"):
+        c = arrs[k].mean()
+        arrs[k] = c + 0.93 * (arrs[k] - c)
+    w = {k: rng.uniform(-10.0, 10.0, n) for k in ("wx", "wy", "wz")}
+    with pa.Context(d) as ctx:
+        upload_all(ctx, arrs)
+        for k, v in w.items():
+            ctx.upload(k, v)
+        ctx.run_equations([pa.EQ_DEM_CONTACT], pa.LOOP_CELLS, d.cell_size)
+        f = np.stack([ctx.download(k) for k in ("fx", "fy", "fz")])
+        t = np.stack([ctx.download(k) for k in ("tx", "ty", "tz")])
+        ctx.step(1)
+        a_ref = np.stack([ctx.download(k) for k in ("ax", "ay", "az")])
+        w_ref = np.stack([ctx.download(k) for k in ("wx", "wy", "wz")])
+    m, r = arrs["mass"], arrs["radius"]
+    assert np.count_nonzero(f[0]) > 100     # many touching pairs
+    assert np.array_equal(f / m, a_ref)
+    inertia = 0.4 * m * r * r
+    w0 = np.stack([w["wx"], w["wy"], w["wz"]])
+    assert np.array_equal(w0 + d.dt * (t / inertia), w_ref)

@@ -81,7 +81,12 @@ def test_sort_policies_over_steps(monkeypatch, policy, speed, n):
         prev = None
         for _ in range(4):
             assert st.step(1) == 0
-            ctx.step(1)
+            # spheres this fast outrun the DEM step (dt |v| > r / 4): the step
+            # still runs and reports E_CFL; the sort must be exact regardless
+            try:
+                ctx.step(1)
+            except pa.PrestigeError as e:
+                assert e.name == "E_CFL" and 3 ** 0.5 * speed * d.dt > 0.25 * arrs["radius"].min(), str(e)
             # rebuild from the step's order with the keys the step computed
             assert st.build() == 0
             ctx.neighbors_build()

@@ -61,6 +61,31 @@ def test_reduced_precision_error(precision, tol):
     assert ex < tol["x"] and ev < tol["v"] and er < tol["rho"] and ep < tol["p"]
 
 
+def test_mixed_with_fp64_density():
+    """PRESTIGE_FLAG_DENSITY_F64: the first step's density and pressure are the F64 form's bit for bit
+    (same positions, same operations); later steps differ only through the FP32 force."""
+    d, arrs = make_scene(pa.SCENE_DAM_BREAK, n_target=8000, dx=0.005)
+    dm = d.copy()
+    dm.precision = pa.MIXED
+    dm.flags = pa.FLAG_DENSITY_F64
+    out = {}
+    for name, dd in (("f64", d), ("mixed_rho64", dm)):
+        with pa.Context(dd) as ctx:
+            upload_all(ctx, arrs)
+            ctx.step(1)
+            out[name] = {k: ctx.download(k) for k in ("rho", "p", "vx")}
+            ctx.step(19)
+            out[name]["v20"] = np.stack([ctx.download(k) for k in ("vx", "vy", "vz")])
+            out[name]["rho20"] = ctx.download("rho")
+    assert np.array_equal(out["mixed_rho64"]["rho"], out["f64"]["rho"])
+    assert np.array_equal(out["mixed_rho64"]["p"], out["f64"]["p"])
+    nf = d.n_moving
+    ev = rel_l2(out["mixed_rho64"]["v20"][:, :nf], out["f64"]["v20"][:, :nf])
+    er = rel_l2(out["mixed_rho64"]["rho20"][:nf], out["f64"]["rho20"][:nf])
+    print(f"mixed with FP64 density after 20 steps: v {ev:.3e} rho {er:.3e}")
+    assert ev < 1e-4 and er < 1e-7
+
+
 def test_domain_error_reported():
     d, arrs = make_scene(pa.SCENE_DAM_BREAK, n_target=8000, dx=0.005)
     x = arrs["x"].copy()

@@ -30,6 +30,19 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_density_f64_flag_is_validated():
+    """PRESTIGE_FLAG_DENSITY_F64 belongs to the MIXED WCSPH form; other forms and unknown bits are
+    refused before any device work."""
+    lib = pa.load_library()
+    for model, prec, flags in ((pa.MODEL_WCSPH, pa.F64, 1), (pa.MODEL_DEM, pa.MIXED, 1),
+                               (pa.MODEL_WCSPH, pa.MIXED, 2)):
+        d, _ = pa.scene(pa.SCENE_DAM_BREAK, n_target=1000, dx=0.01)
+        d.model, d.precision, d.flags = model, prec, flags
+        h = ctypes.c_void_p()
+        assert lib.prestige_create(ctypes.byref(d), ctypes.byref(h)) == 1
+        assert b"FLAG" in lib.prestige_last_error(None).upper()
+
+
 def test_desc_layout_matches_header():
     # offsets of a few fields pin the ctypes mirror to the C struct layout
     assert pa.Desc.n_particles.offset == 16
@@ -37,6 +50,114 @@ def test_desc_layout_matches_header():
     assert pa.Desc.wall_mask.offset == ctypes.sizeof(pa.Desc) - 8
     d = pa.desc_init(pa.MODEL_DEM)
     assert d.abi_version == 1 and d.gravity[2] == -9.81 and abs(d.kt - 2 / 7 * d.kn) < 1e-9
+
+
+# ---------------------------------------------------------------------------
+# the Rust -sys block of INTEGRATION.md against the header
+# ---------------------------------------------------------------------------
+
+_C_SCALARS = {"int32_t": "i32", "uint32_t": "u32", "uint64_t": "u64", "uint8_t": "u8", "double": "f64",
+              "size_t": "usize", "char": "c_char", "void": "c_void", "prestige_status": "prestige_status",
+              "prestige_ctx": "prestige_ctx", "prestige_desc": "prestige_desc",
+              "prestige_scene_params": "prestige_scene_params"}
+
+
+def _c_type_to_rust(decl: str) -> str:
+    """Rust spelling of one C parameter or return type (name dropped; T x[3] is a pointer)."""
+    decl = " ".join(decl.replace("*", " * ").split())
+    array = bool(re.search(r"\[\d*\]$", decl))
+    decl = re.sub(r"\s*\[\d*\]$", "", decl)
+    toks = decl.split()
+    # drop the parameter name: the last identifier when more than the type is present
+    if toks and toks[-1] not in ("*",) and toks[-1] not in _C_SCALARS and len(toks) > 1:
+        toks = toks[:-1]
+    if array:
+        toks.append("*")
+    # C pointer declarators read right to left: "const char * const *" -> *const *const c_char
+    base_const = toks[0] == "const"
+    if base_const:
+        toks = toks[1:]
+    rust = _C_SCALARS[toks[0]]
+    const_next = base_const
+    for t in toks[1:]:
+        if t == "const":
+            const_next = True
+            continue
+        assert t == "*", decl
+        rust = ("*const " if const_next else "*mut ") + rust
+        const_next = False
+    return rust
+
+
+def _header_functions():
+    text = (ROOT / "include" / "prestige_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = []
+    for ret, name, params in re.findall(r"\n([A-Za-z_][\w \*]*?)\b(prestige_\w+)\s*\(([^;{]*?)\)\s*;", text, re.S):
+        ps = [p.strip() for p in params.split(",") if p.strip() and p.strip() != "void"]
+        r = _c_type_to_rust(ret.strip() + " x")
+        out.append((name, [_c_type_to_rust(p) for p in ps], None if r == "c_void" else r))
+    return out
+
+
+def _header_struct(name):
+    text = (ROOT / "include" / "prestige_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    body = re.search(r"typedef struct " + name + r" \{(.*?)\} " + name + ";", text, re.S).group(1)
+    fields = []
+    for line in body.split(";"):
+        line = " ".join(line.split())
+        if not line:
+            continue
+        m = re.match(r"(\w+) (\w+)(\[(\d+)\])?$", line)
+        assert m, line
+        t = _C_SCALARS[m.group(1)]
+        fields.append((m.group(2), f"[{t}; {m.group(4)}]" if m.group(3) else t))
+    return fields
+
+
+def _rust_block():
+    text = (ROOT / "INTEGRATION.md").read_text()
+    blocks = re.findall(r"```rust\n(.*?)```", text, re.S)
+    return next(b for b in blocks if 'extern "C"' in b)
+
+
+def _rust_functions(block):
+    ext = block[block.index('extern "C" {'):]
+    out = []
+    for name, params, ret in re.findall(r"pub fn (prestige_\w+)\((.*?)\)\s*(?:->\s*([^;]+))?;", ext, re.S):
+        ps = [" ".join(p.split(":", 1)[1].split()) for p in params.split(",") if p.strip()]
+        out.append((name, ps, " ".join(ret.split()) if ret else None))
+    return out
+
+
+def _rust_struct(block, name):
+    body = re.search(r"pub struct " + name + r" \{(.*?)\}", block, re.S).group(1)
+    return [(m.group(1), " ".join(m.group(2).split())) for m in re.finditer(r"pub (\w+): ([^,]+),", body)]
+
+
+def test_rust_sys_block_matches_header():
+    """Every function of the header, in order, with the same parameter and return types, and the
+    #[repr(C)] structs with the same fields in the same order and types."""
+    block = _rust_block()
+    c_fns = _header_functions()
+    r_fns = _rust_functions(block)
+    assert [f[0] for f in r_fns] == [f[0] for f in c_fns]
+    for (cn, cp, cr), (rn, rp, rr) in zip(c_fns, r_fns):
+        assert rp == cp, f"{cn}: parameters {rp} != {cp}"
+        assert rr == cr, f"{cn}: return {rr} != {cr}"
+    for st in ("prestige_desc", "prestige_scene_params"):
+        assert _rust_struct(block, st) == _header_struct(st), st
+
+
+def test_c_to_rust_type_mapping():
+    assert _c_type_to_rust("const char* const* files") == "*const *const c_char"
+    assert _c_type_to_rust("const char** name") == "*mut *const c_char"
+    assert _c_type_to_rust("prestige_ctx** out") == "*mut *mut prestige_ctx"
+    assert _c_type_to_rust("const double grid_lo[3]") == "*const f64"
+    assert _c_type_to_rust("uint32_t dims[3]") == "*mut u32"
+    assert _c_type_to_rust("void* stream") == "*mut c_void"
+    assert _c_type_to_rust("uint64_t n") == "u64"
 
 
 # ---------------------------------------------------------------------------
