@@ -225,6 +225,28 @@ prestige_status prestige_sync(prestige_ctx* ctx);
  * context (creation, equation definition and code generation). */
 const char* prestige_last_error(const prestige_ctx* ctx);
 
+/* eq1 through the cell iterator on caller-owned device memory, allocating
+ * nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same result, bit for bit, as
+ * prestige_eval_equation(ctx, PRESTIGE_EQ1, PRESTIGE_LOOP_CELLS, cutoff) on a
+ * PRESTIGE_MODEL_PAIRS context with the same grid, positions, force and mass:
+ * force[i] += mass[j] over every j with |x_i - x_j| < cutoff (j == i included).
+ * x, y, z, mass (read) and force (read and written) are DEVICE pointers to n
+ * doubles in particle-id order; grid_lo / grid_hi / cell_size as in
+ * prestige_desc; stream is a hipStream_t (NULL: the null stream).  The work
+ * runs on that stream and the call returns once it has finished (it reads
+ * back the domain flag).  Intermediate arrays live in a fixed device array
+ * of about 64.5 MiB per GPU, held until the process ends: n <= 1,048,576
+ * particles and at most 4,194,304 cells, else PRESTIGE_E_CAPACITY.  Calls on
+ * one GPU share that array, so calls that overlap on one GPU race.
+ * PRESTIGE_E_DOMAIN when a particle lies outside the grid.  Errors are read
+ * with prestige_last_error(NULL). */
+prestige_status prestige_eq1_cells_noalloc(const double* x, const double* y, const double* z,
+                                           const double* mass, double* force, uint64_t n,
+                                           const double grid_lo[3], const double grid_hi[3],
+                                           double cell_size, double cutoff, void* stream);
+
 /* ---- equation interface (prestige/src/equations/ir.rs:3-14, fuse.rs:14-40,
  *      codegen/simple_cpu.rs:3-22) -------------------------------------- */
 

@@ -116,6 +116,8 @@ def _bind(lib):
         "prestige_neighbors_download": (S, [_P, _U32, _U32, _U32, _U32]),
         "prestige_grid_dims": (S, [_P, _U32]),
         "prestige_eval_equation": (S, [_P, C.c_int32, C.c_int32, C.c_double]),
+        "prestige_eq1_cells_noalloc": (S, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                           _D, _D, C.c_double, C.c_double, C.c_void_p]),
         "prestige_step": (S, [_P, C.c_uint32]),
         "prestige_forward_tape": (S, [_P, C.c_uint32, C.c_uint32]),
         "prestige_adjoint": (S, [_P, _D, _D, _D, C.c_uint32, _D]),

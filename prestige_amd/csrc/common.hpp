@@ -416,6 +416,9 @@ __device__ inline int cell_shift(uint32_t wi, uint32_t wj, int axis) {
   return (int)t - 1;
 }
 
+// Error message of calls without a context (prestige_last_error(NULL)).
+void set_context_free_error(const std::string& msg);
+
 // Per-form kernel entry points (defined in the .hip files).
 void neighbors_build(Ctx& c);
 uint64_t sort_tiles(uint64_t n);   // radix-sort tiles of n keys (histogram rows per digit)

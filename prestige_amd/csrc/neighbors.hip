@@ -5,6 +5,8 @@
 // The sorted layout is what every gather kernel iterates: for a particle in
 // cell (cx, cy, cz) the three cells cx-1..cx+1 of each of the nine (dy, dz)
 // rows are one contiguous run [lb[row + cx-1], lb[row + cx+2]) of slots.
+#include <mutex>
+
 #include "common.hpp"
 
 namespace prestige {
@@ -975,5 +977,143 @@ void eq1_eval(Ctx& c, int loop, double cutoff) {
   PR_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// eq1 over the cell iterator without allocating (prestige_eq1_cells_noalloc)
+// ---------------------------------------------------------------------------
+// The same kernels as prestige_eval_equation(PRESTIGE_EQ1, PRESTIGE_LOOP_CELLS)
+// on a fresh context (cell keys, radix sort, positions in key order, cell
+// table, the eq1 gather), on the caller's device arrays, with every
+// intermediate array in one file-scope device array: `scratch`.  It holds
+// the sort's key and value ping-pong pairs (16 B per particle), the sorted
+// positions (32 B), the cell table (4 B per cell + 4), the digit histogram
+// and totals, and the error flags, each at a 256-byte boundary, sized for at
+// most kNoAllocMaxN particles and kNoAllocMaxCells cells.  Each device holds
+// its own copy; host code reaches it only through the address
+// hipGetSymbolAddress gives on that device (noalloc_scratch), never through
+// the array's name.  Every call on a device reuses that device's copy, so
+// calls that overlap on one device race; each call clears what must start
+// at zero (the flags) with a kernel.
+constexpr uint64_t kNoAllocMaxN = 1ull << 20;
+constexpr uint64_t kNoAllocMaxCells = 1ull << 22;
+
+constexpr uint64_t na_round(uint64_t b) { return (b + 255ull) & ~255ull; }
+constexpr uint64_t kNaKeys = 0;                                               // keys[2]: 2 x 4 N
+constexpr uint64_t kNaVals = kNaKeys + 2 * na_round(4 * kNoAllocMaxN);        // vals[2]: 2 x 4 N
+constexpr uint64_t kNaPos = kNaVals + 2 * na_round(4 * kNoAllocMaxN);         // double4 x N
+constexpr uint64_t kNaLb = kNaPos + na_round(32 * kNoAllocMaxN);              // cells + 1
+constexpr uint64_t kNaHist = kNaLb + na_round(4 * (kNoAllocMaxCells + 1));    // kBuckets x tiles
+constexpr uint64_t kNaTot = kNaHist + na_round(4ull * kBuckets * ((kNoAllocMaxN + kTile - 1) / kTile));
+constexpr uint64_t kNaFlags = kNaTot + na_round(4ull * kBuckets);
+constexpr uint64_t kNoAllocBytes = kNaFlags + na_round(4ull * FLAG_COUNT);
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+__global__ void k_noalloc_clear(uint32_t* flags) {
+  if (threadIdx.x < (uint32_t)FLAG_COUNT) flags[threadIdx.x] = 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cell_keys_soa(uint32_t n, const double* __restrict__ x,
+                                                          const double* __restrict__ y, const double* __restrict__ z,
+                                                          Grid g, uint32_t* __restrict__ keys, uint32_t* flags) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) keys[i] = cell_key_of(x[i], y[i], z[i], g, flags);
+}
+
+// Positions in key order: spos[k] = x[perm[k]] (the ids of a first build are
+// the permutation itself).
+__global__ __launch_bounds__(kBlock) void k_sorted_pos_soa(uint32_t n, const double* __restrict__ x,
+                                                           const double* __restrict__ y, const double* __restrict__ z,
+                                                           const uint32_t* __restrict__ perm,
+                                                           double4* __restrict__ spos) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) {
+    const uint32_t s = perm[k];
+    spos[k] = make_double4(x[s], y[s], z[s], 0.0);
+  }
+}
+
+// Device address of this device's scratch, looked up once per device with
+// that device current.
+static void* noalloc_scratch(int device) {
+  static std::mutex mu;
+  static void* addr[64] = {};
+  PR_CHECK(device >= 0 && device < 64, PRESTIGE_E_INVALID, "device index out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    void* p = nullptr;
+    PR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    PR_CHECK(p != nullptr, PRESTIGE_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    addr[device] = p;
+  }
+  return addr[device];
+}
+
+static void eq1_cells_noalloc(const double* x, const double* y, const double* z, const double* mass, double* force,
+                              uint64_t n64, const double lo[3], const double hi[3], double cell_size, double cutoff,
+                              hipStream_t st) {
+  PR_CHECK(lo != nullptr && hi != nullptr, PRESTIGE_E_INVALID, "null grid bounds");
+  const Grid g = make_grid(lo, hi, cell_size);
+  PR_CHECK(cutoff > 0.0 && cutoff <= g.cs, PRESTIGE_E_INVALID, "cutoff must be in (0, cell_size]");
+  PR_CHECK(n64 <= kNoAllocMaxN, PRESTIGE_E_CAPACITY,
+           "n = " + std::to_string(n64) + " exceeds the " + std::to_string(kNoAllocMaxN) +
+               " particles the allocation-free scratch holds");
+  PR_CHECK(g.n_cells <= kNoAllocMaxCells, PRESTIGE_E_CAPACITY,
+           "the grid has " + std::to_string(g.n_cells) + " cells, more than the " + std::to_string(kNoAllocMaxCells) +
+               " the allocation-free scratch holds");
+  if (n64 == 0) return;
+  PR_CHECK(x && y && z && mass && force, PRESTIGE_E_INVALID, "null particle array");
+  const uint32_t n = (uint32_t)n64;
+  int dev = 0;
+  PR_HIP(hipGetDevice(&dev));
+  unsigned char* base = static_cast<unsigned char*>(noalloc_scratch(dev));
+  uint32_t* flags = reinterpret_cast<uint32_t*>(base + kNaFlags);
+  const SortBufs b{{reinterpret_cast<uint32_t*>(base + kNaKeys),
+                    reinterpret_cast<uint32_t*>(base + kNaKeys + na_round(4 * kNoAllocMaxN))},
+                   {reinterpret_cast<uint32_t*>(base + kNaVals),
+                    reinterpret_cast<uint32_t*>(base + kNaVals + na_round(4 * kNoAllocMaxN))},
+                   reinterpret_cast<uint32_t*>(base + kNaHist), reinterpret_cast<uint32_t*>(base + kNaTot), nullptr};
+  double4* spos = reinterpret_cast<double4*>(base + kNaPos);
+  uint32_t* lb = reinterpret_cast<uint32_t*>(base + kNaLb);
+  k_noalloc_clear<<<1, 64, 0, st>>>(flags);
+  PR_HIP(hipGetLastError());
+  k_cell_keys_soa<<<blocks_for(n), kBlock, 0, st>>>(n, x, y, z, g, b.keys[0], flags);
+  PR_HIP(hipGetLastError());
+  int kb = 0, vb = 0;
+  radix_sort_bufs(nullptr, b, 0, n, g.n_cells, st, &kb, &vb);
+  const uint32_t* skeys = b.keys[kb];
+  const uint32_t* perm = b.vals[vb];
+  k_sorted_pos_soa<<<blocks_for(n), kBlock, 0, st>>>(n, x, y, z, perm, spos);
+  PR_HIP(hipGetLastError());
+  const uint32_t key_blocks = (uint32_t)blocks_for(n);
+  k_cell_lb<<<key_blocks + kTailBlocks, kBlock, 0, st>>>(n, g.n_cells, skeys, lb, key_blocks);
+  PR_HIP(hipGetLastError());
+  k_eq1_cells<double4><<<blocks_for(n), kBlock, 0, st>>>(n, spos, perm, skeys, lb, g, cutoff * cutoff, mass, force);
+  PR_HIP(hipGetLastError());
+  uint32_t h[FLAG_COUNT] = {0, 0, 0, 0};
+  PR_HIP(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, st));
+  PR_HIP(hipStreamSynchronize(st));
+  PR_CHECK(h[FLAG_DOMAIN] == 0u, PRESTIGE_E_DOMAIN, "a particle lies outside the cell grid");
+}
+
 }  // namespace prestige
 
+extern "C" prestige_status prestige_eq1_cells_noalloc(const double* x, const double* y, const double* z,
+                                                      const double* mass, double* force, uint64_t n,
+                                                      const double grid_lo[3], const double grid_hi[3],
+                                                      double cell_size, double cutoff, void* stream) {
+  try {
+    prestige::eq1_cells_noalloc(x, y, z, mass, force, n, grid_lo, grid_hi, cell_size, cutoff,
+                                static_cast<hipStream_t>(stream));
+    prestige::set_context_free_error("");
+    return PRESTIGE_OK;
+  } catch (const prestige::Error& e) {
+    prestige::set_context_free_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    prestige::set_context_free_error(e.what());
+    return PRESTIGE_E_INVALID;
+  } catch (...) {
+    prestige::set_context_free_error("unknown C++ exception");
+    return PRESTIGE_E_INVALID;
+  }
+}
