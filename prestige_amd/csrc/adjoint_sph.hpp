@@ -380,15 +380,30 @@ __global__ __launch_bounds__(kAdjBlock, kAdjDensityWaves<F>) void k_adj_density(
 }
 
 // Sum the block partials of one reverse step into acc[0..1] (one block).
-static __global__ void k_adj_reduce(uint32_t nparts, const double2* __restrict__ partial, double* __restrict__ acc) {
-  __shared__ double sa[256], sb[256];
-  double a = 0.0, b = 0.0;
-  for (uint32_t k = threadIdx.x; k < nparts; k += blockDim.x) {
-    a += partial[k].x;
-    b += partial[k].y;
+constexpr uint32_t kAdjReduceBlock = 1024;
+// One block, fixed order: thread t sums partials t + 1024 (4 q + s) into its
+// accumulator s (four independent load chains, so the loads overlap), then the
+// four and the block's threads are combined in a fixed tree.
+static __global__ __launch_bounds__(kAdjReduceBlock) void k_adj_reduce(uint32_t nparts,
+                                                                       const double2* __restrict__ partial,
+                                                                       double* __restrict__ acc) {
+  __shared__ double sa[kAdjReduceBlock], sb[kAdjReduceBlock];
+  double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
+  for (uint32_t k0 = 0; k0 < nparts; k0 += 4u * kAdjReduceBlock) {
+    double2 v[4];
+#pragma unroll
+    for (uint32_t s = 0; s < 4u; ++s) {
+      const uint32_t k = k0 + s * kAdjReduceBlock + threadIdx.x;
+      v[s] = k < nparts ? partial[k] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (uint32_t s = 0; s < 4u; ++s) {
+      a[s] += v[s].x;
+      b[s] += v[s].y;
+    }
   }
-  sa[threadIdx.x] = a;
-  sb[threadIdx.x] = b;
+  sa[threadIdx.x] = (a[0] + a[1]) + (a[2] + a[3]);
+  sb[threadIdx.x] = (b[0] + b[1]) + (b[2] + b[3]);
   __syncthreads();
   for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) {
@@ -433,7 +448,7 @@ void sph_reverse_step(Ctx& c, const SphStepRecord& rec, AdjBufs& A) {
         n, c.n_moving, rec, c.grid, c.sph, u, c.desc.alpha, c.desc.c0, c.desc.rho0,
         (const typename AdjForm<F>::C4*)A.cvec, A.xbf, A.vb, (typename AdjForm<F>::ar*)A.rhob, A.partial, tq);
     PR_HIP(hipGetLastError());
-    k_adj_reduce<<<1, 256, 0, c.stream>>>(n_tiles, A.partial, A.acc);
+    k_adj_reduce<<<1, kAdjReduceBlock, 0, c.stream>>>(n_tiles, A.partial, A.acc);
     PR_HIP(hipGetLastError());
   }
   {

@@ -692,17 +692,35 @@ std::vector<uint32_t> schedule(const int32_t* ids, uint32_t n) {
 }
 
 const char* kPreamble = R"(// Device helpers available to equation bodies.
+// pg_sqrt: correctly rounded square root of a positive normal x, the hardware
+// reciprocal square root refined by Newton steps with a final remainder
+// correction (Markstein 2000); sqrt() adds exponent scaling for denormal and
+// special arguments, which pair distances never are.
+__device__ inline double pg_sqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y;
+  double hh = y * 0.5;
+  const double r = __builtin_fma(-hh, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  hh = __builtin_fma(hh, r, hh);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, hh, g);
+  d = __builtin_fma(-g, g, x);
+  return __builtin_fma(d, hh, g);
+}
 // kernel_w: cubic spline W(r, h) = sigma f(r / h), sigma = 1 / (pi h^3),
-// support 2h (Monaghan & Lattanzio 1985).
+// support 2h (Monaghan & Lattanzio 1985).  Both pieces are evaluated and
+// selected (lanes of a wavefront straddle r = h); 1 / h and sigma depend on h
+// only and leave the pair loop.
 __device__ inline double kernel_w(double dx, double dy, double dz, double h) {
-  const double q = sqrt(dx * dx + dy * dy + dz * dz) / h;
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  const double inv_h = 1.0 / h;
   const double s = 1.0 / (3.141592653589793 * h * h * h);
-  if (q < 1.0) return s * (1.0 - 1.5 * q * q + 0.75 * q * q * q);
-  if (q < 2.0) {
-    const double t = 2.0 - q;
-    return s * 0.25 * t * t * t;
-  }
-  return 0.0;
+  const double q = r2 > 0.0 ? pg_sqrt(r2) * inv_h : 0.0;
+  const double inner = 1.0 - 1.5 * q * q + 0.75 * q * q * q;
+  const double t = 2.0 - q;
+  const double outer = 0.25 * t * t * t;
+  return q < 1.0 ? s * inner : (q < 2.0 ? s * outer : 0.0);
 }
 // tait: p = max(rho0 c0^2 / 7 ((rho / rho0)^7 - 1), 0)
 __device__ inline double tait(double rho, double rho0, double c0) {
@@ -717,9 +735,9 @@ struct prestige_consts {
 // Cell-loop passes run one wavefront per block over 64 consecutive slots.  Per row
 // run the wavefront stages the union of its lanes' candidate ranges in LDS as FP32
 // offsets from its first slot in units of the cutoff (|q|^2 in .w); each lane scans
-// its own range 32 candidates at a time with the dot form q.w - 2 s.q < t and walks
-// the set bits in ascending slot order, applying the exact FP64 test before the
-// bodies.  The threshold t carries the rounding margin of the FP32 form, so the
+// its own range into words of 32 candidates with the dot form q.w - 2 s.q < t, up
+// to 128 candidates at a time, and walks their set bits in ascending slot order, two
+// neighbours per trip, applying the exact FP64 test before the bodies.  The threshold t carries the rounding margin of the FP32 form, so the
 // scan keeps a superset of the exact neighbours and the bodies see exactly the
 // pairs and the order of the plain loop.
 __device__ inline unsigned pg_wave_min(unsigned v) {
@@ -753,7 +771,7 @@ __device__ inline pt_tuple<3> momentum(double dx, double dy, double dz, double d
   const double r2 = dx * dx + dy * dy + dz * dz;
   if (!(r2 < H2 && r2 > 0.0)) return pt_tuple<3>{{0.0, 0.0, 0.0}};
   const double eps2 = (0.01 * k.h) * k.h;
-  const double rr = sqrt(r2);
+  const double rr = pg_sqrt(r2);
   const double q = rr * inv_h;
   double dw;
   if (q < 1.0) {
@@ -788,7 +806,7 @@ __device__ inline pt_tuple<6> contact(double dx, double dy, double dz, double ri
   const double r2 = dx * dx + dy * dy + dz * dz;
   const double R = ri + rj;
   if (!(r2 < R * R && r2 > 0.0)) return pt_tuple<6>{{0.0, 0.0, 0.0, 0.0, 0.0, 0.0}};
-  const double d = sqrt(r2);
+  const double d = pg_sqrt(r2);
   const double invd = 1.0 / d;
   const double nx = dx * invd, ny = dy * invd, nz = dz * invd;
   const double delta = R - d;
@@ -900,7 +918,7 @@ GenProgram build_program(const int32_t* ids, uint32_t n, int32_t loop, bool f32_
            "(\n    unsigned n, const prestige_pos* __restrict__ pos, prestige_consts prestige_k" + params + ") {\n";
     s += "#pragma clang fp contract(off)\n";
     if (loop == PRESTIGE_LOOP_CELLS) {
-      s += "  __shared__ float4 s_q[256];\n"
+      s += "  __shared__ float4 s_q[256 + 4];\n"
            "  const unsigned lane = threadIdx.x, wbase = blockIdx.x * 64u, i = wbase + lane;\n"
            "  const bool live = i < n;\n"
            "  const unsigned me = live ? ids[i] : 0u;\n"
@@ -957,32 +975,66 @@ GenProgram build_program(const int32_t* ids, uint32_t n, int32_t loop, bool f32_
              "        t = thr + 0x1.0p-19f * (m2 + __builtin_sqrtf(m2) + 1.0f) - ss;\n"
              "        __syncthreads();\n"
              "      }\n"
-             "      for (unsigned w0 = 0u; w0 < len; w0 += 32u) {\n"
-             "        const unsigned cnt = len - w0 < 32u ? len - w0 : 32u;\n"
-             "        unsigned word = 0u;\n"
-             "        for (unsigned b = 0u; b < cnt; ++b) {\n"
-             "          const unsigned jc = j0 + w0 + b;\n"
-             "          bool hit;\n"
-             "          if (staged) {\n"
-             "            const float4 q = s_q[jc - base];\n"
-             "            hit = __builtin_fmaf(ax_, q.x, __builtin_fmaf(ay_, q.y, __builtin_fmaf(az_, q.z, q.w))) < t;\n"
-             "          } else {\n"
-             "            const float4 q = offset(pos[jc]);\n"
-             "            const float ddx = so.x - q.x, ddy = so.y - q.y, ddz = so.z - q.z;\n"
-             "            hit = __builtin_fmaf(ddx, ddx, __builtin_fmaf(ddy, ddy, ddz * ddz)) < thr;\n"
+             "      // up to 128 candidates at a time: four scan words, then one walk of their set bits\n"
+             "      for (unsigned c0 = 0u; c0 < len; c0 += 128u) {\n"
+             "        const unsigned clen = len - c0 < 128u ? len - c0 : 128u;\n"
+             "        unsigned wd[4] = {0u, 0u, 0u, 0u};\n"
+             "#pragma unroll\n"
+             "        for (unsigned w = 0u; w < 4u; ++w) {\n"
+             "          if (32u * w >= clen) break;\n"
+             "          const unsigned cnt = clen - 32u * w < 32u ? clen - 32u * w : 32u;\n"
+             "          unsigned word = 0u;\n"
+             "          // four candidates per trip; those past cnt (stage padding, clamped slots) are masked off\n"
+             "          for (unsigned b = 0u; b < cnt; b += 4u) {\n"
+             "            unsigned hb = 0u;\n"
+             "#pragma unroll\n"
+             "            for (unsigned u = 0u; u < 4u; ++u) {\n"
+             "              const unsigned jc = j0 + c0 + 32u * w + b + u;\n"
+             "              bool hit;\n"
+             "              if (staged) {\n"
+             "                const float4 q = s_q[jc - base];\n"
+             "                hit = __builtin_fmaf(ax_, q.x, __builtin_fmaf(ay_, q.y, __builtin_fmaf(az_, q.z, q.w))) < t;\n"
+             "              } else {\n"
+             "                const float4 q = offset(pos[jc < n ? jc : n - 1u]);\n"
+             "                const float ddx = so.x - q.x, ddy = so.y - q.y, ddz = so.z - q.z;\n"
+             "                hit = __builtin_fmaf(ddx, ddx, __builtin_fmaf(ddy, ddy, ddz * ddz)) < thr;\n"
+             "              }\n"
+             "              hb |= (hit ? 1u : 0u) << u;\n"
+             "            }\n"
+             "            word |= hb << b;\n"
              "          }\n"
-             "          word |= (hit ? 1u : 0u) << b;\n"
+             "          wd[w] = cnt < 32u ? word & ((1u << cnt) - 1u) : word;\n"
              "        }\n"
-             "        while (word) {\n"
-             "          const unsigned js = j0 + w0 + (unsigned)__builtin_ctz(word);\n"
-             "          word &= word - 1u;\n"
-             "          const prestige_pos pj = pos[js];\n"
+             "        unsigned long long m0 = (unsigned long long)wd[0] | ((unsigned long long)wd[1] << 32);\n"
+             "        unsigned long long m1 = (unsigned long long)wd[2] | ((unsigned long long)wd[3] << 32);\n"
+             "        auto pop = [&]() -> unsigned {\n"
+             "          if (m0) {\n"
+             "            const unsigned b = (unsigned)__builtin_ctzll(m0);\n"
+             "            m0 &= m0 - 1ull;\n"
+             "            return b;\n"
+             "          }\n"
+             "          const unsigned b = 64u + (unsigned)__builtin_ctzll(m1);\n"
+             "          m1 &= m1 - 1ull;\n"
+             "          return b;\n"
+             "        };\n"
+             "        while (m0 | m1) {\n"
+             "          // two neighbours per trip, both positions loaded before either body runs\n"
+             "          unsigned jsv[2];\n"
+             "          jsv[0] = j0 + c0 + pop();\n"
+             "          const bool two = (m0 | m1) != 0ull;\n"
+             "          jsv[1] = two ? j0 + c0 + pop() : jsv[0];\n"
+             "          const prestige_pos pjv[2] = {pos[jsv[0]], pos[jsv[1]]};\n"
+             "#pragma unroll\n"
+             "          for (int u_ = 0; u_ < 2; ++u_) {\n"
+             "          if (u_ == 1 && !two) break;\n"
+             "          const unsigned js = jsv[u_];\n"
+             "          const prestige_pos pj = pjv[u_];\n"
              "          const double ex = (double)pi.x - (double)pj.x, ey = (double)pi.y - (double)pj.y,\n"
              "                       ez = (double)pi.z - (double)pj.z;\n"
              "          if (!(ex * ex + ey * ey + ez * ez < c2)) continue;\n"
              "          const unsigned jd = ids[js];\n";
         s += jcode;
-        s += "        }\n      }\n      if (staged) __syncthreads();\n    }\n  }\n  if (!live) return;\n";
+        s += "          }\n        }\n      }\n      if (staged) __syncthreads();\n    }\n  }\n  if (!live) return;\n";
       } else {
         s += "  for (unsigned jd = 0; jd < n; ++jd) {\n      const prestige_pos pj = pos[jd];\n";
         s += jcode;
