@@ -49,8 +49,9 @@ void orc_cell_lb(uint64_t n, const uint32_t* sorted_keys, uint32_t n_cells, uint
   }
 }
 
+/* the WCSPH passes' fused forms of the cubic spline (physics_impl.h) */
 double orc_spline_f(double q) {
-  if (q < 1.0) return 1.0 - 1.5 * q * q + 0.75 * q * q * q;
+  if (q < 1.0) return fma(q * q, fma(0.75, q, -1.5), 1.0);
   if (q < 2.0) {
     double t = 2.0 - q;
     return 0.25 * t * t * t;
@@ -59,7 +60,7 @@ double orc_spline_f(double q) {
 }
 
 double orc_spline_df(double q) {
-  if (q < 1.0) return -3.0 * q + 2.25 * q * q;
+  if (q < 1.0) return q * fma(2.25, q, -3.0);
   if (q < 2.0) {
     double t = 2.0 - q;
     return -0.75 * t * t;
@@ -85,6 +86,7 @@ void orc_eq1_allpairs(uint64_t n, double* force, const double* mass) {
 #define RE(v) (v)
 #define SQRT(v) sqrt(v)
 #define LOG(v) log(v)
+#define FMA(a, b, c) fma((a), (b), (c))
 #include "physics_impl.h"
 #undef REAL
 #undef NAME
@@ -92,6 +94,7 @@ void orc_eq1_allpairs(uint64_t n, double* force, const double* mass) {
 #undef RE
 #undef SQRT
 #undef LOG
+#undef FMA
 
 /* ---------------------------------------------------------------------- */
 /* complex-step instantiation                                              */
@@ -102,6 +105,8 @@ void orc_eq1_allpairs(uint64_t n, double* force, const double* mass) {
 #define RE(v) creal(v)
 #define SQRT(v) csqrt(v)
 #define LOG(v) clog(v)
+/* complex step: the tangent of a fused multiply-add is that of a b + c */
+#define FMA(a, b, c) ((a) * (b) + (c))
 #include "physics_impl.h"
 #undef REAL
 #undef NAME
@@ -109,6 +114,7 @@ void orc_eq1_allpairs(uint64_t n, double* force, const double* mass) {
 #undef RE
 #undef SQRT
 #undef LOG
+#undef FMA
 
 /* ---------------------------------------------------------------------- */
 /* public FP64 API                                                         */

@@ -244,12 +244,14 @@ static void NAME(density)(STATE* s) {
     for (int r = 0; r < nr; ++r) {
       for (uint32_t j = rlo[r]; j < rhi[r]; ++j) {
         REAL dx = xi - s->x[j], dy = yi - s->y[j], dz = zi - s->z[j];
-        REAL r2 = dx * dx + dy * dy + dz * dz;
+        /* fused forms (fma: one rounding), the device's operation order */
+        REAL r2 = FMA(dx, dx, FMA(dy, dy, dz * dz));
         if (RE(r2) < H2) {
           REAL q = SQRT(r2) * inv_h;
           REAL w;
           if (RE(q) < 1.0) {
-            w = 1.0 - 1.5 * q * q + 0.75 * q * q * q;
+            /* 1 - 1.5 q^2 + 0.75 q^3 = 1 + q^2 (0.75 q - 1.5) */
+            w = FMA(q * q, FMA(0.75, q, -1.5), 1.0);
           } else {
             REAL t = 2.0 - q;
             w = 0.25 * t * t * t;
@@ -293,35 +295,36 @@ static void NAME(sph_force_integrate)(STATE* s) {
     for (int r = 0; r < nr; ++r) {
       for (uint32_t j = rlo[r]; j < rhi[r]; ++j) {
         REAL dx = xi - s->x[j], dy = yi - s->y[j], dz = zi - s->z[j];
-        REAL r2 = dx * dx + dy * dy + dz * dz;
+        REAL r2 = FMA(dx, dx, FMA(dy, dy, dz * dz));
         if (RE(r2) < H2 && RE(r2) > 0.0) {
           REAL rr = SQRT(r2);
           REAL q = rr * inv_h;
           REAL dw;
           if (RE(q) < 1.0) {
-            dw = -3.0 * q + 2.25 * q * q;
+            /* -3 q + 2.25 q^2 = q (2.25 q - 3) */
+            dw = q * FMA(2.25, q, -3.0);
           } else {
             REAL t = 2.0 - q;
             dw = -0.75 * t * t;
           }
           REAL dvx = vxi - s->vx[j], dvy = vyi - s->vy[j], dvz = vzi - s->vz[j];
-          REAL vdotr = dvx * dx + dvy * dy + dvz * dz;
+          REAL vdotr = FMA(dvx, dx, FMA(dvy, dy, dvz * dz));
           /* sc = (pr_i + pr_j + Pi_ij) f'(q) / r with Pi_ij = -alpha c0 h (v.r) /
            * ((r^2 + eps2) rhobar) for approaching pairs, put over one common
            * denominator so that every pair costs a single division */
           REAL num, den;
           if (RE(vdotr) < 0.0) {
             REAL drb = (r2 + eps2) * (0.5 * (rhoi + s->rho[j]));
-            num = ((pri + s->pr[j]) * drb - ach * vdotr) * dw;
+            num = FMA(pri + s->pr[j], drb, -(ach * vdotr)) * dw;
             den = drb * rr;
           } else {
             num = (pri + s->pr[j]) * dw;
             den = rr;
           }
           REAL sc = num / den;
-          sx = sx + sc * dx;
-          sy = sy + sc * dy;
-          sz = sz + sc * dz;
+          sx = FMA(sc, dx, sx);
+          sy = FMA(sc, dy, sy);
+          sz = FMA(sc, dz, sz);
         }
       }
     }

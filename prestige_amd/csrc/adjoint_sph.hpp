@@ -194,7 +194,7 @@ __global__ __launch_bounds__(kAdjBlock, kAdjForceWaves<F>) void k_adj_force(
           if (jj[v] == kPad) continue;
           real dxr, dyr, dzr;
           form_pair<F>(pi, pj[v], u.cs_u, dxr, dyr, dzr);
-          real r2r = dxr * dxr + dyr * dyr + dzr * dzr;
+          real r2r = dot3(dxr, dyr, dzr, dxr, dyr, dzr);
           if (!(r2r < H2 && r2r > real(0.0))) continue;
           const ar dx = (ar)dxr, dy = (ar)dyr, dz = (ar)dzr, r2 = (ar)r2r;
           ar rr, irr;
@@ -204,7 +204,8 @@ __global__ __launch_bounds__(kAdjBlock, kAdjForceWaves<F>) void k_adj_force(
           const ar gr = f1 * irr;
           const ar gp = (f2 * inv_h - gr) * irr;   // d(f'(q)/r)/dr
           const ar ex = (ar)vi.x - (ar)vj[v].x, ey = (ar)vi.y - (ar)vj[v].y, ez = (ar)vi.z - (ar)vj[v].z;
-          const ar w = ex * dx + ey * dy + ez * dz;
+          // the forward pass's branch: v.r in its operation order
+          const ar w = dot3(ex, ey, ez, dx, dy, dz);
           const ar D = r2 + eps2, iD = adj_rcp(D);
           const ar rbar = ar(0.5) * (rhoi + (ar)rpj[v].x), irb = adj_rcp(rbar);
           ar mu = ar(0.0), Pi = ar(0.0);
@@ -331,7 +332,7 @@ __global__ __launch_bounds__(kAdjBlock, kAdjDensityWaves<F>) void k_adj_density(
         if (jj[v] == kPad) continue;
         real dxr, dyr, dzr;
         form_pair<F>(pi, pj[v], u.cs_u, dxr, dyr, dzr);
-        real r2r = dxr * dxr + dyr * dyr + dzr * dzr;
+        real r2r = dot3(dxr, dyr, dzr, dxr, dyr, dzr);
         if (!(r2r < H2 && r2r > real(0.0))) continue;
         ar rr, irr;
         adj_len((ar)r2r, rr, irr);

@@ -713,11 +713,11 @@ __device__ inline double pg_sqrt(double x) {
 // selected (lanes of a wavefront straddle r = h); 1 / h and sigma depend on h
 // only and leave the pair loop.
 __device__ inline double kernel_w(double dx, double dy, double dz, double h) {
-  const double r2 = dx * dx + dy * dy + dz * dz;
+  const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
   const double inv_h = 1.0 / h;
   const double s = 1.0 / (3.141592653589793 * h * h * h);
   const double q = r2 > 0.0 ? pg_sqrt(r2) * inv_h : 0.0;
-  const double inner = 1.0 - 1.5 * q * q + 0.75 * q * q * q;
+  const double inner = __builtin_fma(q * q, __builtin_fma(0.75, q, -1.5), 1.0);
   const double t = 2.0 - q;
   const double outer = 0.25 * t * t * t;
   return q < 1.0 ? s * inner : (q < 2.0 ? s * outer : 0.0);
@@ -740,6 +740,28 @@ struct prestige_consts {
 // neighbours per trip, applying the exact FP64 test before the bodies.  The threshold t carries the rounding margin of the FP32 form, so the
 // scan keeps a superset of the exact neighbours and the bodies see exactly the
 // pairs and the order of the plain loop.
+// Tiles of 64 slots over per-XCD queues: block b starts on range b mod 8 (the
+// XCD it runs on) and takes that range's tiles in order from a counter, then
+// the other ranges' in turn; the last block to finish clears the counters.
+// One wavefront per block.
+template <class F>
+__device__ inline void pg_for_each_tile(unsigned* q_ctr, unsigned n_tiles, unsigned per, F&& body) {
+  unsigned q = blockIdx.x & 7u;
+  for (unsigned tries = 0; tries < 8u;) {
+    unsigned t = 0;
+    if (threadIdx.x == 0u) t = atomicAdd(&q_ctr[q * 32u], 1u);
+    t = (unsigned)__builtin_amdgcn_readfirstlane((int)t);
+    if (t >= per) {
+      q = (q + 1u) & 7u;
+      ++tries;
+      continue;
+    }
+    const unsigned tile = q * per + t;
+    if (tile < n_tiles) body(tile);
+  }
+  if (threadIdx.x == 0u && atomicAdd(&q_ctr[8u * 32u], 1u) == gridDim.x - 1u)
+    for (unsigned k = 0; k <= 8u; ++k) __hip_atomic_store(&q_ctr[k * 32u], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ inline unsigned pg_wave_min(unsigned v) {
   for (int d = 32; d > 0; d >>= 1) {
     const unsigned t = __shfl_xor(v, d, 64);
@@ -768,24 +790,24 @@ __device__ inline pt_tuple<3> momentum(double dx, double dy, double dz, double d
                                        double psum, double rsum, double mj, const prestige_consts& k) {
   const double inv_h = 1.0 / k.h;
   const double H2 = (2.0 * k.h) * (2.0 * k.h);
-  const double r2 = dx * dx + dy * dy + dz * dz;
+  const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
   if (!(r2 < H2 && r2 > 0.0)) return pt_tuple<3>{{0.0, 0.0, 0.0}};
   const double eps2 = (0.01 * k.h) * k.h;
   const double rr = pg_sqrt(r2);
   const double q = rr * inv_h;
   double dw;
   if (q < 1.0) {
-    dw = -3.0 * q + 2.25 * q * q;
+    dw = q * __builtin_fma(2.25, q, -3.0);
   } else {
     const double t = 2.0 - q;
     dw = -0.75 * t * t;
   }
-  const double vdotr = dvx * dx + dvy * dy + dvz * dz;
+  const double vdotr = __builtin_fma(dvx, dx, __builtin_fma(dvy, dy, dvz * dz));
   const double ach = (k.alpha * k.c0) * k.h;
   double num, den;
   if (vdotr < 0.0) {
     const double drb = (r2 + eps2) * (0.5 * rsum);
-    num = (psum * drb - ach * vdotr) * dw;
+    num = __builtin_fma(psum, drb, -(ach * vdotr)) * dw;
     den = drb * rr;
   } else {
     num = psum * dw;
@@ -912,14 +934,16 @@ GenProgram build_program(const int32_t* ids, uint32_t n, int32_t loop, bool f32_
       s += "extern \"C\" __global__ void __launch_bounds__(64) " + gp.kernel +
            "(\n    unsigned n, const prestige_pos* __restrict__ pos, const unsigned* __restrict__ ids,\n"
            "    const unsigned* __restrict__ skeys, const unsigned* __restrict__ lb, int nx, int ny, int nz,\n"
-           "    double c2, double inv_cut, float thr, prestige_consts prestige_k" + params + ") {\n";
+           "    double c2, double inv_cut, float thr, unsigned* __restrict__ pg_q, unsigned pg_tiles,\n"
+           "    unsigned pg_per, prestige_consts prestige_k" + params + ") {\n";
     else
       s += "extern \"C\" __global__ void __launch_bounds__(256) " + gp.kernel +
            "(\n    unsigned n, const prestige_pos* __restrict__ pos, prestige_consts prestige_k" + params + ") {\n";
     s += "#pragma clang fp contract(off)\n";
     if (loop == PRESTIGE_LOOP_CELLS) {
       s += "  __shared__ float4 s_q[256 + 4];\n"
-           "  const unsigned lane = threadIdx.x, wbase = blockIdx.x * 64u, i = wbase + lane;\n"
+           "  pg_for_each_tile(pg_q, pg_tiles, pg_per, [&](unsigned pg_tile) {\n"
+           "  const unsigned lane = threadIdx.x, wbase = pg_tile * 64u, i = wbase + lane;\n"
            "  const bool live = i < n;\n"
            "  const unsigned me = live ? ids[i] : 0u;\n"
            "  const prestige_pos pi = pos[live ? i : wbase];\n"
@@ -1046,6 +1070,7 @@ GenProgram build_program(const int32_t* ids, uint32_t n, int32_t loop, bool f32_
       const bool sp = pn.split.count(f) > 0;
       s += "  f_" + f + (sp ? "_out" : "") + "[me] = " + f + "_i;\n";
     }
+    if (loop == PRESTIGE_LOOP_CELLS) s += "  });\n";
     s += "}\n";
     prog.passes.push_back(gp);
   }

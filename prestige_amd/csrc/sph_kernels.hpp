@@ -203,18 +203,31 @@ __device__ inline void form_pair(const typename SphForm<F>::G4& pi, const typena
   }
 }
 
-// Cubic spline shape f(q) for q < 2 (W = sigma f).
+// Fused multiply-add, one rounding (a correctly rounded a b + c in either
+// precision).  The WCSPH pair arithmetic writes its dot products and spline
+// polynomials with it, in the same places as the oracle.
+__device__ inline double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ inline float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// a.b of two 3-vectors as fma(ax, bx, fma(ay, by, az bz)).
+template <class real>
+__device__ inline real dot3(real ax, real ay, real az, real bx, real by, real bz) {
+  return fmad(ax, bx, fmad(ay, by, az * bz));
+}
+
+// Cubic spline shape f(q) for q < 2 (W = sigma f): 1 + q^2 (0.75 q - 1.5)
+// below q = 1, 0.25 (2 - q)^3 above.
 template <class real>
 __device__ inline real spline_f(real q) {
-  if (q < real(1.0)) return real(1.0) - real(1.5) * q * q + real(0.75) * q * q * q;
+  if (q < real(1.0)) return fmad(q * q, fmad(real(0.75), q, real(-1.5)), real(1.0));
   real t = real(2.0) - q;
   return real(0.25) * t * t * t;
 }
 
-// df/dq for q < 2.
+// df/dq for q < 2: q (2.25 q - 3) below q = 1, -0.75 (2 - q)^2 above.
 template <class real>
 __device__ inline real spline_df(real q) {
-  if (q < real(1.0)) return real(-3.0) * q + real(2.25) * q * q;
+  if (q < real(1.0)) return q * fmad(real(2.25), q, real(-3.0));
   real t = real(2.0) - q;
   return real(-0.75) * t * t;
 }
@@ -818,7 +831,7 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
             } else {
               form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
             }
-            real r2 = dx * dx + dy * dy + dz * dz;
+            real r2 = dot3(dx, dy, dz, dx, dy, dz);
             if (r2 < H2) {
               // the self pair (and coincident slots) has r2 = 0, q = 0
               real q = r2 > real(0.0) ? sqrt_pos(r2) * inv_h : real(0.0);
@@ -904,55 +917,38 @@ constexpr int kForceWaves = F == kF64 ? 4 : (F == kF32 ? PRESTIGE_F32_FORCE_WAVE
 #endif
 constexpr uint32_t kForceCap = PRESTIGE_FORCE_CAP;
 
-// One momentum pair term: (rx, ry, rz) = sc x_ij for 0 < r < 2h; false
-// outside (then nothing is added).  sc = (pr_i + pr_j + Pi_ij) f'(q) / r with
-// Pi_ij = -alpha c0 h (v.r) / ((r^2 + eps2) rhobar) for approaching pairs,
-// put over one common denominator: a single division per pair.  ach is
+// One momentum pair: sc with a_i -= (m sigma / h) sc x_ij for 0 < r < 2h;
+// false outside (then nothing is added).  sc = (pr_i + pr_j + Pi_ij) f'(q) / r
+// with Pi_ij = -alpha c0 h (v.r) / ((r^2 + eps2) rhobar) for approaching
+// pairs, put over one common denominator: a single division per pair.  ach is
 // alpha c0 times hv.
 template <int F>
-__device__ inline bool force_pair_vec(typename SphForm<F>::real dx, typename SphForm<F>::real dy,
-                                      typename SphForm<F>::real dz, const typename SphForm<F>::G4& vi,
-                                      typename SphForm<F>::real rhoi, typename SphForm<F>::real pri,
-                                      const typename SphForm<F>::G4& vj, typename SphForm<F>::real rhoj,
-                                      typename SphForm<F>::real prj, const SphUnits& u,
-                                      typename SphForm<F>::real ach, typename SphForm<F>::real& rx,
-                                      typename SphForm<F>::real& ry, typename SphForm<F>::real& rz) {
+__device__ inline bool force_pair_sc(typename SphForm<F>::real dx, typename SphForm<F>::real dy,
+                                     typename SphForm<F>::real dz, const typename SphForm<F>::G4& vi,
+                                     typename SphForm<F>::real rhoi, typename SphForm<F>::real pri,
+                                     const typename SphForm<F>::G4& vj, typename SphForm<F>::real rhoj,
+                                     typename SphForm<F>::real prj, const SphUnits& u,
+                                     typename SphForm<F>::real ach, typename SphForm<F>::real& sc) {
   using real = typename SphForm<F>::real;
   const real H2 = (real)u.H2, inv_h = (real)u.inv_h, eps2 = (real)u.eps2;
-  real r2 = dx * dx + dy * dy + dz * dz;
+  real r2 = dot3(dx, dy, dz, dx, dy, dz);
   if (!(r2 < H2 && r2 > real(0.0))) return false;
   real rr = sqrt_pos(r2);
   real q = rr * inv_h;
   real dw = spline_df(q);
   real dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
-  real vdotr = dvx * dx + dvy * dy + dvz * dz;
+  real vdotr = dot3(dvx, dvy, dvz, dx, dy, dz);
   real num, den;
   if (vdotr < real(0.0)) {
     real drb = (r2 + eps2) * (real(0.5) * (rhoi + rhoj));
-    num = ((pri + prj) * drb - ach * vdotr) * dw;
+    num = fmad(pri + prj, drb, -(ach * vdotr)) * dw;
     den = drb * rr;
   } else {
     num = (pri + prj) * dw;
     den = rr;
   }
-  real sc = div_pos(num, den);
-  rx = sc * dx;
-  ry = sc * dy;
-  rz = sc * dz;
+  sc = div_pos(num, den);
   return true;
-}
-
-template <int F>
-__device__ inline bool force_pair(const typename SphForm<F>::G4& pi, const typename SphForm<F>::G4& vi,
-                                  typename SphForm<F>::real rhoi, typename SphForm<F>::real pri,
-                                  const typename SphForm<F>::G4& pj, const typename SphForm<F>::G4& vj,
-                                  typename SphForm<F>::real rhoj, typename SphForm<F>::real prj,
-                                  const SphUnits& u, typename SphForm<F>::real ach,
-                                  typename SphForm<F>::real& rx, typename SphForm<F>::real& ry,
-                                  typename SphForm<F>::real& rz) {
-  typename SphForm<F>::real dx, dy, dz;
-  form_pair<F>(pi, pj, u.cs_u, dx, dy, dz);
-  return force_pair_vec<F>(dx, dy, dz, vi, rhoi, pri, vj, rhoj, prj, u, ach, rx, ry, rz);
 }
 
 template <int F>
@@ -1018,21 +1014,25 @@ __global__ __launch_bounds__(kForceBlock, kForceWaves<F>) void k_sph_force(
   #pragma unroll
       for (int v = 0; v < kUnr; ++v) {
         if (jj[v] == kPad) continue;
-        real rx, ry, rz;
-        bool hit;
+        real dx, dy, dz, sc;
         if constexpr (F == kMixed) {
-          const real dx = (pi.x - pj[v].x) -
-                          (float)cell_shift(__float_as_uint(pi.w), __float_as_uint(pj[v].w), 0) * u.cs_u;
-          const real dy = (tt[v] ? by[1] : by[0]) - pj[v].y;
-          const real dz = (tt[v] ? bz[1] : bz[0]) - pj[v].z;
-          hit = force_pair_vec<F>(dx, dy, dz, vi, rhoi, pri, vj[v], rhoj[v], prj[v], u, ach, rx, ry, rz);
+          dx = (pi.x - pj[v].x) - (float)cell_shift(__float_as_uint(pi.w), __float_as_uint(pj[v].w), 0) * u.cs_u;
+          dy = (tt[v] ? by[1] : by[0]) - pj[v].y;
+          dz = (tt[v] ? bz[1] : bz[0]) - pj[v].z;
         } else {
-          hit = force_pair<F>(pi, vi, rhoi, pri, pj[v], vj[v], rhoj[v], prj[v], u, ach, rx, ry, rz);
+          form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
         }
-        if (hit) {
-          sx.add(rx);
-          sy.add(ry);
-          sz.add(rz);
+        if (force_pair_sc<F>(dx, dy, dz, vi, rhoi, pri, vj[v], rhoj[v], prj[v], u, ach, sc)) {
+          if constexpr (F == kMixed) {
+            // compensated sums of the FP32 terms
+            sx.add(sc * dx);
+            sy.add(sc * dy);
+            sz.add(sc * dz);
+          } else {
+            sx.s = fmad(sc, dx, sx.s);
+            sy.s = fmad(sc, dy, sy.s);
+            sz.s = fmad(sc, dz, sz.s);
+          }
         }
       }
     };

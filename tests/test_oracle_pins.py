@@ -57,8 +57,8 @@ def test_hydrostatic_column():
 # planted mistakes: (name, text in physics_impl.h, replacement, checks that must fail)
 MUTATIONS = [
     ("viscosity sign flipped",
-     "num = ((pri + s->pr[j]) * drb - ach * vdotr) * dw;",
-     "num = ((pri + s->pr[j]) * drb + ach * vdotr) * dw;",
+     "num = FMA(pri + s->pr[j], drb, -(ach * vdotr)) * dw;",
+     "num = FMA(pri + s->pr[j], drb, ach * vdotr) * dw;",
      ("viscous_pair", "dissipation")),
     ("mu without h",
      "const REAL ach = s->alpha_c0 * h;",
