@@ -713,14 +713,15 @@ __device__ inline double pg_sqrt(double x) {
 // selected (lanes of a wavefront straddle r = h); 1 / h and sigma depend on h
 // only and leave the pair loop.
 __device__ inline double kernel_w(double dx, double dy, double dz, double h) {
-  const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
+  // |x|^2 as the cell walk's exact test forms it, so the two share one evaluation
+  const double r2 = dx * dx + dy * dy + dz * dz;
   const double inv_h = 1.0 / h;
   const double s = 1.0 / (3.141592653589793 * h * h * h);
   const double q = r2 > 0.0 ? pg_sqrt(r2) * inv_h : 0.0;
   const double inner = __builtin_fma(q * q, __builtin_fma(0.75, q, -1.5), 1.0);
   const double t = 2.0 - q;
   const double outer = 0.25 * t * t * t;
-  return q < 1.0 ? s * inner : (q < 2.0 ? s * outer : 0.0);
+  return s * (q < 1.0 ? inner : (q < 2.0 ? outer : 0.0));
 }
 // tait: p = max(rho0 c0^2 / 7 ((rho / rho0)^7 - 1), 0)
 __device__ inline double tait(double rho, double rho0, double c0) {
@@ -740,27 +741,13 @@ struct prestige_consts {
 // neighbours per trip, applying the exact FP64 test before the bodies.  The threshold t carries the rounding margin of the FP32 form, so the
 // scan keeps a superset of the exact neighbours and the bodies see exactly the
 // pairs and the order of the plain loop.
-// Tiles of 64 slots over per-XCD queues: block b starts on range b mod 8 (the
-// XCD it runs on) and takes that range's tiles in order from a counter, then
-// the other ranges' in turn; the last block to finish clears the counters.
-// One wavefront per block.
-template <class F>
-__device__ inline void pg_for_each_tile(unsigned* q_ctr, unsigned n_tiles, unsigned per, F&& body) {
-  unsigned q = blockIdx.x & 7u;
-  for (unsigned tries = 0; tries < 8u;) {
-    unsigned t = 0;
-    if (threadIdx.x == 0u) t = atomicAdd(&q_ctr[q * 32u], 1u);
-    t = (unsigned)__builtin_amdgcn_readfirstlane((int)t);
-    if (t >= per) {
-      q = (q + 1u) & 7u;
-      ++tries;
-      continue;
-    }
-    const unsigned tile = q * per + t;
-    if (tile < n_tiles) body(tile);
-  }
-  if (threadIdx.x == 0u && atomicAdd(&q_ctr[8u * 32u], 1u) == gridDim.x - 1u)
-    for (unsigned k = 0; k <= 8u; ++k) __hip_atomic_store(&q_ctr[k * 32u], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// word = 2 word + hit in one instruction: the wavefront's hit mask is the carry-in
+// of an add with carry, each lane taking its own bit.
+__device__ inline unsigned pg_shift_in(unsigned w, bool hit) {
+  const unsigned long long m = __builtin_amdgcn_ballot_w64(hit);
+  unsigned r;
+  asm("v_addc_co_u32 %0, vcc, %1, %1, %2" : "=v"(r) : "v"(w), "s"(m) : "vcc");
+  return r;
 }
 __device__ inline unsigned pg_wave_min(unsigned v) {
   for (int d = 32; d > 0; d >>= 1) {
@@ -931,19 +918,18 @@ GenProgram build_program(const int32_t* ids, uint32_t n, int32_t loop, bool f32_
     for (auto& kv : fields) s += " " + kv.first + (kv.second.split ? "(in,out)" : "");
     s += "\n";
     if (loop == PRESTIGE_LOOP_CELLS)
-      s += "extern \"C\" __global__ void __launch_bounds__(64) " + gp.kernel +
+      // 8 wavefronts per SIMD (64 VGPRs): the pair loop is latency-bound
+      s += "extern \"C\" __global__ void __launch_bounds__(64, 8) " + gp.kernel +
            "(\n    unsigned n, const prestige_pos* __restrict__ pos, const unsigned* __restrict__ ids,\n"
            "    const unsigned* __restrict__ skeys, const unsigned* __restrict__ lb, int nx, int ny, int nz,\n"
-           "    double c2, double inv_cut, float thr, unsigned* __restrict__ pg_q, unsigned pg_tiles,\n"
-           "    unsigned pg_per, prestige_consts prestige_k" + params + ") {\n";
+           "    double c2, double inv_cut, float thr, prestige_consts prestige_k" + params + ") {\n";
     else
       s += "extern \"C\" __global__ void __launch_bounds__(256) " + gp.kernel +
            "(\n    unsigned n, const prestige_pos* __restrict__ pos, prestige_consts prestige_k" + params + ") {\n";
     s += "#pragma clang fp contract(off)\n";
     if (loop == PRESTIGE_LOOP_CELLS) {
       s += "  __shared__ float4 s_q[256 + 4];\n"
-           "  pg_for_each_tile(pg_q, pg_tiles, pg_per, [&](unsigned pg_tile) {\n"
-           "  const unsigned lane = threadIdx.x, wbase = pg_tile * 64u, i = wbase + lane;\n"
+           "  const unsigned lane = threadIdx.x, wbase = blockIdx.x * 64u, i = wbase + lane;\n"
            "  const bool live = i < n;\n"
            "  const unsigned me = live ? ids[i] : 0u;\n"
            "  const prestige_pos pi = pos[live ? i : wbase];\n"
@@ -1010,7 +996,6 @@ GenProgram build_program(const int32_t* ids, uint32_t n, int32_t loop, bool f32_
              "          unsigned word = 0u;\n"
              "          // four candidates per trip; those past cnt (stage padding, clamped slots) are masked off\n"
              "          for (unsigned b = 0u; b < cnt; b += 4u) {\n"
-             "            unsigned hb = 0u;\n"
              "#pragma unroll\n"
              "            for (unsigned u = 0u; u < 4u; ++u) {\n"
              "              const unsigned jc = j0 + c0 + 32u * w + b + u;\n"
@@ -1023,10 +1008,11 @@ GenProgram build_program(const int32_t* ids, uint32_t n, int32_t loop, bool f32_
              "                const float ddx = so.x - q.x, ddy = so.y - q.y, ddz = so.z - q.z;\n"
              "                hit = __builtin_fmaf(ddx, ddx, __builtin_fmaf(ddy, ddy, ddz * ddz)) < thr;\n"
              "              }\n"
-             "              hb |= (hit ? 1u : 0u) << u;\n"
+             "              word = pg_shift_in(word, hit);\n"
              "            }\n"
-             "            word |= hb << b;\n"
              "          }\n"
+             "          // candidate k sits at bit P - 1 - k, P the count rounded up to 4\n"
+             "          word = __builtin_bitreverse32(word) >> (32u - ((cnt + 3u) & ~3u));\n"
              "          wd[w] = cnt < 32u ? word & ((1u << cnt) - 1u) : word;\n"
              "        }\n"
              "        unsigned long long m0 = (unsigned long long)wd[0] | ((unsigned long long)wd[1] << 32);\n"
@@ -1070,7 +1056,6 @@ GenProgram build_program(const int32_t* ids, uint32_t n, int32_t loop, bool f32_
       const bool sp = pn.split.count(f) > 0;
       s += "  f_" + f + (sp ? "_out" : "") + "[me] = " + f + "_i;\n";
     }
-    if (loop == PRESTIGE_LOOP_CELLS) s += "  });\n";
     s += "}\n";
     prog.passes.push_back(gp);
   }

@@ -174,9 +174,6 @@ void run_equations(Ctx& c, const int32_t* eq_ids, uint32_t neq, int32_t loop, do
     E *= inv_cut;
     thr = (float)(1.0 + 1.0 / 1024.0 + 16.0 * E * 0x1.0p-24);
   }
-  // tile queues of the cell passes (the context's counters, clear between launches)
-  uint32_t* q_ctr = c.tile_q.as<uint32_t>();
-  uint32_t n_tiles = blocks_for(n, 64), q_per = (n_tiles + 7u) / 8u;
   for (const GenPass& gp : prog.passes) {
     hipFunction_t fn;
     PR_HIP(hipModuleGetFunction(&fn, mod, gp.kernel.c_str()));
@@ -218,26 +215,15 @@ void run_equations(Ctx& c, const int32_t* eq_ids, uint32_t neq, int32_t loop, do
       args.push_back(&c2);
       args.push_back(&inv_cut);
       args.push_back(&thr);
-      args.push_back(&q_ctr);
-      args.push_back(&n_tiles);
-      args.push_back(&q_per);
     }
     args.push_back(&k);
     for (void*& p : fptr) args.push_back(&p);
     for (void*& p : gptr) args.push_back(&p);
     {
       LaunchScope ls(&c, "generated_pass", 0.0);
-      if (loop == PRESTIGE_LOOP_CELLS) {
-        // one wavefront per block (the scan stages in LDS per wavefront), the
-        // resident wavefronts looping over 64-slot tiles from per-XCD queues
-        int per_cu = 0;
-        PR_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, 0));
-        PR_HIP(hipModuleLaunchKernel(fn, queue_grid(per_cu, c.n_cus, n_tiles), 1, 1, 64, 1, 1, 0, c.stream,
-                                     args.data(), nullptr));
-      } else {
-        PR_HIP(hipModuleLaunchKernel(fn, blocks_for(n, kBlock), 1, 1, kBlock, 1, 1, 0, c.stream, args.data(),
-                                     nullptr));
-      }
+      // cell passes: one wavefront per block (their scan stages in LDS per wavefront)
+      const int bs = loop == PRESTIGE_LOOP_CELLS ? 64 : kBlock;
+      PR_HIP(hipModuleLaunchKernel(fn, blocks_for(n, bs), 1, 1, bs, 1, 1, 0, c.stream, args.data(), nullptr));
     }
     PR_HIP(hipStreamSynchronize(c.stream));
     // the pass wrote the split fields' new values to their outputs
