@@ -256,7 +256,8 @@ void dem_step_impl(Ctx& c) {
   const int h = c.cur_hist;
   DemHist hist{c.hcnt[h].as<uint32_t>(), c.hid[h].as<uint32_t>(), c.hxi[h].p, c.wxi[h].p,
                c.hcnt[h ^ 1].as<uint32_t>(), c.hid[h ^ 1].as<uint32_t>(), c.hxi[h ^ 1].p, c.wxi[h ^ 1].p};
-  const uint32_t grid = xcd_grid(blocks_for(n));
+  // one wavefront per block: small scenes (the 10k lattice) spread over more CUs
+  const uint32_t grid = xcd_grid(blocks_for(n, 64));
   const double ge = (double)sizeof(real);
   const double lb_bytes = 4.0 * (double)c.grid.n_cells / (double)n;
   // gathered: pos 3 + r, vel 3 + m, omega 3; ids + key; state in/out (pos, vel, omega) 3 x 24 x 2;
@@ -265,7 +266,7 @@ void dem_step_impl(Ctx& c) {
   double bytes = (11.0 * ge + 8.0 + lb_bytes + 2.0 * 3.0 * 32.0 + 4.0 + hist_bytes) * n;
   LaunchScope ls(&c, F == kF64 ? "dem_force_f64" : "dem_force_mixed", bytes);
   const int dst = c.cur ^ 1, kdst = c.cur_key ^ 1;
-  k_dem_force<F><<<grid, kBlock, 0, c.stream>>>(
+  k_dem_force<F><<<grid, 64, 0, c.stream>>>(
       n, c.n_moving, gpos, gvel, gom, c.ids[c.cur_ids].as<uint32_t>(), c.keys[c.cur_key].as<uint32_t>(),
       c.lb.as<uint32_t>(), c.grid, c.dem, c.pos[c.cur].as<double4>(), c.vel[c.cur].as<double4>(),
       c.omega[c.cur].as<double4>(), c.pos[dst].as<double4>(), c.vel[dst].as<double4>(),

@@ -741,14 +741,6 @@ struct prestige_consts {
 // neighbours per trip, applying the exact FP64 test before the bodies.  The threshold t carries the rounding margin of the FP32 form, so the
 // scan keeps a superset of the exact neighbours and the bodies see exactly the
 // pairs and the order of the plain loop.
-// word = 2 word + hit in one instruction: the wavefront's hit mask is the carry-in
-// of an add with carry, each lane taking its own bit.
-__device__ inline unsigned pg_shift_in(unsigned w, bool hit) {
-  const unsigned long long m = __builtin_amdgcn_ballot_w64(hit);
-  unsigned r;
-  asm("v_addc_co_u32 %0, vcc, %1, %1, %2" : "=v"(r) : "v"(w), "s"(m) : "vcc");
-  return r;
-}
 __device__ inline unsigned pg_wave_min(unsigned v) {
   for (int d = 32; d > 0; d >>= 1) {
     const unsigned t = __shfl_xor(v, d, 64);
@@ -996,6 +988,7 @@ GenProgram build_program(const int32_t* ids, uint32_t n, int32_t loop, bool f32_
              "          unsigned word = 0u;\n"
              "          // four candidates per trip; those past cnt (stage padding, clamped slots) are masked off\n"
              "          for (unsigned b = 0u; b < cnt; b += 4u) {\n"
+             "            unsigned hb = 0u;\n"
              "#pragma unroll\n"
              "            for (unsigned u = 0u; u < 4u; ++u) {\n"
              "              const unsigned jc = j0 + c0 + 32u * w + b + u;\n"
@@ -1008,11 +1001,10 @@ GenProgram build_program(const int32_t* ids, uint32_t n, int32_t loop, bool f32_
              "                const float ddx = so.x - q.x, ddy = so.y - q.y, ddz = so.z - q.z;\n"
              "                hit = __builtin_fmaf(ddx, ddx, __builtin_fmaf(ddy, ddy, ddz * ddz)) < thr;\n"
              "              }\n"
-             "              word = pg_shift_in(word, hit);\n"
+             "              hb |= (hit ? 1u : 0u) << u;\n"
              "            }\n"
+             "            word |= hb << b;\n"
              "          }\n"
-             "          // candidate k sits at bit P - 1 - k, P the count rounded up to 4\n"
-             "          word = __builtin_bitreverse32(word) >> (32u - ((cnt + 3u) & ~3u));\n"
              "          wd[w] = cnt < 32u ? word & ((1u << cnt) - 1u) : word;\n"
              "        }\n"
              "        unsigned long long m0 = (unsigned long long)wd[0] | ((unsigned long long)wd[1] << 32);\n"
