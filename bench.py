@@ -425,6 +425,38 @@ def generated_density(pa, scene_args, reps=20):
             "launches": kt["launches"], "first_call_s": round(t_first, 3)}
 
 
+def eq1_allpairs(pa, n=10_000, reps=20):
+    """The reference's own loop, force[i] += mass[j] over every (i, j) with i outer and j inner, j == i
+    included (the generated loop and body of the reference's simple CPU backend), on the 10,000-particle
+    lattice: the device kernel (one lane per i, j in order, so each force is the sequential sum) against
+    the oracle's literal loop on one host thread, as the reference runs it.  1e8 pairs per call."""
+    import oracle
+
+    d, a = pa.scene(pa.SCENE_DEM_LATTICE, n_target=n, seed=0)
+    d.model = pa.MODEL_PAIRS
+    n = int(d.n_particles)
+    mass = np.random.default_rng(3).random(n)
+    with pa.Context(d) as ctx:
+        for k in ("x", "y", "z"):
+            ctx.upload(k, a[k])
+        ctx.upload("mass", mass)
+        ctx.upload("force", np.zeros(n))
+        ctx.eval_equation(pa.EQ1, pa.LOOP_ALLPAIRS)
+        ctx.timing_enable(True)
+        for _ in range(reps):
+            ctx.eval_equation(pa.EQ1, pa.LOOP_ALLPAIRS)
+        tm = {k["kernel"]: k for k in ctx.timing()}
+    kt = tm["eq1_allpairs"]
+    us = 1e3 * kt["total_ms"] / kt["launches"]
+    t0 = time.perf_counter()
+    oracle.eq1_allpairs(np.zeros(n), mass)
+    cpu_s = time.perf_counter() - t0
+    return {"workload": f"eq1 force[i] += mass[j], all pairs, {n} particles ({n * n:.0e} pairs per call)",
+            "value": n * n / (us * 1e-6), "unit": "pairs/s", "avg_us": round(us, 2), "launches": kt["launches"],
+            "cpu": {"value": n * n / cpu_s, "unit": "pairs/s", "cores": 1, "kind": "port",
+                    "sample": f"one call of the oracle's literal i-j loop, n = {n}"}}
+
+
 def copy_probe(nbytes):
     """hipMemcpyDtoD of nbytes (the runtime's copy kernel): reads and writes exactly nbytes of HBM."""
     import ctypes as C
@@ -522,7 +554,7 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="skip the other forms, accuracy and adjoint runs")
     ap.add_argument("--forms", default="all",
                     help="comma-separated subset of the forms to run (f64_1M, f32_1M, mixed_1M, mixed_4M, accuracy, "
-                         "dem_lattice_10k, adjoint_wcsph_1M, adjoint_dem_100k, fd_dem, fd_wcsph, generated)")
+                         "dem_lattice_10k, eq1_allpairs_10k, adjoint_wcsph_1M, adjoint_dem_100k, fd_dem, fd_wcsph, generated)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="first run a plain device-to-device copy of BYTES (a reference kernel of known "
@@ -564,6 +596,9 @@ def main():
     if run("dem_lattice_10k"):
         log("[bench] DEM lattice 10k FP64, 1000 steps")
         forms["dem_lattice_10k"] = dem_lattice(pa, 1000, 10)
+    if run("eq1_allpairs_10k"):
+        log("[bench] eq1 all pairs, 10k particles")
+        forms["eq1_allpairs_10k"] = eq1_allpairs(pa)
     if run("adjoint_wcsph_1M"):
         log("[bench] adjoint: WCSPH 1M mixed, 200 steps, K = 20")
         forms["adjoint_wcsph_1M"] = adjoint_wcsph(pa)

@@ -895,18 +895,47 @@ void gather_from_ids(Ctx& c, const double* d_in, void* dst, int dst_kind, int co
 
 // All pairs: thread t is particle id t, j runs over ids 0..n-1 in id order,
 // exactly the generated loop.  force and mass are held in id order.
+// The sum is one dependent FP64 add per pair (about 7 cycles each).  The
+// masses pass through LDS a block-wide tile at a time: the next tile is loaded
+// from global memory while the current one is summed, and a full tile is read
+// eight masses ahead of the adds.
 __global__ __launch_bounds__(kBlock) void k_eq1_allpairs(uint32_t n,
                                                          const double* __restrict__ mass,
                                                          double* __restrict__ force) {
-  __shared__ double tile[kBlock];
+  __shared__ double2 tile[kBlock / 2];
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc = i < n ? force[i] : 0.0;
+  double* tile1 = reinterpret_cast<double*>(tile);
+  double next = threadIdx.x < n ? mass[threadIdx.x] : 0.0;
   for (uint32_t j0 = 0; j0 < n; j0 += kBlock) {
-    uint32_t jj = j0 + threadIdx.x;
-    tile[threadIdx.x] = jj < n ? mass[jj] : 0.0;
+    tile1[threadIdx.x] = next;
     __syncthreads();
+    const uint32_t jn = j0 + kBlock + threadIdx.x;
+    next = jn < n ? mass[jn] : 0.0;
     uint32_t m = n - j0 < (uint32_t)kBlock ? n - j0 : (uint32_t)kBlock;
-    for (uint32_t t = 0; t < m; ++t) acc += tile[t];
+    if (m == (uint32_t)kBlock) {
+      double2 cur[4], nxt[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) cur[u] = tile[u];
+      for (uint32_t t = 4; t < (uint32_t)kBlock / 2; t += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) nxt[u] = tile[t + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          acc += cur[u].x;
+          acc += cur[u].y;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) cur[u] = nxt[u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        acc += cur[u].x;
+        acc += cur[u].y;
+      }
+    } else {
+      for (uint32_t t = 0; t < m; ++t) acc += tile1[t];
+    }
     __syncthreads();
   }
   if (i < n) force[i] = acc;

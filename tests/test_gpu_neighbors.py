@@ -140,6 +140,24 @@ def test_eq1_allpairs_bitexact():
     assert np.array_equal(oracle.eq1_allpairs(np.zeros_like(dy), dy), np.full_like(dy, dy.sum()))
 
 
+@pytest.mark.parametrize("n", [1, 15, 16, 17, 32, 33, 100])
+def test_eq1_allpairs_small_counts_bitexact(n):
+    """Counts around the kernel's 16-mass batches: below one batch, exactly one or two, and a partial
+    last batch after the double-buffered loop."""
+    rng = np.random.default_rng(n)
+    d, arrs = make_scene(pa.SCENE_DEM_LATTICE, n_target=n, seed=1)
+    assert d.n_particles == n
+    force0 = rng.standard_normal(n)
+    mass = rng.random(n)
+    with pa.Context(d) as ctx:
+        upload_all(ctx, arrs)
+        ctx.upload("force", force0)
+        ctx.upload("mass", mass)
+        ctx.eval_equation(pa.EQ1, pa.LOOP_ALLPAIRS)
+        got = ctx.download("force")
+    assert np.array_equal(got, oracle.eq1_allpairs(force0, mass))
+
+
 @pytest.mark.parametrize("cut_frac", [1.0, 0.5])
 def test_eq1_cells_bitexact(cut_frac):
     rng = np.random.default_rng(1)
