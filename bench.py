@@ -457,6 +457,56 @@ def eq1_allpairs(pa, n=10_000, reps=20):
                     "sample": f"one call of the oracle's literal i-j loop, n = {n}"}}
 
 
+def eq1_cells(pa, reps=20):
+    """eq1 over the cell iterator (j over the 27 cells of i's cell within the cutoff 2h, in row-run order)
+    on the 1M dam break: the hand-written kernel and the same body through the generator, against the
+    oracle's cell loop on the host cores."""
+    import oracle
+
+    d, a = pa.scene(pa.SCENE_DAM_BREAK, **SCENE_1M)
+    n = int(d.n_particles)
+    cutoff = d.cell_size
+    mass = np.random.default_rng(4).random(n)
+    out = {"workload": f"eq1 force[i] += mass[j] over j within 2h (cell iterator), {n} particles "
+                       "of the 1M dam break"}
+    d.model = pa.MODEL_PAIRS
+    with pa.Context(d) as ctx:
+        for k in ("x", "y", "z"):
+            ctx.upload(k, a[k])
+        ctx.upload("mass", mass)
+        ctx.upload("force", np.zeros(n))
+        ctx.neighbors_build()
+        ctx.eval_equation(pa.EQ1, pa.LOOP_CELLS, cutoff)
+        ctx.run_equations([pa.EQ1], pa.LOOP_CELLS, cutoff)   # compiles
+        ctx.timing_enable(True)
+        for _ in range(reps):
+            ctx.eval_equation(pa.EQ1, pa.LOOP_CELLS, cutoff)
+            ctx.run_equations([pa.EQ1], pa.LOOP_CELLS, cutoff)
+        tm = {k["kernel"]: k for k in ctx.timing()}
+    for name, key in (("hand_written", "eq1_cells"), ("generated", "generated_pass")):
+        k = tm[key]
+        us = 1e3 * k["total_ms"] / k["launches"]
+        out[name] = {"avg_us": round(us, 1), "particles_per_s": n / (us * 1e-6)}
+    if "generated_slot_copy" in tm:
+        k = tm["generated_slot_copy"]
+        out["generated"]["slot_copy_us"] = round(1e3 * k["total_ms"] / k["launches"], 1)
+    cores = host_cores()
+    prev = oracle.set_threads(cores)
+    try:
+        st = oracle.State(oracle.params_from_desc(d), a)
+        assert st.build() == 0
+        st.eq1_cells(cutoff, np.zeros(n), mass)
+        t0 = time.perf_counter()
+        st.eq1_cells(cutoff, np.zeros(n), mass)
+        sec = time.perf_counter() - t0
+        st.close()
+    finally:
+        oracle.set_threads(prev)
+    out["cpu"] = {"particles_per_s": n / sec, "cores": cores, "kind": "port",
+                  "sample": f"one call of the oracle's cell loop, OpenMP {cores} threads"}
+    return out
+
+
 def copy_probe(nbytes):
     """hipMemcpyDtoD of nbytes (the runtime's copy kernel): reads and writes exactly nbytes of HBM."""
     import ctypes as C
@@ -554,7 +604,7 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="skip the other forms, accuracy and adjoint runs")
     ap.add_argument("--forms", default="all",
                     help="comma-separated subset of the forms to run (f64_1M, f32_1M, mixed_1M, mixed_4M, accuracy, "
-                         "dem_lattice_10k, eq1_allpairs_10k, adjoint_wcsph_1M, adjoint_dem_100k, fd_dem, fd_wcsph, generated)")
+                         "dem_lattice_10k, eq1_allpairs_10k, eq1_cells_1M, adjoint_wcsph_1M, adjoint_dem_100k, fd_dem, fd_wcsph, generated)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="first run a plain device-to-device copy of BYTES (a reference kernel of known "
@@ -599,6 +649,9 @@ def main():
     if run("eq1_allpairs_10k"):
         log("[bench] eq1 all pairs, 10k particles")
         forms["eq1_allpairs_10k"] = eq1_allpairs(pa)
+    if run("eq1_cells_1M"):
+        log("[bench] eq1 over the cell iterator, 1M dam break")
+        forms["eq1_cells_1M"] = eq1_cells(pa)
     if run("adjoint_wcsph_1M"):
         log("[bench] adjoint: WCSPH 1M mixed, 200 steps, K = 20")
         forms["adjoint_wcsph_1M"] = adjoint_wcsph(pa)

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -375,6 +376,44 @@ inline uint32_t queue_grid(int blocks_per_cu, int n_cus, uint32_t n_tiles) {
   const uint32_t cap = resident >= 8u ? resident / 8u * 8u : 8u;
   const uint32_t need = xcd_grid(n_tiles);
   return need < cap ? need : cap;
+}
+
+// Minimum / maximum over the wavefront, returned as a wave-uniform value.
+__device__ inline uint32_t wave_min(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const uint32_t t = __shfl_xor(v, d, 64);
+    v = t < v ? t : v;
+  }
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+__device__ inline uint32_t wave_max(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const uint32_t t = __shfl_xor(v, d, 64);
+    v = t > v ? t : v;
+  }
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+// Maximum of a non-negative float over the wavefront (their bit patterns
+// order like the values).
+__device__ inline float wave_max_pos(float v) { return __uint_as_float(wave_max(__float_as_uint(v))); }
+
+// Threshold of the FP32 cell scan in units of the cutoff: 1 plus the margin
+// that keeps the scan a superset of the exact FP64 test, for offsets up to the
+// grid's extent E (in cutoffs) from a wavefront's reference point: 2^-10
+// (the test's own rounding) + 16 E 2^-24 (the offsets' rounding).
+inline float cell_scan_threshold(const Grid& g, double cutoff) {
+  double E = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double lo = g.lo[a], hi = lo + g.dims[a] * g.cs;
+    const double e = (std::fabs(lo) > std::fabs(hi) ? std::fabs(lo) : std::fabs(hi)) + (hi - lo);
+    E = e > E ? e : E;
+  }
+  E /= cutoff;
+  return (float)(1.0 + 1.0 / 1024.0 + 16.0 * E * 0x1.0p-24);
 }
 
 // body(tile) for every tile, each exactly once over the grid; one wavefront

@@ -163,17 +163,7 @@ void run_equations(Ctx& c, const int32_t* eq_ids, uint32_t neq, int32_t loop, do
   // that keeps the FP32 scan a superset of the exact FP64 test (rounding of
   // offsets up to the grid's extent E, in cutoffs, and of the test itself)
   double inv_cut = loop == PRESTIGE_LOOP_CELLS ? 1.0 / cutoff : 0.0;
-  float thr = 0.0f;
-  if (loop == PRESTIGE_LOOP_CELLS) {
-    double E = 0.0;
-    for (int a = 0; a < 3; ++a) {
-      const double lo = c.grid.lo[a], hi = lo + c.grid.dims[a] * c.grid.cs;
-      const double e = (std::fabs(lo) > std::fabs(hi) ? std::fabs(lo) : std::fabs(hi)) + (hi - lo);
-      E = e > E ? e : E;
-    }
-    E *= inv_cut;
-    thr = (float)(1.0 + 1.0 / 1024.0 + 16.0 * E * 0x1.0p-24);
-  }
+  float thr = loop == PRESTIGE_LOOP_CELLS ? cell_scan_threshold(c.grid, cutoff) : 0.0f;
   for (const GenPass& gp : prog.passes) {
     hipFunction_t fn;
     PR_HIP(hipModuleGetFunction(&fn, mod, gp.kernel.c_str()));

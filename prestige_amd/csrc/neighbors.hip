@@ -943,41 +943,138 @@ __global__ __launch_bounds__(kBlock) void k_eq1_allpairs(uint32_t n,
 
 // Cells: j over the nine row runs of slot i, kept when |x_i - x_j|^2 < cutoff^2.
 // force / mass are in id order; slot i updates force[ids[i]] only.
+//
+// One wavefront per block (64 consecutive slots).  Per row run the wavefront
+// copies the union of its lanes' candidate ranges into LDS as FP32 offsets
+// from its first slot in units of the cutoff (|q|^2 in .w), with their masses;
+// each lane scans its
+// own range with the dot form q.w - 2 s.q < t, whose threshold carries the
+// rounding margin of the FP32 form (so the scan keeps a superset of the exact
+// neighbours), then walks the set bits in ascending slot order and applies the
+// exact FP64 test: the pairs and the order of the plain loop, so each force is
+// the plain loop's sum bit for bit.  A union wider than the stage is scanned
+// from global memory.
+constexpr uint32_t kEq1Stage = 256;
+
 template <class T4>
-__global__ __launch_bounds__(kBlock) void k_eq1_cells(uint32_t n, const T4* __restrict__ pos,
-                                                      const uint32_t* __restrict__ ids,
-                                                      const uint32_t* __restrict__ skeys,
-                                                      const uint32_t* __restrict__ lb, Grid g,
-                                                      double c2, const double* __restrict__ mass,
-                                                      double* __restrict__ force) {
+__global__ __launch_bounds__(64) void k_eq1_cells(uint32_t n, const T4* __restrict__ pos,
+                                                  const uint32_t* __restrict__ ids,
+                                                  const uint32_t* __restrict__ skeys,
+                                                  const uint32_t* __restrict__ lb, Grid g, double c2,
+                                                  double inv_cut, float thr, const double* __restrict__ mass,
+                                                  double* __restrict__ force) {
 #pragma clang fp contract(off)
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  __shared__ float4 s_q[kEq1Stage + 4];   // 4 entries of padding for the scan's last trip
+  __shared__ double s_m[kEq1Stage];        // mass of the staged slots, gathered once per row run
+  const uint32_t lane = threadIdx.x, wbase = blockIdx.x * 64u, i = wbase + lane;
+  const bool live = i < n;
+  const T4 pi = pos[live ? i : wbase];
+  const T4 p0 = pos[wbase];
+  const double xr = (double)p0.x, yr = (double)p0.y, zr = (double)p0.z;
+  auto offset = [&](const T4& p) {
+    return make_float4((float)(((double)p.x - xr) * inv_cut), (float)(((double)p.y - yr) * inv_cut),
+                       (float)(((double)p.z - zr) * inv_cut), 0.0f);
+  };
+  const uint32_t me = live ? ids[i] : 0u;
+  double acc = live ? force[me] : 0.0;
   const uint32_t nx = (uint32_t)g.dims[0], ny = (uint32_t)g.dims[1], nz = (uint32_t)g.dims[2];
-  uint32_t key = skeys[i];
-  int cx = (int)(key % nx), cy = (int)((key / nx) % ny), cz = (int)(key / (nx * ny));
-  int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < (int)nx - 1 ? cx + 1 : (int)nx - 1;
-  T4 pi = pos[i];
-  const uint32_t me = ids[i];
-  double acc = force[me];
+  const uint32_t key = live ? skeys[i] : 0u;
+  const int cx = (int)(key % nx), cy = (int)((key / nx) % ny), cz = (int)(key / (nx * ny));
+  const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < (int)nx - 1 ? cx + 1 : (int)nx - 1;
+  const float4 so = offset(pi);
+  const float ax = -2.0f * so.x, ay = -2.0f * so.y, az = -2.0f * so.z;
+  const float ss = __builtin_fmaf(so.x, so.x, __builtin_fmaf(so.y, so.y, so.z * so.z));
+  const float ss_max = wave_max_pos(live ? ss : 0.0f);
   for (int dz = -1; dz <= 1; ++dz) {
-    int zz = cz + dz;
-    if (zz < 0 || zz >= (int)nz) continue;
     for (int dy = -1; dy <= 1; ++dy) {
-      int yy = cy + dy;
-      if (yy < 0 || yy >= (int)ny) continue;
-      uint32_t row = ((uint32_t)zz * ny + (uint32_t)yy) * nx;
-      uint32_t j0 = lb[row + (uint32_t)x0], j1 = lb[row + (uint32_t)x1 + 1];
-      for (uint32_t j = j0; j < j1; ++j) {
-        T4 pj = pos[j];
-        double dx = (double)pi.x - (double)pj.x, dy_ = (double)pi.y - (double)pj.y,
-               dz_ = (double)pi.z - (double)pj.z;
-        double r2 = dx * dx + dy_ * dy_ + dz_ * dz_;
-        if (r2 < c2) acc += mass[ids[j]];
+      const int zz = cz + dz, yy = cy + dy;
+      uint32_t j0 = 0u, len = 0u;
+      if (live && zz >= 0 && zz < (int)nz && yy >= 0 && yy < (int)ny) {
+        const uint32_t row = ((uint32_t)zz * ny + (uint32_t)yy) * nx;
+        j0 = lb[row + (uint32_t)x0];
+        len = lb[row + (uint32_t)x1 + 1] - j0;
       }
+      const uint32_t base = wave_min(len ? j0 : 0xFFFFFFFFu);
+      const uint32_t top = wave_max(len ? j0 + len : 0u);
+      if (top <= base) continue;
+      const bool staged = top - base <= kEq1Stage;
+      float t = 0.0f;
+      if (staged) {
+        float mq = 0.0f;
+        for (uint32_t kk = lane; kk < top - base; kk += 64u) {
+          float4 q = offset(pos[base + kk]);
+          q.w = __builtin_fmaf(q.x, q.x, __builtin_fmaf(q.y, q.y, q.z * q.z));
+          s_q[kk] = q;
+          s_m[kk] = mass[ids[base + kk]];
+          mq = q.w > mq ? q.w : mq;
+        }
+        // rounding of the dot form with |s|, |q| <= M, as in the WCSPH scan
+        const float m2 = fmaxf(ss_max, wave_max_pos(mq));
+        t = thr + 0x1.0p-19f * (m2 + __builtin_sqrtf(m2) + 1.0f) - ss;
+        __syncthreads();
+      }
+      // up to 128 candidates at a time: four scan words, then a walk of their set bits
+      for (uint32_t c0 = 0u; c0 < len; c0 += 128u) {
+        const uint32_t clen = len - c0 < 128u ? len - c0 : 128u;
+        uint32_t wd[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (uint32_t w = 0u; w < 4u; ++w) {
+          if (32u * w >= clen) break;
+          const uint32_t cnt = clen - 32u * w < 32u ? clen - 32u * w : 32u;
+          uint32_t word = 0u;
+          for (uint32_t b = 0u; b < cnt; b += 4u) {
+            uint32_t hb = 0u;
+#pragma unroll
+            for (uint32_t u = 0u; u < 4u; ++u) {
+              const uint32_t jc = j0 + c0 + 32u * w + b + u;
+              bool hit;
+              if (staged) {
+                const float4 q = s_q[jc - base];
+                hit = __builtin_fmaf(ax, q.x, __builtin_fmaf(ay, q.y, __builtin_fmaf(az, q.z, q.w))) < t;
+              } else {
+                const float4 q = offset(pos[jc < n ? jc : n - 1u]);
+                const float ddx = so.x - q.x, ddy = so.y - q.y, ddz = so.z - q.z;
+                hit = __builtin_fmaf(ddx, ddx, __builtin_fmaf(ddy, ddy, ddz * ddz)) < thr;
+              }
+              hb |= (hit ? 1u : 0u) << u;
+            }
+            word |= hb << b;
+          }
+          wd[w] = cnt < 32u ? word & ((1u << cnt) - 1u) : word;
+        }
+        uint64_t m0 = (uint64_t)wd[0] | ((uint64_t)wd[1] << 32);
+        uint64_t m1 = (uint64_t)wd[2] | ((uint64_t)wd[3] << 32);
+        auto pop = [&]() -> uint32_t {
+          uint32_t b;
+          if (m0) {
+            b = (uint32_t)__builtin_ctzll(m0);
+            m0 &= m0 - 1;
+          } else {
+            b = 64u + (uint32_t)__builtin_ctzll(m1);
+            m1 &= m1 - 1;
+          }
+          return b;
+        };
+        // two neighbours a trip, so their loads overlap; the sum still runs in slot order
+        while (m0 | m1) {
+          const uint32_t ja = j0 + c0 + pop();
+          const bool two = (m0 | m1) != 0u;
+          const uint32_t jb = two ? j0 + c0 + pop() : ja;
+          const T4 pa = pos[ja], pb = pos[jb];
+          const double ma = staged ? s_m[ja - base] : mass[ids[ja]];
+          const double mb = staged ? s_m[jb - base] : mass[ids[jb]];
+          const double ex = (double)pi.x - (double)pa.x, ey = (double)pi.y - (double)pa.y,
+                       ez = (double)pi.z - (double)pa.z;
+          const double fx = (double)pi.x - (double)pb.x, fy = (double)pi.y - (double)pb.y,
+                       fz = (double)pi.z - (double)pb.z;
+          if (ex * ex + ey * ey + ez * ez < c2) acc += ma;
+          if (two && fx * fx + fy * fy + fz * fz < c2) acc += mb;
+        }
+      }
+      if (staged) __syncthreads();
     }
   }
-  force[me] = acc;
+  if (live) force[me] = acc;
 }
 
 void eq1_eval(Ctx& c, int loop, double cutoff) {
@@ -995,14 +1092,15 @@ void eq1_eval(Ctx& c, int loop, double cutoff) {
   if (!c.sorted) neighbors_build(c);
   LaunchScope ls(&c, "eq1_cells", 56.0 * n);
   const uint32_t* ids = c.ids[c.cur_ids].as<uint32_t>();
+  const float thr = cell_scan_threshold(c.grid, cutoff);
   if (c.desc.precision == PRESTIGE_F32)
-    k_eq1_cells<float4><<<blocks_for(n), kBlock, 0, c.stream>>>(
+    k_eq1_cells<float4><<<blocks_for(n, 64), 64, 0, c.stream>>>(
         n, c.pos[c.cur].as<float4>(), ids, c.keys[c.cur_key].as<uint32_t>(), c.lb.as<uint32_t>(),
-        c.grid, cutoff * cutoff, c.mass.as<double>(), c.force.as<double>());
+        c.grid, cutoff * cutoff, 1.0 / cutoff, thr, c.mass.as<double>(), c.force.as<double>());
   else
-    k_eq1_cells<double4><<<blocks_for(n), kBlock, 0, c.stream>>>(
+    k_eq1_cells<double4><<<blocks_for(n, 64), 64, 0, c.stream>>>(
         n, c.pos[c.cur].as<double4>(), ids, c.keys[c.cur_key].as<uint32_t>(), c.lb.as<uint32_t>(),
-        c.grid, cutoff * cutoff, c.mass.as<double>(), c.force.as<double>());
+        c.grid, cutoff * cutoff, 1.0 / cutoff, thr, c.mass.as<double>(), c.force.as<double>());
   PR_HIP(hipGetLastError());
 }
 
@@ -1116,7 +1214,8 @@ static void eq1_cells_noalloc(const double* x, const double* y, const double* z,
   const uint32_t key_blocks = (uint32_t)blocks_for(n);
   k_cell_lb<<<key_blocks + kTailBlocks, kBlock, 0, st>>>(n, g.n_cells, skeys, lb, key_blocks);
   PR_HIP(hipGetLastError());
-  k_eq1_cells<double4><<<blocks_for(n), kBlock, 0, st>>>(n, spos, perm, skeys, lb, g, cutoff * cutoff, mass, force);
+  k_eq1_cells<double4><<<blocks_for(n, 64), 64, 0, st>>>(n, spos, perm, skeys, lb, g, cutoff * cutoff, 1.0 / cutoff,
+                                                          cell_scan_threshold(g, cutoff), mass, force);
   PR_HIP(hipGetLastError());
   uint32_t h[FLAG_COUNT] = {0, 0, 0, 0};
   PR_HIP(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, st));

@@ -429,29 +429,6 @@ __device__ inline uint32_t sph_run(uint32_t k) {
   return (k & 1u) ? 8u - (k >> 1) : (k >> 1);
 }
 
-// Minimum / maximum over the wavefront, returned as a wave-uniform value.
-__device__ inline uint32_t wave_min(uint32_t v) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) {
-    const uint32_t t = __shfl_xor(v, d, 64);
-    v = t < v ? t : v;
-  }
-  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
-}
-
-__device__ inline uint32_t wave_max(uint32_t v) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) {
-    const uint32_t t = __shfl_xor(v, d, 64);
-    v = t > v ? t : v;
-  }
-  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
-}
-
-// Maximum of a non-negative float over the wavefront (their bit patterns
-// order like the values).
-__device__ inline float wave_max_pos(float v) { return __uint_as_float(wave_max(__float_as_uint(v))); }
-
 // Walk of a wavefront over its lanes' stored masks (one wavefront per block;
 // every lane calls it, live or not), run groups in the WCSPH order.  For each
 // group begin(grp) runs first; then the wavefront copies the gather data of
