@@ -64,10 +64,7 @@ void keys_from_positions(Ctx& c) {
 constexpr int kRadixBits = 8;
 constexpr int kBuckets = 1 << kRadixBits;
 constexpr int kSortThreads = 256;
-#ifndef PRESTIGE_SORT_ITEMS
-#define PRESTIGE_SORT_ITEMS 8
-#endif
-constexpr int kSortItems = PRESTIGE_SORT_ITEMS;   // keys per thread and tile
+constexpr int kSortItems = 8;   // keys per thread and tile
 constexpr int kTile = kSortThreads * kSortItems;
 static_assert(kTile == (int)kSortTile, "the step kernels count moved slots per sort tile");
 static_assert(kSortThreads == kBuckets, "one thread per bucket in the scan phases");

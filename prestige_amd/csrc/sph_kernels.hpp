@@ -237,18 +237,9 @@ __device__ inline real spline_df(real q) {
 // independent loads before it uses any of them: kScanUnr candidates in the
 // density scan, kDensUnr neighbours in the density walk, kUnr neighbours in
 // the walks of the force pass and the adjoint.
-#ifndef PRESTIGE_SCAN_UNROLL
-#define PRESTIGE_SCAN_UNROLL 2
-#endif
-#ifndef PRESTIGE_DENSITY_UNROLL
-#define PRESTIGE_DENSITY_UNROLL 2
-#endif
-#ifndef PRESTIGE_UNROLL
-#define PRESTIGE_UNROLL 2
-#endif
-constexpr int kScanUnr = PRESTIGE_SCAN_UNROLL;
-constexpr int kDensUnr = PRESTIGE_DENSITY_UNROLL;
-constexpr int kUnr = PRESTIGE_UNROLL;
+constexpr int kScanUnr = 2;
+constexpr int kDensUnr = 2;
+constexpr int kUnr = 2;
 static_assert(kScanUnr == 1 || kScanUnr == 2 || kScanUnr == 4, "scan batch divides 32");
 static_assert(kDensUnr >= 1 && kUnr >= 1, "walk batches hold at least one neighbour");
 constexpr uint32_t kPad = 0xFFFFFFFFu;
@@ -562,24 +553,15 @@ constexpr int kDensBlock = 64;  // one wavefront: block barriers are wave barrie
 // compiler keep more values live, and these are the budgets that measured
 // fastest (FP64 / mixed: the occupancy before the loop; FP32: one wavefront
 // fewer than that, which avoids spills).
-#ifndef PRESTIGE_F32_DENS_WAVES
-#define PRESTIGE_F32_DENS_WAVES 7
-#endif
-#ifndef PRESTIGE_F32_FORCE_WAVES
-#define PRESTIGE_F32_FORCE_WAVES 5
-#endif
-#ifndef PRESTIGE_MIXED_FORCE_WAVES
-#define PRESTIGE_MIXED_FORCE_WAVES 5
-#endif
-#ifndef PRESTIGE_DENS_WAVES
-#define PRESTIGE_DENS_WAVES 8
-#endif
 template <int F>
-constexpr int kDensWaves = F == kF32 ? PRESTIGE_F32_DENS_WAVES : PRESTIGE_DENS_WAVES;
-#ifndef PRESTIGE_STAGE_CAP
-#define PRESTIGE_STAGE_CAP 256
-#endif
-constexpr uint32_t kStageCap = PRESTIGE_STAGE_CAP;
+constexpr int kDensWaves = F == kF32 ? 7 : 8;
+constexpr uint32_t kStageCap = 256;   // scan offsets staged per run (4 KB)
+// Row runs per lockstep walk of the density pass: opposite rows together in
+// the FP64 form (1106 against 1147 us a pass at 4M); one at a time in the
+// FP32 form (no gain measured) and the mixed form, whose per-run probe
+// shifts would otherwise spill.
+template <int F>
+constexpr uint32_t kDensGroup = F == kF64 ? 2u : 1u;
 
 // Tait pressure p = max(B ((rho / rho0)^7 - 1), 0), one operation order for
 // every caller.
@@ -710,114 +692,139 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
     Sum<real, F == kMixed> acc;
     bool overflow = false;
     const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
-    // runs in the WCSPH order, each scanned and walked before the next (a
-    // group's masks held together would cost registers, and this pass's walk
-    // reads global memory, which wants occupancy; the two-level loop also
-    // compiles to fewer registers than one loop of nine)
-    for (uint32_t kz = 0; kz < 3u; ++kz)
-    for (uint32_t ky = 0; ky < 3u; ++ky) {
-      const uint32_t r = sph_run(3u * kz + ky);
-      const int zz = cr.cz + (int)(r / 3u) - 1, yy = cr.cy + (int)(r % 3u) - 1;
-      const bool row_ok = live && zz >= 0 && zz < nz && yy >= 0 && yy < ny;
-      uint32_t j0 = 0, len = 0;
+    float bxm = 0.0f, bx0 = 0.0f, bxp = 0.0f;
+    if constexpr (F == kMixed) {
+      bx0 = (float)pi.x;
+      bxm = bx0 + u.cs_u;
+      bxp = bx0 - u.cs_u;
+    }
+    // Runs in the WCSPH order, scanned one at a time and walked G at a time
+    // (kDensGroup): with G = 2 a run and its opposite row ({0, 8}, {1, 7},
+    // {2, 6}, {3, 5}, {4}) as one stream of set bits, so a lane with few hits
+    // in one row and many in the other idles less of the lockstep walk.  The
+    // order of the pairs stays the scan order.
+    constexpr uint32_t G = kDensGroup<F>;
+    for (uint32_t grp = 0; grp < (G == 2u ? kGroups : kRuns); ++grp) {
+      const uint32_t nrun = G == 2u && grp + 1u < kGroups ? 2u : 1u;
+      uint64_t m[2 * G];
+      uint32_t gj0[G];
       // mixed form: the run's candidates lie in row (dy, dz) of cells cx - 1,
       // cx, cx + 1; slots [b0, b1) are cell cx, so a candidate's cell shift
-      // and with it the probe pair vector come from its slot (pair_base)
-      uint32_t b0 = 0, b1 = 0;
-      if (row_ok) {
-        const uint32_t row = ((uint32_t)zz * (uint32_t)ny + (uint32_t)yy) * (uint32_t)nx;
-        j0 = lb[row + (uint32_t)cr.x0];
-        len = lb[row + (uint32_t)cr.x1 + 1] - j0;
-        if constexpr (F == kMixed) {
-          b0 = lb[row + (uint32_t)cr.cx];
-          b1 = lb[row + (uint32_t)cr.cx + 1];
-        }
-        if (len > 128u) {
-          overflow = true;
-          len = 128u;
-        }
+      // and with it the probe pair vector come from its slot
+      uint32_t gb0[G], gb1[G];
+      float gby[G], gbz[G];
+#pragma unroll
+      for (uint32_t t = 0; t < G; ++t) {
+        m[2u * t] = m[2u * t + 1u] = 0u;
+        gj0[t] = gb0[t] = gb1[t] = 0u;
+        gby[t] = gbz[t] = 0.0f;
       }
-      // probe pair vector (o_i - o_j) - (c_j - c_i) cs/h as (base - o_j): the
-      // y and z shifts are the run's, x is -1, 0, +1 by slot
-      float bxm = 0.0f, bx0 = 0.0f, bxp = 0.0f, by = 0.0f, bz = 0.0f;
-      if constexpr (F == kMixed) {
-        bx0 = (float)pi.x;
-        bxm = bx0 + u.cs_u;
-        bxp = bx0 - u.cs_u;
-        by = (float)pi.y - (float)((int)(r % 3u) - 1) * u.cs_u;
-        bz = (float)pi.z - (float)((int)(r / 3u) - 1) * u.cs_u;
-      }
-      if (live) meta[meta_index(r, n, i)] = make_uint2(j0, len);
-      // union of the wavefront's candidate ranges of this run
-      const uint32_t base = wave_min(len ? j0 : 0xFFFFFFFFu);
-      const uint32_t top = wave_max(len ? j0 + len : 0u);
-      if (top <= base) continue;
-      const bool staged = top - base <= kStageCap;
-      // scan offset of candidate j in this wavefront's frame
-      const int rdy = (int)(r % 3u) - 1, rdz = (int)(r / 3u) - 1;
-      const uint32_t rowbase = (row_ref + (uint32_t)(rdz * g.dims[1] + rdy)) * (uint32_t)nx;
-      const float ofy = (float)rdy * u.cs_u, ofz = (float)rdz * u.cs_u;
-      auto cand = [&](uint32_t j) {
-        return probe_frame ? probe_offset(j, rowbase, ofy, ofz) : offset(spos[j]);
-      };
-      float t = 0.0f;
-      if (staged) {
-        float mq = 0.0f;
-        for (uint32_t kk = lane; kk < top - base; kk += kDensBlock) {
-          float4 q = cand(base + kk);
-          q.w = __builtin_fmaf(q.x, q.x, __builtin_fmaf(q.y, q.y, q.z * q.z));
-          s_q[kk] = q;
-          mq = q.w > mq ? q.w : mq;
-        }
-        // Rounding of the dot form with |s|, |q| <= M: |q|^2 (3 roundings),
-        // the three FMAs (partial sums up to 3 M^2), |s|^2 and the threshold,
-        // and the FP32 offsets themselves (2 u M per component on a pair vector
-        // of length < 2) stay below u (22 M^2 + 14 M + 5); the margin is
-        // 32 u (M^2 + M + 1), u = 2^-24, on top of the common probe margin.
-        const float m2 = fmaxf(ss_max, wave_max_pos(mq));
-        const float h2p = u.H2_probe + 0x1.0p-19f * (m2 + __builtin_sqrtf(m2) + 1.0f);
-        t = h2p - ss;
-        __syncthreads();
-      }
-      if (len) {
-        uint64_t m0, m1;
-        if (staged) {
-          scan_run([&](uint32_t k0, uint32_t cnt) {
-                     return scan_word_dot([&](uint32_t j) { return s_q[j - base]; }, k0, cnt, ax, ay, az, t);
-                   },
-                   j0, len, r, words, n, i, m0, m1);
-        } else {
-          scan_run([&](uint32_t k0, uint32_t cnt) {
-                     return scan_word([&](uint32_t j) { return cand(j < n ? j : n - 1); }, k0, cnt, sp);
-                   },
-                   j0, len, r, words, n, i, m0, m1);
-        }
-        walk_bits<kDensUnr>(m0, m1, [&](const uint32_t* jj) {
-          G4 pj[kDensUnr];
-  #pragma unroll
-          for (int v = 0; v < kDensUnr; ++v) pj[v] = pos[j0 + (jj[v] == kPad ? jj[0] : jj[v])];
-  #pragma unroll
-          for (int v = 0; v < kDensUnr; ++v) {
-            if (jj[v] == kPad) continue;
-            real dx, dy, dz;
-            if constexpr (F == kMixed) {
-              const uint32_t js = j0 + jj[v];
-              dx = (js < b0 ? bxm : (js < b1 ? bx0 : bxp)) - pj[v].x;
-              dy = by - pj[v].y;
-              dz = bz - pj[v].z;
-            } else {
-              form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
-            }
-            real r2 = dot3(dx, dy, dz, dx, dy, dz);
-            if (r2 < H2) {
-              // the self pair (and coincident slots) has r2 = 0, q = 0
-              real q = r2 > real(0.0) ? sqrt_pos(r2) * inv_h : real(0.0);
-              acc.add(spline_f(q));
-            }
+#pragma unroll
+      for (uint32_t t = 0; t < G; ++t) {
+        if (t >= nrun) break;
+        const uint32_t r = sph_run(G * grp + t);
+        const int zz = cr.cz + (int)(r / 3u) - 1, yy = cr.cy + (int)(r % 3u) - 1;
+        const bool row_ok = live && zz >= 0 && zz < nz && yy >= 0 && yy < ny;
+        uint32_t j0 = 0, len = 0;
+        if (row_ok) {
+          const uint32_t row = ((uint32_t)zz * (uint32_t)ny + (uint32_t)yy) * (uint32_t)nx;
+          j0 = lb[row + (uint32_t)cr.x0];
+          len = lb[row + (uint32_t)cr.x1 + 1] - j0;
+          if constexpr (F == kMixed) {
+            gb0[t] = lb[row + (uint32_t)cr.cx];
+            gb1[t] = lb[row + (uint32_t)cr.cx + 1];
           }
-        });
+          if (len > 128u) {
+            overflow = true;
+            len = 128u;
+          }
+        }
+        gj0[t] = j0;
+        // probe pair vector (o_i - o_j) - (c_j - c_i) cs/h as (base - o_j): the
+        // y and z shifts are the run's, x is -1, 0, +1 by slot
+        if constexpr (F == kMixed) {
+          gby[t] = (float)pi.y - (float)((int)(r % 3u) - 1) * u.cs_u;
+          gbz[t] = (float)pi.z - (float)((int)(r / 3u) - 1) * u.cs_u;
+        }
+        if (live) meta[meta_index(r, n, i)] = make_uint2(j0, len);
+        // union of the wavefront's candidate ranges of this run
+        const uint32_t base = wave_min(len ? j0 : 0xFFFFFFFFu);
+        const uint32_t top = wave_max(len ? j0 + len : 0u);
+        if (top <= base) continue;
+        const bool staged = top - base <= kStageCap;
+        // scan offset of candidate j in this wavefront's frame
+        const int rdy = (int)(r % 3u) - 1, rdz = (int)(r / 3u) - 1;
+        const uint32_t rowbase = (row_ref + (uint32_t)(rdz * g.dims[1] + rdy)) * (uint32_t)nx;
+        const float ofy = (float)rdy * u.cs_u, ofz = (float)rdz * u.cs_u;
+        auto cand = [&](uint32_t j) {
+          return probe_frame ? probe_offset(j, rowbase, ofy, ofz) : offset(spos[j]);
+        };
+        float thr = 0.0f;
+        if (staged) {
+          float mq = 0.0f;
+          for (uint32_t kk = lane; kk < top - base; kk += kDensBlock) {
+            float4 q = cand(base + kk);
+            q.w = __builtin_fmaf(q.x, q.x, __builtin_fmaf(q.y, q.y, q.z * q.z));
+            s_q[kk] = q;
+            mq = q.w > mq ? q.w : mq;
+          }
+          // Rounding of the dot form with |s|, |q| <= M: |q|^2 (3 roundings),
+          // the three FMAs (partial sums up to 3 M^2), |s|^2 and the threshold,
+          // and the FP32 offsets themselves (2 u M per component on a pair vector
+          // of length < 2) stay below u (22 M^2 + 14 M + 5); the margin is
+          // 32 u (M^2 + M + 1), u = 2^-24, on top of the common probe margin.
+          const float m2 = fmaxf(ss_max, wave_max_pos(mq));
+          const float h2p = u.H2_probe + 0x1.0p-19f * (m2 + __builtin_sqrtf(m2) + 1.0f);
+          thr = h2p - ss;
+          __syncthreads();
+        }
+        if (len) {
+          if (staged) {
+            scan_run([&](uint32_t k0, uint32_t cnt) {
+                       return scan_word_dot([&](uint32_t j) { return s_q[j - base]; }, k0, cnt, ax, ay, az, thr);
+                     },
+                     j0, len, r, words, n, i, m[2u * t], m[2u * t + 1u]);
+          } else {
+            scan_run([&](uint32_t k0, uint32_t cnt) {
+                       return scan_word([&](uint32_t j) { return cand(j < n ? j : n - 1); }, k0, cnt, sp);
+                     },
+                     j0, len, r, words, n, i, m[2u * t], m[2u * t + 1u]);
+          }
+        }
+        if (staged) __syncthreads();  // the next run's staging overwrites s_q
       }
-      if (staged) __syncthreads();  // the next run's staging overwrites s_q
+      // bit q of the group stream: run q / 128, slot gj0[run] + q % 128
+      walk_bits_w<kDensUnr>(m, [&](const uint32_t* jj) {
+        uint32_t js[kDensUnr], tt[kDensUnr];
+#pragma unroll
+        for (int v = 0; v < kDensUnr; ++v) {
+          const uint32_t q = jj[v] == kPad ? jj[0] : jj[v];
+          tt[v] = G == 2u ? q >> 7 : 0u;
+          js[v] = (tt[v] ? gj0[G - 1u] : gj0[0]) + (q & 127u);
+        }
+        G4 pj[kDensUnr];
+#pragma unroll
+        for (int v = 0; v < kDensUnr; ++v) pj[v] = pos[js[v]];
+#pragma unroll
+        for (int v = 0; v < kDensUnr; ++v) {
+          if (jj[v] == kPad) continue;
+          real dx, dy, dz;
+          if constexpr (F == kMixed) {
+            const uint32_t b0 = tt[v] ? gb0[G - 1u] : gb0[0], b1 = tt[v] ? gb1[G - 1u] : gb1[0];
+            dx = (js[v] < b0 ? bxm : (js[v] < b1 ? bx0 : bxp)) - pj[v].x;
+            dy = (tt[v] ? gby[G - 1u] : gby[0]) - pj[v].y;
+            dz = (tt[v] ? gbz[G - 1u] : gbz[0]) - pj[v].z;
+          } else {
+            form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
+          }
+          real r2 = dot3(dx, dy, dz, dx, dy, dz);
+          if (r2 < H2) {
+            // the self pair (and coincident slots) has r2 = 0, q = 0
+            real q = r2 > real(0.0) ? sqrt_pos(r2) * inv_h : real(0.0);
+            acc.add(spline_f(q));
+          }
+        }
+      });
     }
     if (!live) return;
     if (overflow) atomicOr(&flags[FLAG_CAPACITY], 1u);
@@ -888,11 +895,8 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
 // ---------------------------------------------------------------------------
 constexpr int kForceBlock = 64;  // one wavefront: block barriers are wave barriers
 template <int F>
-constexpr int kForceWaves = F == kF64 ? 4 : (F == kF32 ? PRESTIGE_F32_FORCE_WAVES : PRESTIGE_MIXED_FORCE_WAVES);
-#ifndef PRESTIGE_FORCE_CAP
-#define PRESTIGE_FORCE_CAP 160
-#endif
-constexpr uint32_t kForceCap = PRESTIGE_FORCE_CAP;
+constexpr int kForceWaves = F == kF64 ? 4 : 5;
+constexpr uint32_t kForceCap = 160;
 
 // One momentum pair: sc with a_i -= (m sigma / h) sc x_ij for 0 < r < 2h;
 // false outside (then nothing is added).  sc = (pr_i + pr_j + Pi_ij) f'(q) / r
@@ -1186,11 +1190,7 @@ void sph_density_launch(Ctx& c, const SphStepIO<F>& io, typename SphForm<F>::R2*
   double per = (F == kMixed ? 12.0 : 3.0 * ge) + 3.0 * ge;
   LaunchScope ls(&c, F == kF64 ? "sph_density_f64" : (F == kF32 ? "sph_density_f32" : "sph_density_mixed"),
                  per * n);
-#ifdef PRESTIGE_DENS_PER_CU   // resident blocks per CU fixed at build time (occupancy sweeps)
-  static const int per_cu = PRESTIGE_DENS_PER_CU;
-#else
   static const int per_cu = blocks_per_cu(k_sph_density<F>, kDensBlock);
-#endif
   const uint32_t dgrid = queue_grid(per_cu, c.n_cus, io.tq.n_tiles);
   k_sph_density<F><<<dgrid, kDensBlock, 0, c.stream>>>(n, c.pos[c.cur].as<S4>(), io.gpos, io.skeys,
                                                       c.lb.as<uint32_t>(), c.grid, c.sph, io.u, c.nmeta.as<uint2>(),
@@ -1236,11 +1236,7 @@ void sph_step_impl(Ctx& c) {
     LaunchScope ls(&c, F == kF64 ? "sph_force_f64" : (F == kF32 ? "sph_force_f32" : "sph_force_mixed"), per * nm);
     const int dst = c.cur ^ 1;
     const int kdst = c.cur_key ^ 1;
-#ifdef PRESTIGE_FORCE_PER_CU   // as PRESTIGE_DENS_PER_CU, for the force pass
-    static const int per_cu = PRESTIGE_FORCE_PER_CU;
-#else
     static const int per_cu = blocks_per_cu(k_sph_force<F>, kForceBlock);
-#endif
     const uint32_t fgrid = queue_grid(per_cu, c.n_cus, tq.n_tiles);
     k_sph_force<F><<<fgrid, kForceBlock, 0, c.stream>>>(
         n, c.n_moving, gpos, gvel, c.rp.as<R2>(), c.ids[c.cur_ids].as<uint32_t>(), skeys,
