@@ -361,48 +361,41 @@ __device__ inline void load_run(const uint2* __restrict__ meta, const uint32_t* 
   m1 = (uint64_t)wd[2] | ((uint64_t)wd[3] << 32);
 }
 
-// Visit the set bits of the W-word mask m (bit b of word w is index 64 w + b)
-// in ascending order, B at a time: fn(jj) receives bit indices, kPad for
-// unused batch entries.
+// Visit the set bits of the W-word mask m in ascending word order, B at a
+// time: fn(jj) receives wb[w] + b for bit b of word w, kPad for unused batch
+// entries.  The nonzero words are first moved to the front (order kept), and
+// every bit is then taken from the front word, the queue moving up one word
+// when it empties: a pop costs the same few instructions whichever word it
+// comes from.
 template <int B, int W, class Fn>
-__device__ inline void walk_bits_w(uint64_t (&m)[W], Fn&& fn) {
-  auto any = [&] {
-    uint64_t o = 0;
+__device__ inline void walk_bits_q(uint64_t (&m)[W], uint32_t (&wb)[W], Fn&& fn) {
 #pragma unroll
-    for (int w = 0; w < W; ++w) o |= m[w];
-    return o != 0;
-  };
-  while (any()) {
+  for (int p = 0; p < W - 1; ++p) {
+#pragma unroll
+    for (int k = 0; k < W - 1 - p; ++k) {
+      if (m[k] == 0u) {
+        m[k] = m[k + 1];
+        wb[k] = wb[k + 1];
+        m[k + 1] = 0u;
+      }
+    }
+  }
+  while (m[0]) {
     uint32_t jj[B];
 #pragma unroll
     for (int u = 0; u < B; ++u) {
       jj[u] = kPad;
+      if (m[0]) {
+        jj[u] = wb[0] + (uint32_t)__builtin_ctzll(m[0]);
+        m[0] &= m[0] - 1u;
+        if (m[0] == 0u) {
 #pragma unroll
-      for (int w = 0; w < W; ++w) {
-        if (jj[u] == kPad && m[w]) {
-          jj[u] = 64u * (uint32_t)w + (uint32_t)__builtin_ctzll(m[w]);
-          m[w] &= m[w] - 1;
+          for (int k = 0; k < W - 1; ++k) {
+            m[k] = m[k + 1];
+            wb[k] = wb[k + 1];
+          }
+          m[W - 1] = 0u;
         }
-      }
-    }
-    fn(jj);
-  }
-}
-
-template <int B, class Fn>
-__device__ inline void walk_bits(uint64_t m0, uint64_t m1, Fn&& fn) {
-  while (m0 | m1) {
-    uint32_t jj[B];
-#pragma unroll
-    for (int u = 0; u < B; ++u) {
-      if (m0) {
-        jj[u] = (uint32_t)__builtin_ctzll(m0);
-        m0 &= m0 - 1;
-      } else if (m1) {
-        jj[u] = 64u + (uint32_t)__builtin_ctzll(m1);
-        m1 &= m1 - 1;
-      } else {
-        jj[u] = kPad;
       }
     }
     fn(jj);
@@ -464,14 +457,14 @@ __device__ inline void walk_runs_staged(const uint2* __restrict__ meta, const ui
       for (uint32_t k = lane; k < total; k += 64u)
         stage(k, k < span[0] ? base[0] + k : base[1] + (k - span[0]));
       __syncthreads();
-      walk_bits_w<B>(m, [&](const uint32_t* jb) {
-        uint32_t jj[B], tt[B];
+      // stage entry of bit 0 of each mask word
+      uint32_t wb[4] = {j0[0] - base[0], j0[0] - base[0] + 64u, span[0] + j0[1] - base[1],
+                        span[0] + j0[1] - base[1] + 64u};
+      const uint32_t s0 = span[0];
+      walk_bits_q<B>(m, wb, [&](const uint32_t* jj) {
+        uint32_t tt[B];
 #pragma unroll
-        for (int v = 0; v < B; ++v) {
-          const uint32_t q = jb[v];
-          tt[v] = q != kPad && q >= 128u ? 1u : 0u;
-          jj[v] = q == kPad ? kPad : (q < 128u ? j0[0] + q - base[0] : span[0] + j0[1] + (q - 128u) - base[1]);
-        }
+        for (int v = 0; v < B; ++v) tt[v] = jj[v] != kPad && jj[v] >= s0 ? 1u : 0u;
         lds(jj, tt);
       });
       __syncthreads();  // the next group's staging overwrites the stage
@@ -482,23 +475,16 @@ __device__ inline void walk_runs_staged(const uint2* __restrict__ meta, const ui
         uint32_t tt[B];
 #pragma unroll
         for (int v = 0; v < B; ++v) tt[v] = t;
+        uint64_t mm[2] = {m[2u * t], m[2u * t + 1u]};
         if (span[t] <= CAP) {
           for (uint32_t k = lane; k < span[t]; k += 64u) stage(k, base[t] + k);
           __syncthreads();
-          walk_bits<B>(m[2u * t], m[2u * t + 1u], [&](const uint32_t* jb) {
-            uint32_t jj[B];
-#pragma unroll
-            for (int v = 0; v < B; ++v) jj[v] = jb[v] == kPad ? kPad : j0[t] + jb[v] - base[t];
-            lds(jj, tt);
-          });
+          uint32_t wb[2] = {j0[t] - base[t], j0[t] - base[t] + 64u};
+          walk_bits_q<B>(mm, wb, [&](const uint32_t* jj) { lds(jj, tt); });
           __syncthreads();
         } else {
-          walk_bits<B>(m[2u * t], m[2u * t + 1u], [&](const uint32_t* jb) {
-            uint32_t jj[B];
-#pragma unroll
-            for (int v = 0; v < B; ++v) jj[v] = jb[v] == kPad ? kPad : j0[t] + jb[v];
-            glob(jj, tt);
-          });
+          uint32_t wb[2] = {j0[t], j0[t] + 64u};
+          walk_bits_q<B>(mm, wb, [&](const uint32_t* jj) { glob(jj, tt); });
         }
       }
     }
@@ -793,15 +779,17 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
         }
         if (staged) __syncthreads();  // the next run's staging overwrites s_q
       }
-      // bit q of the group stream: run q / 128, slot gj0[run] + q % 128
-      walk_bits_w<kDensUnr>(m, [&](const uint32_t* jj) {
-        uint32_t js[kDensUnr], tt[kDensUnr];
+      // the group's words in stream order, each with the slot of its bit 0
+      uint32_t wb[2 * G];
 #pragma unroll
-        for (int v = 0; v < kDensUnr; ++v) {
-          const uint32_t q = jj[v] == kPad ? jj[0] : jj[v];
-          tt[v] = G == 2u ? q >> 7 : 0u;
-          js[v] = (tt[v] ? gj0[G - 1u] : gj0[0]) + (q & 127u);
-        }
+      for (uint32_t t = 0; t < G; ++t) {
+        wb[2u * t] = gj0[t];
+        wb[2u * t + 1u] = gj0[t] + 64u;
+      }
+      walk_bits_q<kDensUnr>(m, wb, [&](const uint32_t* jj) {
+        uint32_t js[kDensUnr];
+#pragma unroll
+        for (int v = 0; v < kDensUnr; ++v) js[v] = jj[v] == kPad ? jj[0] : jj[v];
         G4 pj[kDensUnr];
 #pragma unroll
         for (int v = 0; v < kDensUnr; ++v) pj[v] = pos[js[v]];
@@ -810,10 +798,10 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
           if (jj[v] == kPad) continue;
           real dx, dy, dz;
           if constexpr (F == kMixed) {
-            const uint32_t b0 = tt[v] ? gb0[G - 1u] : gb0[0], b1 = tt[v] ? gb1[G - 1u] : gb1[0];
-            dx = (js[v] < b0 ? bxm : (js[v] < b1 ? bx0 : bxp)) - pj[v].x;
-            dy = (tt[v] ? gby[G - 1u] : gby[0]) - pj[v].y;
-            dz = (tt[v] ? gbz[G - 1u] : gbz[0]) - pj[v].z;
+            static_assert(F != kMixed || G == 1u, "the mixed form walks one run at a time");
+            dx = (js[v] < gb0[0] ? bxm : (js[v] < gb1[0] ? bx0 : bxp)) - pj[v].x;
+            dy = gby[0] - pj[v].y;
+            dz = gbz[0] - pj[v].z;
           } else {
             form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
           }
