@@ -182,8 +182,9 @@ __global__ __launch_bounds__(kAdjBlock, kAdjForceWaves<F>) void k_adj_force(
         pb += (double)qpb; rb += (double)qrb; sa += (double)qsa;
         qxx = qxy = qxz = qvx = qvy = qvz = qpb = qrb = qsa = ar(0.0);
       };
+      ProbeRunBases pb_;   // mixed form: the forward force pass's pair vector
       // pair rows over a batch of neighbours fetched by get(j, p, v, rp, c)
-      auto body = [&](const uint32_t* jj, auto&& get) {
+      auto body = [&](const uint32_t* jj, const uint32_t* tt, auto&& get) {
         G4 pj[kUnr], vj[kUnr];
         R2 rpj[kUnr];
         C4 cj[kUnr];
@@ -193,7 +194,8 @@ __global__ __launch_bounds__(kAdjBlock, kAdjForceWaves<F>) void k_adj_force(
         for (int v = 0; v < kUnr; ++v) {
           if (jj[v] == kPad) continue;
           real dxr, dyr, dzr;
-          form_pair<F>(pi, pj[v], u.cs_u, dxr, dyr, dzr);
+          if constexpr (F == kMixed) pb_.pair(pi, pj[v], tt[v], u.cs_u, dxr, dyr, dzr);
+          else form_pair<F>(pi, pj[v], u.cs_u, dxr, dyr, dzr);
           real r2r = dot3(dxr, dyr, dzr, dxr, dyr, dzr);
           if (!(r2r < H2 && r2r > real(0.0))) continue;
           const ar dx = (ar)dxr, dy = (ar)dyr, dz = (ar)dzr, r2 = (ar)r2r;
@@ -239,23 +241,26 @@ __global__ __launch_bounds__(kAdjBlock, kAdjForceWaves<F>) void k_adj_force(
         }
       };
       walk_runs_staged<kUnr, kForceCap>(
-          rec.meta, rec.words, n, i, live, lane, [](uint32_t) {},
+          rec.meta, rec.words, n, i, live, lane,
+          [&](uint32_t grp) {
+            if constexpr (F == kMixed) pb_.begin(pi, grp, u.cs_u);
+          },
           [&](uint32_t k, uint32_t j) {
             s_p[k] = gpos[j];
             s_v[k] = gvel[j];
             s_rp[k] = rp[j];
             s_c[k] = cvec[j];
           },
-          [&](const uint32_t* jj, const uint32_t*) {
-            body(jj, [&](uint32_t k, G4& p, G4& v, R2& r, C4& cc) {
+          [&](const uint32_t* jj, const uint32_t* tt) {
+            body(jj, tt, [&](uint32_t k, G4& p, G4& v, R2& r, C4& cc) {
               p = s_p[k];
               v = s_v[k];
               r = s_rp[k];
               cc = s_c[k];
             });
           },
-          [&](const uint32_t* jj, const uint32_t*) {
-            body(jj, [&](uint32_t j, G4& p, G4& v, R2& r, C4& cc) {
+          [&](const uint32_t* jj, const uint32_t* tt) {
+            body(jj, tt, [&](uint32_t j, G4& p, G4& v, R2& r, C4& cc) {
               p = gpos[j];
               v = gvel[j];
               r = rp[j];
@@ -322,7 +327,8 @@ __global__ __launch_bounds__(kAdjBlock, kAdjDensityWaves<F>) void k_adj_density(
     const ar inv_h = (ar)u.inv_h;
     double sx = 0, sy = 0, sz = 0;
     ar qx = 0, qy = 0, qz = 0;  // per row run, then FP64
-    auto body = [&](const uint32_t* jj, auto&& get) {
+    ProbeRunBases pb_;   // mixed form: the forward force pass's pair vector
+    auto body = [&](const uint32_t* jj, const uint32_t* tt, auto&& get) {
       G4 pj[kUnr];
       ar rbj[kUnr];
   #pragma unroll
@@ -331,7 +337,8 @@ __global__ __launch_bounds__(kAdjBlock, kAdjDensityWaves<F>) void k_adj_density(
       for (int v = 0; v < kUnr; ++v) {
         if (jj[v] == kPad) continue;
         real dxr, dyr, dzr;
-        form_pair<F>(pi, pj[v], u.cs_u, dxr, dyr, dzr);
+        if constexpr (F == kMixed) pb_.pair(pi, pj[v], tt[v], u.cs_u, dxr, dyr, dzr);
+        else form_pair<F>(pi, pj[v], u.cs_u, dxr, dyr, dzr);
         real r2r = dot3(dxr, dyr, dzr, dxr, dyr, dzr);
         if (!(r2r < H2 && r2r > real(0.0))) continue;
         ar rr, irr;
@@ -343,19 +350,22 @@ __global__ __launch_bounds__(kAdjBlock, kAdjDensityWaves<F>) void k_adj_density(
       }
     };
     walk_runs_staged<kUnr, kForceCap>(
-        rec.meta, rec.words, n, i, live, lane, [](uint32_t) {},
+        rec.meta, rec.words, n, i, live, lane,
+        [&](uint32_t grp) {
+          if constexpr (F == kMixed) pb_.begin(pi, grp, u.cs_u);
+        },
         [&](uint32_t k, uint32_t j) {
           s_p[k] = gpos[j];
           s_rb[k] = rhob[j];
         },
-        [&](const uint32_t* jj, const uint32_t*) {
-          body(jj, [&](uint32_t k, G4& p, ar& rb) {
+        [&](const uint32_t* jj, const uint32_t* tt) {
+          body(jj, tt, [&](uint32_t k, G4& p, ar& rb) {
             p = s_p[k];
             rb = s_rb[k];
           });
         },
-        [&](const uint32_t* jj, const uint32_t*) {
-          body(jj, [&](uint32_t j, G4& p, ar& rb) {
+        [&](const uint32_t* jj, const uint32_t* tt) {
+          body(jj, tt, [&](uint32_t j, G4& p, ar& rb) {
             p = gpos[j];
             rb = rhob[j];
           });

@@ -190,6 +190,22 @@ __device__ inline void probe_pair(const float4& qi, const float4& qj, float cs_u
   dz = (qi.z - qj.z) - (float)cell_shift(wi, wj, 2) * cs_u;
 }
 
+// Mixed-form pair vector in the walks over stored row masks: (o_i - o_j) -
+// (c_j - c_i) cs/h with the y and z cell shifts of a run group's two runs
+// folded into per-run bases (by, bz; set by begin at each group) and only the
+// x shift read from the cell codes.  The forward force pass and the reverse
+// passes form it the same way.
+struct ProbeRunBases {
+  float by[2] = {0.0f, 0.0f}, bz[2] = {0.0f, 0.0f};
+  __device__ void begin(const float4& pi, uint32_t grp, float cs_u);
+  __device__ void pair(const float4& pi, const float4& pj, uint32_t t, float cs_u, float& dx, float& dy,
+                       float& dz) const {
+    dx = (pi.x - pj.x) - (float)cell_shift(__float_as_uint(pi.w), __float_as_uint(pj.w), 0) * cs_u;
+    dy = (t ? by[1] : by[0]) - pj.y;
+    dz = (t ? bz[1] : bz[0]) - pj.z;
+  }
+};
+
 template <int F>
 __device__ inline void form_pair(const typename SphForm<F>::G4& pi, const typename SphForm<F>::G4& pj,
                                  float cs_u, typename SphForm<F>::real& dx, typename SphForm<F>::real& dy,
@@ -411,6 +427,15 @@ constexpr uint32_t kGroups = 5;  // {0, 8}, {1, 7}, {2, 6}, {3, 5}, {4}
 __device__ inline uint32_t sph_run(uint32_t k) {
   // k-th run of the order 0, 8, 1, 7, 2, 6, 3, 5, 4
   return (k & 1u) ? 8u - (k >> 1) : (k >> 1);
+}
+
+__device__ inline void ProbeRunBases::begin(const float4& pi, uint32_t grp, float cs_u) {
+#pragma unroll
+  for (uint32_t t = 0; t < 2u; ++t) {
+    const uint32_t r = sph_run(2u * grp + t);
+    by[t] = pi.y - (float)((int)(r % 3u) - 1) * cs_u;
+    bz[t] = pi.z - (float)((int)(r / 3u) - 1) * cs_u;
+  }
 }
 
 // Walk of a wavefront over its lanes' stored masks (one wavefront per block;
@@ -960,19 +985,9 @@ __global__ __launch_bounds__(kForceBlock, kForceWaves<F>) void k_sph_force(
     }
     const real alpha_c0 = (real)kc.alpha_c0, ach = alpha_c0 * (real)u.hv;
     Sum<real, F == kMixed> sx, sy, sz;
-    // mixed form: probe pair vector (o_i - o_j) - (c_j - c_i) cs/h with the
-    // y and z cell shifts of the group's two runs folded into per-run bases
-    // (by, bz) and only the x shift read from the cell codes
-    float by[2] = {0.0f, 0.0f}, bz[2] = {0.0f, 0.0f};
+    ProbeRunBases rb;   // mixed form: per-run bases of the pair vector
     auto begin_group = [&](uint32_t grp) {
-      if constexpr (F == kMixed) {
-#pragma unroll
-        for (uint32_t t = 0; t < 2u; ++t) {
-          const uint32_t r = sph_run(2u * grp + t);
-          by[t] = (float)pi.y - (float)((int)(r % 3u) - 1) * u.cs_u;
-          bz[t] = (float)pi.z - (float)((int)(r / 3u) - 1) * u.cs_u;
-        }
-      }
+      if constexpr (F == kMixed) rb.begin(pi, grp, u.cs_u);
     };
     // pair bodies over a batch of neighbours fetched by get(j, p, v, rho, pr)
     auto body = [&](const uint32_t* jj, const uint32_t* tt, auto&& get) {
@@ -984,13 +999,8 @@ __global__ __launch_bounds__(kForceBlock, kForceWaves<F>) void k_sph_force(
       for (int v = 0; v < kUnr; ++v) {
         if (jj[v] == kPad) continue;
         real dx, dy, dz, sc;
-        if constexpr (F == kMixed) {
-          dx = (pi.x - pj[v].x) - (float)cell_shift(__float_as_uint(pi.w), __float_as_uint(pj[v].w), 0) * u.cs_u;
-          dy = (tt[v] ? by[1] : by[0]) - pj[v].y;
-          dz = (tt[v] ? bz[1] : bz[0]) - pj[v].z;
-        } else {
-          form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
-        }
+        if constexpr (F == kMixed) rb.pair(pi, pj[v], tt[v], u.cs_u, dx, dy, dz);
+        else form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
         if (force_pair_sc<F>(dx, dy, dz, vi, rhoi, pri, vj[v], rhoj[v], prj[v], u, ach, sc)) {
           if constexpr (F == kMixed) {
             // compensated sums of the FP32 terms
