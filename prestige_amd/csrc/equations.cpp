@@ -889,7 +889,14 @@ GenProgram build_program(const int32_t* ids, uint32_t n, int32_t loop, bool f32_
     for (const Eq* e : jb) jcode += "      { " + emit_body(*e, pn, &consts) + "}\n";
     for (const Eq* e : ib) icode += "  { " + emit_body(*e, pn, &consts) + "}\n";
     GenPass gp;
+    // kernel name: the pass index and its equations (identifier characters only)
     gp.kernel = "prestige_pass_" + std::to_string(p);
+    for (const std::vector<const Eq*>* v : {&jb, &ib})
+      for (const Eq* e : *v) {
+        std::string nm;
+        for (char ch : e->name) nm += std::isalnum((unsigned char)ch) ? ch : '_';
+        if (gp.kernel.size() + nm.size() < 96) gp.kernel += "_" + nm;
+      }
     std::string params;
     for (auto& kv : fields) {
       const GenField& f = kv.second;

@@ -64,7 +64,7 @@ def test_schedule():
 
 def test_generated_text():
     cells = pa.generate_hip([pa.EQ1], pa.LOOP_CELLS)
-    assert 'extern "C" __global__ void __launch_bounds__(64, 8) prestige_pass_0(' in cells
+    assert 'extern "C" __global__ void __launch_bounds__(64, 8) prestige_pass_0_eq1(' in cells
     assert "force_i += g_mass[js]" in cells and "const double* __restrict__ g_mass" in cells
     assert "< c2" in cells and "lb[row" in cells and "s_q[kk] = q" in cells
     pairs = pa.generate_hip([pa.EQ1], pa.LOOP_ALLPAIRS)
