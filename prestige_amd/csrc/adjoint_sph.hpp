@@ -151,9 +151,9 @@ __global__ __launch_bounds__(kAdjBlock, kAdjForceWaves<F>) void k_adj_force(
   const G4* gpos = (const G4*)rec.gpos;
   const G4* gvel = (const G4*)rec.gvel;
   const R2* rp = (const R2*)rec.rp;
-  __shared__ G4 s_p[kForceCap], s_v[kForceCap];
+  __shared__ StageArr<G4, kForceCap> s_p, s_v;
   __shared__ R2 s_rp[kForceCap];
-  __shared__ C4 s_c[kForceCap];
+  __shared__ StageArr<C4, kForceCap> s_c;
   const uint32_t lane = threadIdx.x;
   for_each_tile(tq, [&](uint32_t tile) {
     const uint32_t i = tile * kAdjBlock + lane;
@@ -246,17 +246,17 @@ __global__ __launch_bounds__(kAdjBlock, kAdjForceWaves<F>) void k_adj_force(
             if constexpr (F == kMixed) pb_.begin(pi, grp, u.cs_u);
           },
           [&](uint32_t k, uint32_t j) {
-            s_p[k] = gpos[j];
-            s_v[k] = gvel[j];
+            s_p.put(k, gpos[j]);
+            s_v.put(k, gvel[j]);
             s_rp[k] = rp[j];
-            s_c[k] = cvec[j];
+            s_c.put(k, cvec[j]);
           },
           [&](const uint32_t* jj, const uint32_t* tt) {
             body(jj, tt, [&](uint32_t k, G4& p, G4& v, R2& r, C4& cc) {
-              p = s_p[k];
-              v = s_v[k];
+              p = s_p.get(k);
+              v = s_v.get(k);
               r = s_rp[k];
-              cc = s_c[k];
+              cc = s_c.get(k);
             });
           },
           [&](const uint32_t* jj, const uint32_t* tt) {
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(kAdjBlock, kAdjDensityWaves<F>) void k_adj_density(
   using G4 = typename SphForm<F>::G4;
   using ar = typename AdjForm<F>::ar;
   const G4* gpos = (const G4*)rec.gpos;
-  __shared__ G4 s_p[kForceCap];
+  __shared__ StageArr<G4, kForceCap> s_p;
   __shared__ ar s_rb[kForceCap];
   const uint32_t lane = threadIdx.x;
   for_each_tile(tq, [&](uint32_t tile) {
@@ -355,12 +355,12 @@ __global__ __launch_bounds__(kAdjBlock, kAdjDensityWaves<F>) void k_adj_density(
           if constexpr (F == kMixed) pb_.begin(pi, grp, u.cs_u);
         },
         [&](uint32_t k, uint32_t j) {
-          s_p[k] = gpos[j];
+          s_p.put(k, gpos[j]);
           s_rb[k] = rhob[j];
         },
         [&](const uint32_t* jj, const uint32_t* tt) {
           body(jj, tt, [&](uint32_t k, G4& p, ar& rb) {
-            p = s_p[k];
+            p = s_p.get(k);
             rb = s_rb[k];
           });
         },

@@ -906,6 +906,29 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
 // dependent global loads.  A union wider than kForceCap slots is walked from
 // global memory.
 // ---------------------------------------------------------------------------
+// LDS array of staged 4-vectors.  A double4 entry is kept as two 16-byte
+// halves in two arrays: lanes reading nearby entries of a run then touch
+// nearby 16-byte slots, which spreads a wavefront's reads over more LDS banks
+// than 32-byte entries do.  float4 entries are 16 bytes already.
+template <class T4, uint32_t N>
+struct StageArr {
+  T4 v[N];
+  __device__ void put(uint32_t k, const T4& x) { v[k] = x; }
+  __device__ T4 get(uint32_t k) const { return v[k]; }
+};
+template <uint32_t N>
+struct StageArr<double4, N> {
+  double2 lo[N], hi[N];
+  __device__ void put(uint32_t k, const double4& x) {
+    lo[k] = make_double2(x.x, x.y);
+    hi[k] = make_double2(x.z, x.w);
+  }
+  __device__ double4 get(uint32_t k) const {
+    const double2 a = lo[k], b = hi[k];
+    return make_double4(a.x, a.y, b.x, b.y);
+  }
+};
+
 constexpr int kForceBlock = 64;  // one wavefront: block barriers are wave barriers
 template <int F>
 constexpr int kForceWaves = F == kF64 ? 4 : 5;
@@ -961,7 +984,7 @@ __global__ __launch_bounds__(kForceBlock, kForceWaves<F>) void k_sph_force(
   using R2 = typename SphForm<F>::R2;
   using S4 = typename SphForm<F>::S4;
   using sreal = typename SphForm<F>::sreal;
-  __shared__ G4 s_p[kForceCap], s_v[kForceCap];
+  __shared__ StageArr<G4, kForceCap> s_p, s_v;
   __shared__ R2 s_rp[F == kMixed ? kForceCap : 1];
   const uint32_t lane = threadIdx.x;
   for_each_tile(tq, [&](uint32_t tile) {
@@ -1018,14 +1041,14 @@ __global__ __launch_bounds__(kForceBlock, kForceWaves<F>) void k_sph_force(
     walk_runs_staged<kUnr, kForceCap>(
         meta, words, n, i, live, lane, begin_group,
         [&](uint32_t k, uint32_t j) {
-          s_p[k] = gpos[j];
-          s_v[k] = gvel[j];
+          s_p.put(k, gpos[j]);
+          s_v.put(k, gvel[j]);
           if constexpr (F == kMixed) s_rp[k] = rp[j];
         },
         [&](const uint32_t* jj, const uint32_t* tt) {
           body(jj, tt, [&](uint32_t k, G4& p, G4& v, real& rho, real& pr) {
-            p = s_p[k];
-            v = s_v[k];
+            p = s_p.get(k);
+            v = s_v.get(k);
             if constexpr (F == kMixed) {
               const R2 t = s_rp[k];
               rho = t.x;
