@@ -562,17 +562,17 @@ struct DryWalls {
 constexpr int kDensBlock = 64;  // one wavefront: block barriers are wave barriers
 // Wavefronts per SIMD the register budget is held to: the tile loop lets the
 // compiler keep more values live, and these are the budgets that measured
-// fastest (FP64 / mixed: the occupancy before the loop; FP32: one wavefront
+// fastest (FP64: the occupancy before the loop; FP32 and mixed: one wavefront
 // fewer than that, which avoids spills).
 template <int F>
-constexpr int kDensWaves = F == kF32 ? 7 : 8;
+constexpr int kDensWaves = F == kF64 ? 8 : 7;
 constexpr uint32_t kStageCap = 256;   // scan offsets staged per run (4 KB)
 // Row runs per lockstep walk of the density pass: opposite rows together in
-// the FP64 form (1106 against 1147 us a pass at 4M); one at a time in the
-// FP32 form (no gain measured) and the mixed form, whose per-run probe
-// shifts would otherwise spill.
+// the FP64 form (1106 against 1147 us a pass at 4M) and the mixed form (892
+// against 941 us, at 7 wavefronts per SIMD); one at a time in the FP32 form
+// (no gain measured).
 template <int F>
-constexpr uint32_t kDensGroup = F == kF64 ? 2u : 1u;
+constexpr uint32_t kDensGroup = F == kF32 ? 1u : 2u;
 
 // Tait pressure p = max(B ((rho / rho0)^7 - 1), 0), one operation order for
 // every caller.
@@ -703,12 +703,6 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
     Sum<real, F == kMixed> acc;
     bool overflow = false;
     const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
-    float bxm = 0.0f, bx0 = 0.0f, bxp = 0.0f;
-    if constexpr (F == kMixed) {
-      bx0 = (float)pi.x;
-      bxm = bx0 + u.cs_u;
-      bxp = bx0 - u.cs_u;
-    }
     // Runs in the WCSPH order, scanned one at a time and walked G at a time
     // (kDensGroup): with G = 2 a run and its opposite row ({0, 8}, {1, 7},
     // {2, 6}, {3, 5}, {4}) as one stream of set bits, so a lane with few hits
@@ -719,15 +713,11 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
       const uint32_t nrun = G == 2u && grp + 1u < kGroups ? 2u : 1u;
       uint64_t m[2 * G];
       uint32_t gj0[G];
-      // mixed form: the run's candidates lie in row (dy, dz) of cells cx - 1,
-      // cx, cx + 1; slots [b0, b1) are cell cx, so a candidate's cell shift
-      // and with it the probe pair vector come from its slot
-      uint32_t gb0[G], gb1[G];
-      float gby[G], gbz[G];
+      float gby[G], gbz[G];   // mixed form: per-run bases of the pair vector
 #pragma unroll
       for (uint32_t t = 0; t < G; ++t) {
         m[2u * t] = m[2u * t + 1u] = 0u;
-        gj0[t] = gb0[t] = gb1[t] = 0u;
+        gj0[t] = 0u;
         gby[t] = gbz[t] = 0.0f;
       }
 #pragma unroll
@@ -741,18 +731,14 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
           const uint32_t row = ((uint32_t)zz * (uint32_t)ny + (uint32_t)yy) * (uint32_t)nx;
           j0 = lb[row + (uint32_t)cr.x0];
           len = lb[row + (uint32_t)cr.x1 + 1] - j0;
-          if constexpr (F == kMixed) {
-            gb0[t] = lb[row + (uint32_t)cr.cx];
-            gb1[t] = lb[row + (uint32_t)cr.cx + 1];
-          }
           if (len > 128u) {
             overflow = true;
             len = 128u;
           }
         }
         gj0[t] = j0;
-        // probe pair vector (o_i - o_j) - (c_j - c_i) cs/h as (base - o_j): the
-        // y and z shifts are the run's, x is -1, 0, +1 by slot
+        // probe pair vector (o_i - o_j) - (c_j - c_i) cs/h: the y and z shifts
+        // are the run's (folded into bases), x comes from the cell codes
         if constexpr (F == kMixed) {
           gby[t] = (float)pi.y - (float)((int)(r % 3u) - 1) * u.cs_u;
           gbz[t] = (float)pi.z - (float)((int)(r / 3u) - 1) * u.cs_u;
@@ -808,13 +794,19 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
       uint32_t wb[2 * G];
 #pragma unroll
       for (uint32_t t = 0; t < G; ++t) {
-        wb[2u * t] = gj0[t];
-        wb[2u * t + 1u] = gj0[t] + 64u;
+        // mixed form: bit 31 names the run of the group (slots stay below 2^31)
+        const uint32_t tag = F == kMixed ? t << 31 : 0u;
+        wb[2u * t] = gj0[t] | tag;
+        wb[2u * t + 1u] = (gj0[t] + 64u) | tag;
       }
       walk_bits_q<kDensUnr>(m, wb, [&](const uint32_t* jj) {
-        uint32_t js[kDensUnr];
+        uint32_t js[kDensUnr], tv[kDensUnr];
 #pragma unroll
-        for (int v = 0; v < kDensUnr; ++v) js[v] = jj[v] == kPad ? jj[0] : jj[v];
+        for (int v = 0; v < kDensUnr; ++v) {
+          const uint32_t q = jj[v] == kPad ? jj[0] : jj[v];
+          tv[v] = F == kMixed ? q >> 31 : 0u;
+          js[v] = F == kMixed ? q & 0x7FFFFFFFu : q;
+        }
         G4 pj[kDensUnr];
 #pragma unroll
         for (int v = 0; v < kDensUnr; ++v) pj[v] = pos[js[v]];
@@ -823,10 +815,9 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
           if (jj[v] == kPad) continue;
           real dx, dy, dz;
           if constexpr (F == kMixed) {
-            static_assert(F != kMixed || G == 1u, "the mixed form walks one run at a time");
-            dx = (js[v] < gb0[0] ? bxm : (js[v] < gb1[0] ? bx0 : bxp)) - pj[v].x;
-            dy = gby[0] - pj[v].y;
-            dz = gbz[0] - pj[v].z;
+            dx = (pi.x - pj[v].x) - (float)cell_shift(__float_as_uint(pi.w), __float_as_uint(pj[v].w), 0) * u.cs_u;
+            dy = (tv[v] ? gby[G - 1u] : gby[0]) - pj[v].y;
+            dz = (tv[v] ? gbz[G - 1u] : gbz[0]) - pj[v].z;
           } else {
             form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
           }
