@@ -562,17 +562,16 @@ struct DryWalls {
 constexpr int kDensBlock = 64;  // one wavefront: block barriers are wave barriers
 // Wavefronts per SIMD the register budget is held to: the tile loop lets the
 // compiler keep more values live, and these are the budgets that measured
-// fastest (FP64: the occupancy before the loop; FP32 and mixed: one wavefront
-// fewer than that, which avoids spills).
+// fastest (density: 7 in every form, one fewer than the occupancy before the
+// loop, which avoids spills).
 template <int F>
-constexpr int kDensWaves = F == kF64 ? 8 : 7;
+constexpr int kDensWaves = 7;
 constexpr uint32_t kStageCap = 256;   // scan offsets staged per run (4 KB)
-// Row runs per lockstep walk of the density pass: opposite rows together in
-// the FP64 form (1106 against 1147 us a pass at 4M) and the mixed form (892
-// against 941 us, at 7 wavefronts per SIMD); one at a time in the FP32 form
-// (no gain measured).
+// Row runs per lockstep walk of the density pass: a run and its opposite row
+// together (at 4M: FP64 1147 -> 1106 us a pass, mixed 941 -> 892, FP32 904 ->
+// 844).
 template <int F>
-constexpr uint32_t kDensGroup = F == kF32 ? 1u : 2u;
+constexpr uint32_t kDensGroup = 2u;
 
 // Tait pressure p = max(B ((rho / rho0)^7 - 1), 0), one operation order for
 // every caller.
