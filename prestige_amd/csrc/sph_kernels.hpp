@@ -520,8 +520,9 @@ __device__ inline void walk_runs_staged(const uint2* __restrict__ meta, const ui
 // ---------------------------------------------------------------------------
 // density + EOS: rho_i = m sigma sum_j f(|x_ij|/h) over j with |x_ij| < 2h
 // (j == i included), p_i = max(B((rho_i/rho0)^7 - 1), 0), pr_i = p_i / rho_i^2.
-// Per row run: probe scan -> run mask (stored for the force pass) -> walk of
-// the mask's set bits in the form's precision.
+// Per row run: probe scan -> run mask (stored for the force pass); per run
+// and its opposite row: one walk of the two masks' set bits in the form's
+// precision.
 //
 // One wavefront per block.  For each row run the wavefront first copies the
 // union of its lanes' candidate ranges into LDS as FP32 scan offsets (one
@@ -567,11 +568,6 @@ constexpr int kDensBlock = 64;  // one wavefront: block barriers are wave barrie
 template <int F>
 constexpr int kDensWaves = 7;
 constexpr uint32_t kStageCap = 256;   // scan offsets staged per run (4 KB)
-// Row runs per lockstep walk of the density pass: a run and its opposite row
-// together (at 4M: FP64 1147 -> 1106 us a pass, mixed 941 -> 892, FP32 904 ->
-// 844).
-template <int F>
-constexpr uint32_t kDensGroup = 2u;
 
 // Tait pressure p = max(B ((rho / rho0)^7 - 1), 0), one operation order for
 // every caller.
@@ -702,14 +698,15 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
     Sum<real, F == kMixed> acc;
     bool overflow = false;
     const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
-    // Runs in the WCSPH order, scanned one at a time and walked G at a time
-    // (kDensGroup): with G = 2 a run and its opposite row ({0, 8}, {1, 7},
-    // {2, 6}, {3, 5}, {4}) as one stream of set bits, so a lane with few hits
-    // in one row and many in the other idles less of the lockstep walk.  The
-    // order of the pairs stays the scan order.
-    constexpr uint32_t G = kDensGroup<F>;
-    for (uint32_t grp = 0; grp < (G == 2u ? kGroups : kRuns); ++grp) {
-      const uint32_t nrun = G == 2u && grp + 1u < kGroups ? 2u : 1u;
+    // Runs in the WCSPH order, scanned one at a time and walked a group at a
+    // time: a run and its opposite row ({0, 8}, {1, 7}, {2, 6}, {3, 5}, {4})
+    // as one stream of set bits, so a lane with few hits in one row and many
+    // in the other idles less of the lockstep walk (at 4M: FP64 1147 -> 1106
+    // us a pass, mixed 941 -> 892, FP32 904 -> 844).  The order of the pairs
+    // stays the scan order.
+    constexpr uint32_t G = 2u;   // runs per group
+    for (uint32_t grp = 0; grp < kGroups; ++grp) {
+      const uint32_t nrun = grp + 1u < kGroups ? 2u : 1u;
       uint64_t m[2 * G];
       uint32_t gj0[G];
       float gby[G], gbz[G];   // mixed form: per-run bases of the pair vector
@@ -815,8 +812,8 @@ __global__ __launch_bounds__(kDensBlock, kDensWaves<F>) void k_sph_density(
           real dx, dy, dz;
           if constexpr (F == kMixed) {
             dx = (pi.x - pj[v].x) - (float)cell_shift(__float_as_uint(pi.w), __float_as_uint(pj[v].w), 0) * u.cs_u;
-            dy = (tv[v] ? gby[G - 1u] : gby[0]) - pj[v].y;
-            dz = (tv[v] ? gbz[G - 1u] : gbz[0]) - pj[v].z;
+            dy = (tv[v] ? gby[1] : gby[0]) - pj[v].y;
+            dz = (tv[v] ? gbz[1] : gbz[0]) - pj[v].z;
           } else {
             form_pair<F>(pi, pj[v], u.cs_u, dx, dy, dz);
           }
