@@ -893,47 +893,49 @@ void gather_from_ids(Ctx& c, const double* d_in, void* dst, int dst_kind, int co
 // All pairs: thread t is particle id t, j runs over ids 0..n-1 in id order,
 // exactly the generated loop.  force and mass are held in id order.
 // The sum is one dependent FP64 add per pair (about 7 cycles each).  The
-// masses pass through LDS a block-wide tile at a time: the next tile is loaded
-// from global memory while the current one is summed, and a full tile is read
-// eight masses ahead of the adds.
+// masses pass through LDS a block-wide tile at a time, in two alternating
+// buffers (one barrier per tile): the next tile is loaded from global memory
+// while the current one is summed, and a full tile is read 32 masses ahead of
+// the adds, enough to cover the LDS latency at one wavefront per SIMD.
 __global__ __launch_bounds__(kBlock) void k_eq1_allpairs(uint32_t n,
                                                          const double* __restrict__ mass,
                                                          double* __restrict__ force) {
-  __shared__ double2 tile[kBlock / 2];
+  __shared__ double2 tiles[2][kBlock / 2];
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc = i < n ? force[i] : 0.0;
-  double* tile1 = reinterpret_cast<double*>(tile);
   double next = threadIdx.x < n ? mass[threadIdx.x] : 0.0;
-  for (uint32_t j0 = 0; j0 < n; j0 += kBlock) {
+  uint32_t buf = 0;
+  for (uint32_t j0 = 0; j0 < n; j0 += kBlock, buf ^= 1u) {
+    double2* tile = tiles[buf];
+    double* tile1 = reinterpret_cast<double*>(tile);
     tile1[threadIdx.x] = next;
     __syncthreads();
     const uint32_t jn = j0 + kBlock + threadIdx.x;
     next = jn < n ? mass[jn] : 0.0;
     uint32_t m = n - j0 < (uint32_t)kBlock ? n - j0 : (uint32_t)kBlock;
     if (m == (uint32_t)kBlock) {
-      double2 cur[4], nxt[4];
+      double2 cur[16], nxt[16];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) cur[u] = tile[u];
-      for (uint32_t t = 4; t < (uint32_t)kBlock / 2; t += 4) {
+      for (int u = 0; u < 16; ++u) cur[u] = tile[u];
+      for (uint32_t t = 16; t < (uint32_t)kBlock / 2; t += 16) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) nxt[u] = tile[t + u];
+        for (int u = 0; u < 16; ++u) nxt[u] = tile[t + u];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 16; ++u) {
           acc += cur[u].x;
           acc += cur[u].y;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) cur[u] = nxt[u];
+        for (int u = 0; u < 16; ++u) cur[u] = nxt[u];
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < 16; ++u) {
         acc += cur[u].x;
         acc += cur[u].y;
       }
     } else {
       for (uint32_t t = 0; t < m; ++t) acc += tile1[t];
     }
-    __syncthreads();
   }
   if (i < n) force[i] = acc;
 }
@@ -944,13 +946,12 @@ __global__ __launch_bounds__(kBlock) void k_eq1_allpairs(uint32_t n,
 // One wavefront per block (64 consecutive slots).  Per row run the wavefront
 // copies the union of its lanes' candidate ranges into LDS as FP32 offsets
 // from its first slot in units of the cutoff (|q|^2 in .w), with their masses;
-// each lane scans its
-// own range with the dot form q.w - 2 s.q < t, whose threshold carries the
-// rounding margin of the FP32 form (so the scan keeps a superset of the exact
-// neighbours), then walks the set bits in ascending slot order and applies the
-// exact FP64 test: the pairs and the order of the plain loop, so each force is
-// the plain loop's sum bit for bit.  A union wider than the stage is scanned
-// from global memory.
+// each lane scans its own range with the dot form q.w - 2 s.q < t, whose
+// threshold carries the rounding margin of the FP32 form (so the scan keeps a
+// superset of the exact neighbours), then walks the set bits in ascending slot
+// order and applies the exact FP64 test: the pairs and the order of the plain
+// loop, so each force is the plain loop's sum bit for bit.  A union wider than
+// the stage is scanned from global memory.
 constexpr uint32_t kEq1Stage = 256;
 
 template <class T4>
