@@ -130,12 +130,13 @@ __device__ inline void block_sum2(double a, double b, double2* __restrict__ part
 // (dalpha, dc0).  One wavefront per block; each row run's neighbour data are
 // staged in LDS as in the forward force pass.
 constexpr int kAdjBlock = 64;
-// Wavefronts per SIMD the register budgets are held to (each kernel's
-// occupancy before the tile loop).
+// Wavefronts per SIMD the register budgets are held to: each kernel's
+// occupancy before the tile loop, but for the mixed density adjoint, whose
+// measured best is one fewer (256 against 263 us at 1M; 8: 273).
 template <int F>
 constexpr int kAdjForceWaves = F == kF64 ? 2 : 4;
 template <int F>
-constexpr int kAdjDensityWaves = F == kF64 ? 5 : 7;
+constexpr int kAdjDensityWaves = F == kF64 ? 5 : 6;
 
 template <int F>
 __global__ __launch_bounds__(kAdjBlock, kAdjForceWaves<F>) void k_adj_force(
