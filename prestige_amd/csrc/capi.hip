@@ -544,6 +544,9 @@ prestige_status prestige_create(const prestige_desc* desc, prestige_ctx** out) {
     if (desc->precision < 0 || desc->precision > PRESTIGE_MIXED)
       throw Error(PRESTIGE_E_INVALID, "unknown precision");
     if (desc->n_particles >= 0xFFFFFFFFull) throw Error(PRESTIGE_E_INVALID, "n_particles must be < 2^32 - 1");
+    // the mixed WCSPH density walk tags slots with their run in bit 31
+    if (desc->model == PRESTIGE_MODEL_WCSPH && desc->precision == PRESTIGE_MIXED && desc->n_particles >= 0x80000000ull)
+      throw Error(PRESTIGE_E_INVALID, "n_particles must be < 2^31 in the MIXED WCSPH form");
     if (desc->flags & ~(PRESTIGE_FLAG_DENSITY_F64 | PRESTIGE_FLAG_DEM_EVENTS))
       throw Error(PRESTIGE_E_INVALID, "unknown bits in flags");
     if ((desc->flags & PRESTIGE_FLAG_DEM_EVENTS) && desc->model != PRESTIGE_MODEL_DEM)

@@ -278,6 +278,19 @@ def test_create_rejects_bad_descriptors(field, value):
     assert ei.value.name == "E_INVALID"
 
 
+def test_create_rejects_slot_counts_beyond_the_index_width():
+    # slots are 32-bit indices with one pad value; the mixed WCSPH density walk
+    # also keeps a run tag in bit 31
+    for prec, n_bad in ((pa.F64, 0xFFFFFFFF), (pa.MIXED, 0x80000000)):
+        d = _small_desc()
+        d.precision = prec
+        d.n_particles = n_bad
+        d.n_moving = 0
+        with pytest.raises(pa.PrestigeError) as ei:
+            pa.Context(d)
+        assert ei.value.name == "E_INVALID"
+
+
 def test_create_rejects_cell_smaller_than_support():
     d = _small_desc()
     d.cell_size = 1.9 * d.h
