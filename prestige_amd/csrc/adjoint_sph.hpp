@@ -137,6 +137,12 @@ template <int F>
 constexpr int kAdjForceWaves = F == kF64 ? 2 : 4;
 template <int F>
 constexpr int kAdjDensityWaves = F == kF64 ? 5 : 6;
+// Mask words loaded one run group ahead (walk_runs_staged): in the mixed
+// reverse passes (force 380 -> 374, density 256 -> 252 us at 1M); the FP64
+// ones, at 2 and 5 wavefronts per SIMD, run slower with it (759 -> 795 and
+// 302 -> 309 us).
+template <int F>
+constexpr bool kAdjPrefetch = F == kMixed;
 
 template <int F>
 __global__ __launch_bounds__(kAdjBlock, kAdjForceWaves<F>) void k_adj_force(
@@ -241,7 +247,7 @@ __global__ __launch_bounds__(kAdjBlock, kAdjForceWaves<F>) void k_adj_force(
           }
         }
       };
-      walk_runs_staged<kUnr, kForceCap>(
+      walk_runs_staged<kUnr, kForceCap, kAdjPrefetch<F>>(
           rec.meta, rec.words, n, i, live, lane,
           [&](uint32_t grp) {
             if constexpr (F == kMixed) pb_.begin(pi, grp, u.cs_u);
@@ -350,7 +356,7 @@ __global__ __launch_bounds__(kAdjBlock, kAdjDensityWaves<F>) void k_adj_density(
         qz += s * (ar)dzr;
       }
     };
-    walk_runs_staged<kUnr, kForceCap>(
+    walk_runs_staged<kUnr, kForceCap, kAdjPrefetch<F>>(
         rec.meta, rec.words, n, i, live, lane,
         [&](uint32_t grp) {
           if constexpr (F == kMixed) pb_.begin(pi, grp, u.cs_u);

@@ -449,19 +449,52 @@ __device__ inline void ProbeRunBases::begin(const float4& pi, uint32_t grp, floa
 // walked run by run, and a run whose union alone exceeds CAP with glob(jj,
 // tt) (jj: slots).  Lanes visit their neighbours in group order and scan
 // order either way; end_run() follows each group.
-template <int B, uint32_t CAP, class Begin, class Stage, class Lds, class Glob, class End>
+template <int B, uint32_t CAP, bool PF = false, class Begin, class Stage, class Lds, class Glob, class End>
 __device__ inline void walk_runs_staged(const uint2* __restrict__ meta, const uint32_t* __restrict__ words,
                                         uint32_t n, uint32_t i, bool live, uint32_t lane, Begin&& begin,
                                         Stage&& stage, Lds&& lds, Glob&& glob, End&& end_run) {
+  // PF: the next group's row metadata and all four mask words of each of its
+  // runs are loaded one group ahead (the words masked by the run's length on
+  // use), so their latency overlaps the current group's walk; 12 registers.
+  uint2 pm[2] = {make_uint2(0u, 0u), make_uint2(0u, 0u)};
+  uint32_t pw[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+  auto prefetch = [&](uint32_t g) {
+#pragma unroll
+    for (uint32_t t = 0; t < 2u; ++t) {
+      const uint32_t k = 2u * g + t;
+      pm[t] = make_uint2(0u, 0u);
+      if (live && k < kRuns) {
+        const uint32_t r = sph_run(k);
+        pm[t] = meta[meta_index(r, n, i)];
+#pragma unroll
+        for (uint32_t w = 0; w < 4u; ++w) pw[4u * t + w] = words[word_index(r, w, n, i)];
+      }
+    }
+  };
+  if constexpr (PF) prefetch(0u);
   for (uint32_t grp = 0; grp < kGroups; ++grp) {
     begin(grp);
     const uint32_t nrun = grp + 1u == kGroups ? 1u : 2u;
     uint64_t m[4] = {0u, 0u, 0u, 0u};
     uint32_t j0[2] = {0u, 0u}, base[2] = {0u, 0u}, span[2] = {0u, 0u};
+    if constexpr (PF) {
+#pragma unroll
+      for (uint32_t t = 0; t < 2u; ++t) {
+        if (t >= nrun) break;
+        j0[t] = pm[t].x;
+        uint32_t wd[4];
+#pragma unroll
+        for (uint32_t w = 0; w < 4u; ++w) wd[w] = pm[t].y > 32u * w ? pw[4u * t + w] : 0u;
+        m[2u * t] = (uint64_t)wd[0] | ((uint64_t)wd[1] << 32);
+        m[2u * t + 1u] = (uint64_t)wd[2] | ((uint64_t)wd[3] << 32);
+      }
+      if (grp + 1u < kGroups) prefetch(grp + 1u);
+    }
 #pragma unroll
     for (uint32_t t = 0; t < 2u; ++t) {
       if (t >= nrun) break;
-      if (live) load_run(meta, words, sph_run(2u * grp + t), n, i, j0[t], m[2u * t], m[2u * t + 1u]);
+      if constexpr (!PF)
+        if (live) load_run(meta, words, sph_run(2u * grp + t), n, i, j0[t], m[2u * t], m[2u * t + 1u]);
       const uint64_t lo = m[2u * t], hi = m[2u * t + 1u];
       uint32_t a = 0xFFFFFFFFu, b = 0u;
       if (lo | hi) {
@@ -920,6 +953,12 @@ constexpr int kForceBlock = 64;  // one wavefront: block barriers are wave barri
 template <int F>
 constexpr int kForceWaves = F == kF64 ? 4 : 5;
 constexpr uint32_t kForceCap = 160;
+// Mask words loaded one run group ahead in the force walk: in the FP64 form
+// (1152 -> 1110 us at 4M); the mixed form, at 5 wavefronts per SIMD, has no
+// registers to spare for them (896 -> 951 us), and the FP32 form gains
+// nothing (745 -> 752 us).
+template <int F>
+constexpr bool kForcePrefetch = F == kF64;
 
 // One momentum pair: sc with a_i -= (m sigma / h) sc x_ij for 0 < r < 2h;
 // false outside (then nothing is added).  sc = (pr_i + pr_j + Pi_ij) f'(q) / r
@@ -1025,7 +1064,7 @@ __global__ __launch_bounds__(kForceBlock, kForceWaves<F>) void k_sph_force(
         }
       }
     };
-    walk_runs_staged<kUnr, kForceCap>(
+    walk_runs_staged<kUnr, kForceCap, kForcePrefetch<F>>(
         meta, words, n, i, live, lane, begin_group,
         [&](uint32_t k, uint32_t j) {
           s_p.put(k, gpos[j]);
