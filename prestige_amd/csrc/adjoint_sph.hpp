@@ -138,7 +138,7 @@ constexpr int kAdjForceWaves = F == kF64 ? 2 : 4;
 template <int F>
 constexpr int kAdjDensityWaves = F == kF64 ? 5 : 6;
 // Mask words loaded one run group ahead (walk_runs_staged): in the mixed
-// reverse passes (force 380 -> 374, density 256 -> 252 us at 1M); the FP64
+// reverse passes (force 380 -> 363, density 256 -> 247 us at 1M); the FP64
 // ones, at 2 and 5 wavefronts per SIMD, run slower with it (759 -> 795 and
 // 302 -> 309 us).
 template <int F>

@@ -453,11 +453,12 @@ template <int B, uint32_t CAP, bool PF = false, class Begin, class Stage, class 
 __device__ inline void walk_runs_staged(const uint2* __restrict__ meta, const uint32_t* __restrict__ words,
                                         uint32_t n, uint32_t i, bool live, uint32_t lane, Begin&& begin,
                                         Stage&& stage, Lds&& lds, Glob&& glob, End&& end_run) {
-  // PF: the next group's row metadata and all four mask words of each of its
-  // runs are loaded one group ahead (the words masked by the run's length on
-  // use), so their latency overlaps the current group's walk; 12 registers.
+  // PF: the next group's row metadata and the first two mask words of each of
+  // its runs (candidates 0..63, nearly every run) are loaded one group ahead,
+  // masked by the run's length on use, so their latency overlaps the current
+  // group's walk (8 registers); words 2 and 3 are read when a run needs them.
   uint2 pm[2] = {make_uint2(0u, 0u), make_uint2(0u, 0u)};
-  uint32_t pw[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+  uint32_t pw[4] = {0u, 0u, 0u, 0u};
   auto prefetch = [&](uint32_t g) {
 #pragma unroll
     for (uint32_t t = 0; t < 2u; ++t) {
@@ -467,7 +468,7 @@ __device__ inline void walk_runs_staged(const uint2* __restrict__ meta, const ui
         const uint32_t r = sph_run(k);
         pm[t] = meta[meta_index(r, n, i)];
 #pragma unroll
-        for (uint32_t w = 0; w < 4u; ++w) pw[4u * t + w] = words[word_index(r, w, n, i)];
+        for (uint32_t w = 0; w < 2u; ++w) pw[2u * t + w] = words[word_index(r, w, n, i)];
       }
     }
   };
@@ -483,8 +484,11 @@ __device__ inline void walk_runs_staged(const uint2* __restrict__ meta, const ui
         if (t >= nrun) break;
         j0[t] = pm[t].x;
         uint32_t wd[4];
-#pragma unroll
-        for (uint32_t w = 0; w < 4u; ++w) wd[w] = pm[t].y > 32u * w ? pw[4u * t + w] : 0u;
+        wd[0] = pm[t].y > 0u ? pw[2u * t] : 0u;
+        wd[1] = pm[t].y > 32u ? pw[2u * t + 1u] : 0u;
+        const uint32_t r = sph_run(2u * grp + t);
+        wd[2] = pm[t].y > 64u ? words[word_index(r, 2u, n, i)] : 0u;
+        wd[3] = pm[t].y > 96u ? words[word_index(r, 3u, n, i)] : 0u;
         m[2u * t] = (uint64_t)wd[0] | ((uint64_t)wd[1] << 32);
         m[2u * t + 1u] = (uint64_t)wd[2] | ((uint64_t)wd[3] << 32);
       }
@@ -954,8 +958,8 @@ template <int F>
 constexpr int kForceWaves = F == kF64 ? 4 : 5;
 constexpr uint32_t kForceCap = 160;
 // Mask words loaded one run group ahead in the force walk: in the FP64 form
-// (1152 -> 1110 us at 4M); the mixed form, at 5 wavefronts per SIMD, has no
-// registers to spare for them (896 -> 951 us), and the FP32 form gains
+// (1152 -> 1098 us at 4M); the mixed form, at 5 wavefronts per SIMD, has no
+// registers to spare for them (888 -> 951 us), and the FP32 form gains
 // nothing (745 -> 752 us).
 template <int F>
 constexpr bool kForcePrefetch = F == kF64;
