@@ -93,10 +93,11 @@ __global__ __launch_bounds__(kBlock) void k_dem_force(
   real Fx = 0, Fy = 0, Fz = 0, Tx = 0, Ty = 0, Tz = 0;
   uint32_t nc = 0, nwall = 0, nslip = 0;
   const uint32_t ocnt = hist.cnt_in[me];
-  for_each_run(cr, g, lb, [&](uint32_t j0, uint32_t j1, int ddy, int ddz) {
-    for (uint32_t j = j0; j < j1; ++j) {
-      if (j == i) continue;
-      const G4 pj = gpos[j];
+  // Candidates of a row run in slot order; their positions are loaded four at
+  // a time ahead of the tests, so the loads of a batch overlap (a lane has
+  // few contacts, and its time is the candidate loads' latency).
+  auto contact = [&](uint32_t j, const G4& pj, const G4& wjb, int ddy, int ddz) {
+    {
       real dx, dy, dz;
       if constexpr (F == kMixed) {
         dx = (pi.x - pj.x) - (float)cell_shift(__float_as_uint(pi.w), __float_as_uint(pj.w), 0) * cs_u;
@@ -106,10 +107,11 @@ __global__ __launch_bounds__(kBlock) void k_dem_force(
         dx = pi.x - pj.x; dy = pi.y - pj.y; dz = pi.z - pj.z;
       }
       real r2 = dx * dx + dy * dy + dz * dz;
-      const G4 wj = gom[j];
-      const real rj = (F == kF64) ? (real)pj.w : (real)wj.w;
+      // radius: FP64 form in pos.w, mixed form in om.w (loaded with the batch)
+      const real rj = (F == kF64) ? (real)pj.w : (real)wjb.w;
       real R = ri + rj;
-      if (!(r2 < R * R && r2 > real(0.0))) continue;
+      if (!(r2 < R * R && r2 > real(0.0))) return;
+      const G4 wj = (F == kF64) ? gom[j] : wjb;
       const G4 vj = gvel[j];
       real d = sqrt(r2);
       real invd = real(1.0) / d;
@@ -150,6 +152,24 @@ __global__ __launch_bounds__(kBlock) void k_dem_force(
         xi_out[(size_t)(nc * 3 + 2) * n + me] = xn[2];
       }
       ++nc;
+    }
+  };
+  for_each_run(cr, g, lb, [&](uint32_t j0, uint32_t j1, int ddy, int ddz) {
+    for (uint32_t jb = j0; jb < j1; jb += 4u) {
+      G4 pjs[4], wjs[4];
+#pragma unroll
+      for (uint32_t u = 0; u < 4u; ++u) {
+        const uint32_t jc = jb + u < j1 ? jb + u : j1 - 1u;
+        pjs[u] = gpos[jc];
+        if constexpr (F == kMixed) wjs[u] = gom[jc];
+        else wjs[u] = pjs[u];   // unused: the FP64 radius is pos.w
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < 4u; ++u) {
+        const uint32_t j = jb + u;
+        if (j >= j1) break;
+        if (j != i) contact(j, pjs[u], wjs[u], ddy, ddz);
+      }
     }
   });
   // walls: bit 2a = lower wall of axis a (normal +e_a), bit 2a+1 = upper (-e_a)
