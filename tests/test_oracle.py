@@ -222,6 +222,22 @@ def test_free_fall_discrete_closed_form():
     assert abs(st.get("z")[0] - (z0 + n * 0.7 * dt - g * dt * dt * n * (n + 1) / 2)) < 1e-12
 
 
+def test_lone_particle_density_and_tait_pressure():
+    """A lone particle's only pair is itself (q = 0, f(0) = 1): rho = m sigma.
+    With m = 2 rho0 pi h^3, rho = 2 rho0 and p = B (2^7 - 1), B = rho0 c0^2 / 7:
+    a missing self pair gives rho = 0, a lost 1/7 in B or a wrong exponent a
+    different p."""
+    d, arrs = make_scene(pa.SCENE_DAM_BREAK, n_target=1000, dx=0.01)
+    d.mass = 2.0 * d.rho0 * math.pi * d.h ** 3
+    d.n_particles = d.n_moving = 1
+    mid = [0.5 * (d.grid_lo[a] + d.grid_hi[a]) for a in range(3)]
+    one = {k: np.array([v]) for k, v in zip(("x", "y", "z", "vx", "vy", "vz"), mid + [0.0, 0.0, 0.0])}
+    st = oracle_state(d, one)
+    assert st.step(1) == 0
+    assert st.get("rho")[0] == pytest.approx(2.0 * d.rho0, rel=1e-14)
+    assert st.get("p")[0] == pytest.approx(127.0 * d.rho0 * d.c0 ** 2 / 7.0, rel=1e-13)
+
+
 # ---------------------------------------------------------------------------
 # DEM closed forms
 # ---------------------------------------------------------------------------
