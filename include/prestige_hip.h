@@ -97,7 +97,8 @@ typedef struct prestige_desc {
   int32_t model;              /* prestige_model */
   int32_t precision;          /* prestige_precision */
   uint32_t max_contacts;      /* DEM history slots per particle (<= 32) */
-  uint64_t n_particles;       /* total particles (< 2^32 - 1; < 2^31 in the MIXED WCSPH form) */
+  uint64_t n_particles;       /* total particles (< 2^32 - 1; < 2^31 in the MIXED WCSPH form); 0 is
+                                 valid: steps and gathers then do nothing and fields are empty */
   uint64_t n_moving;          /* ids [0, n_moving) are integrated; [n_moving, n) stay fixed */
   double grid_lo[3];          /* cell-grid origin; particles must stay in [lo, hi) */
   double grid_hi[3];
